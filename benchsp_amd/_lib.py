@@ -1,0 +1,108 @@
+"""Loads libbsp.so (the HIP library) and declares its C-ABI for ctypes.
+
+There is no fallback: if the library is missing or fails to load, importing
+the package's compute modules raises, naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbsp.so")
+
+vp = C.c_void_p
+i32, i64, u64, f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
+i32p, i64p, f64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+
+
+class BspError(RuntimeError):
+    """A libbsp call returned a non-zero bsp_status."""
+
+    def __init__(self, fn: str, status: int, msg: str):
+        super().__init__(f"{fn} failed: status {status} ({msg})")
+        self.status = status
+
+
+class CsrDesc(C.Structure):
+    """struct bsp_csr (include/bsp.h)."""
+    _fields_ = [("nrows", i64), ("ncols", i64), ("nnz", i64), ("rowptr", vp), ("colidx", vp),
+                ("val", vp), ("dtype", C.c_int)]
+
+
+# name -> (restype, argtypes); restype None means bsp_status (checked).
+_SIGS: dict[str, tuple] = {
+    "bsp_status_string": (C.c_char_p, [C.c_int]),
+    "bsp_last_error": (C.c_char_p, []),
+    "bsp_version": (C.c_char_p, []),
+    "bsp_csr_plan_create": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(vp)]),
+    "bsp_csr_spmv": (None, [vp, vp, vp, vp]),
+    "bsp_csr_plan_info": (None, [vp, i64p, i64p]),
+    "bsp_plan_destroy": (None, [vp]),
+    "bsp_gen_uniform": (None, [i64, u64, vp]),
+    "bsp_gen_normal": (None, [i64, u64, vp]),
+    "bsp_gen_stencil5": (None, [i64, i64, C.POINTER(vp)]),
+    "bsp_gen_stencil3d": (None, [i64, i64, i64, C.c_int, u64, C.POINTER(vp)]),
+    "bsp_gen_powerlaw": (None, [i64, i64, f64, f64, i64, u64, C.POINTER(vp)]),
+    "bsp_gen_blocksparse": (None, [i64, i64, C.c_int, i64, u64, C.POINTER(vp)]),
+    "bsp_gen_graph": (None, [i64, f64, f64, C.c_int, u64, C.POINTER(vp)]),
+    "bsp_host_csr_info": (None, [vp, i64p, i64p, i64p]),
+    "bsp_host_csr_copy": (None, [vp, vp, vp, vp]),
+    "bsp_host_csr_free": (C.c_int, [vp]),
+    "bsp_spmv_fp64_warpper": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, C.c_int, f64p]),
+    "bsp_spmv_fp64_balance_warpper": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, C.c_int, f64p]),
+}
+
+_lib = None
+
+
+class _Checked:
+    """Wraps a bsp_status-returning function: raises BspError on failure."""
+
+    def __init__(self, lib, name, fn):
+        self._lib, self._name, self._fn = lib, name, fn
+
+    def __call__(self, *args):
+        st = self._fn(*args)
+        if st != 0:
+            msg = self._lib.bsp_last_error().decode(errors="replace")
+            raise BspError(self._name, st, msg)
+        return st
+
+
+class _Lib:
+    def __init__(self, cdll):
+        self.cdll = cdll
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(cdll, name)
+            fn.argtypes = args
+            fn.restype = C.c_int if res is None else res
+            setattr(self, name, _Checked(cdll, name, fn) if res is None else fn)
+
+    def declare(self, name, restype, argtypes):
+        """Declares an extra symbol (restype None = checked bsp_status)."""
+        fn = getattr(self.cdll, name)
+        fn.argtypes = argtypes
+        fn.restype = C.c_int if restype is None else restype
+        setattr(self, name, _Checked(self.cdll, name, fn) if restype is None else fn)
+        return getattr(self, name)
+
+
+def lib() -> _Lib:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m benchsp_amd.build` "
+                              "(or __graft_entry__.build())")
+        # torch first, so its HIP runtime is the one libbsp binds to.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        _lib = _Lib(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    """Every bsp_* name this module declares."""
+    return sorted(_SIGS)

@@ -1,0 +1,106 @@
+"""GPU parity of the CSR SpMV kernels against the CPU oracle (src/ref.cpp:26-37
+restated in oracle/oracle.c), through the C-ABI."""
+import numpy as np
+import pytest
+
+from tests._parity import abs_spmv, assert_spmv_close
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["adaptive", "scalar", "balanced", "split"]
+
+
+def _run(A, x, method):
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    dA = DeviceCsr.from_host(A)
+    op = CsrSpmv(dA, method)
+    xd = torch.from_numpy(x).cuda()
+    y = op(xd)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def _check(oracle, A, x, method, what):
+    print(f"[spmv] {what} method={method} n={A.nrows} nnz={A.nnz}", flush=True)
+    y = _run(A, x, method)
+    y_ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
+    assert_spmv_close(y, y_ref, abs_spmv(oracle, A, x), what=f"{what}/{method}")
+    return y, y_ref
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_poisson5_256(oracle, bsp_host, method):
+    A = bsp_host.poisson5_256()
+    x = bsp_host.uniform(A.ncols, seed=0)
+    y, y_ref = _check(oracle, A, x, method, "5pt-256")
+    if method == "scalar":
+        # same products, same left-to-right order, no contraction: bit-exact
+        assert np.array_equal(y, y_ref)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_poisson5_ones_closed_form(bsp_host, method):
+    A = bsp_host.poisson5_256()
+    y = _run(A, np.ones(A.ncols), method)
+    n = 256
+    expect = np.zeros((n, n))
+    expect[0, :] += 1
+    expect[-1, :] += 1
+    expect[:, 0] += 1
+    expect[:, -1] += 1
+    assert np.array_equal(y, expect.ravel())
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_stencil27_64(oracle, bsp_host, method):
+    A = bsp_host.stencil27_64()
+    x = bsp_host.uniform(A.ncols, seed=11)
+    _check(oracle, A, x, method, "27pt-64")
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_powerlaw_1m(oracle, bsp_host, method):
+    A = bsp_host.powerlaw_1m()
+    x = bsp_host.uniform(A.ncols, seed=12)
+    _check(oracle, A, x, method, "powerlaw-1M")
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_edge_shapes(oracle, bsp_host, method):
+    from benchsp_amd.host import Csr
+    rng = np.random.default_rng(7)
+    # empty rows, a single huge row (> one workgroup chunk), rectangular shape
+    lens = np.array([0, 3, 0, 0, 5000, 1, 129, 128, 0, 2047, 2048, 2049, 17] + [0] * 40 + [1] * 3000)
+    n, ncols = lens.size, 9000
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(ncols, L, replace=False)) for L in lens]).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1])
+    A = Csr(n, ncols, rp, col, val)
+    x = rng.uniform(-1, 1, ncols)
+    _check(oracle, A, x, method, "edge")
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_all_empty_rows(method):
+    from benchsp_amd.host import Csr
+    A = Csr(1000, 10, np.zeros(1001, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    y = _run(A, np.ones(10), method)
+    assert np.array_equal(y, np.zeros(1000))
+
+
+def test_reference_surface_wrappers(oracle, bsp_host):
+    import ctypes as C
+    from benchsp_amd._lib import lib
+    A = bsp_host.stencil3d(12, 13, 14, 27, seed=3)
+    x = bsp_host.uniform(A.ncols, seed=4)
+    rp = A.rowptr.astype(np.int32)
+    y_ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
+    for fn in ("bsp_spmv_fp64_warpper", "bsp_spmv_fp64_balance_warpper"):
+        out = np.full(A.nrows, 123.0)
+        t = C.c_double(-1)
+        rc = getattr(lib(), fn)(rp.ctypes.data, A.col.ctypes.data, A.nrows, A.ncols, A.nnz,
+                                A.val.ctypes.data, x.ctypes.data, out.ctypes.data, 3, C.byref(t))
+        assert rc == 0 and t.value >= 0
+        assert_spmv_close(out, y_ref, abs_spmv(oracle, A, x), what=fn)
