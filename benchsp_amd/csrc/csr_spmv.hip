@@ -93,13 +93,30 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     return;
   }
 
+  // Issue every index/value load of the block before the first dependent x
+  // gather, so a workgroup pays two memory round trips, not one per
+  // iteration. Iterations past the block end re-read its last entry (same
+  // cache line for the whole wave) instead of branching per lane.
+  if (nnz > 0) {
+    const int iters = (nnz + kWG - 1) / kWG;
+    const int last = p1 - 1;
+    int ci[kLoadIters];
+    double vv[kLoadIters], xv[kLoadIters];
 #pragma unroll
-  for (int k = 0; k < kLoadIters; ++k) {
-    const int i = tid + k * kWG;
-    if (i < nnz) {
-      const int c = ld_stream(&col[p0 + i]);
-      const double v = ld_stream(&val[p0 + i]);
-      prod[i] = v * x[c];
+    for (int k = 0; k < kLoadIters; ++k) {
+      if (k < iters) {
+        const int j = min(p0 + tid + k * kWG, last);
+        ci[k] = ld_stream(&col[j]);
+        vv[k] = ld_stream(&val[j]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kLoadIters; ++k)
+      if (k < iters) xv[k] = x[ci[k]];
+#pragma unroll
+    for (int k = 0; k < kLoadIters; ++k) {
+      const int i = tid + k * kWG;
+      if (k < iters && i < nnz) prod[i] = vv[k] * xv[k];
     }
   }
   for (int i = tid; i <= nrows; i += kWG) rptr[i] = rowptr[r0 + i] - p0;

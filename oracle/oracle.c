@@ -308,6 +308,10 @@ typedef struct {
   uint16_t* map;     /* nblocks */
   int32_t warpnum;
   int32_t* prefix; /* warpnum+1 */
+  /* 1 when the reference's second pass would index past the warpcount
+   * entries its first pass allocated (dbsr.h:135-155, undefined behaviour
+   * there); the oracle then extends the warp list and warpnum counts it. */
+  int32_t overflow;
 } orc_dbsr;
 
 static int cmp_len_idx(const void* a, const void* b) {
@@ -381,7 +385,9 @@ orc_dbsr* orc_dbsr_build(int64_t nrows, int64_t ncols, const int64_t* rowptr, co
     }
   }
   if (now) warpcount++;
-  int32_t* cnt = (int32_t*)calloc((size_t)warpcount + 1, sizeof(int32_t));
+  int64_t cap = warpcount + 2;
+  for (int64_t i = 0; i < d->brows; i++) cap += (d->blcptr[i + 1] - d->blcptr[i] + C - 1) / C + 1;
+  int32_t* cnt = (int32_t*)calloc((size_t)cap, sizeof(int32_t));
   int now_warp = 0;
   for (int64_t i = 0; i < d->brows; i++) {
     int len = d->blcptr[i + 1] - d->blcptr[i];
@@ -401,6 +407,8 @@ orc_dbsr* orc_dbsr_build(int64_t nrows, int64_t ncols, const int64_t* rowptr, co
       }
     }
   }
+  d->overflow = now_warp >= warpcount && d->brows > 0;
+  if (d->overflow) warpcount = now_warp + 1;
   d->warpnum = warpcount;
   d->prefix = (int32_t*)calloc((size_t)warpcount + 1, sizeof(int32_t));
   for (int w = 0; w < warpcount; w++) d->prefix[w + 1] = d->prefix[w] + cnt[w];

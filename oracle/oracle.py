@@ -75,7 +75,7 @@ class _Dbsr(C.Structure):
                 ("recorder", C.POINTER(C.c_int32)), ("blcptr", C.POINTER(C.c_int32)),
                 ("colidx", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double)),
                 ("map", C.POINTER(C.c_uint16)), ("warpnum", C.c_int32),
-                ("prefix", C.POINTER(C.c_int32))]
+                ("prefix", C.POINTER(C.c_int32)), ("overflow", C.c_int32)]
 
 
 _DASP_INTS = ["row_long", "row_block", "row_zero", "short_row_1", "short_row_2", "short_row_3",
@@ -197,7 +197,7 @@ def dbsr(nrows, ncols, rowptr, col, val):
                colidx=_arr(d.colidx, nb * 16, np.int32),
                val=_arr(d.val, nb * 16, np.float64),
                map=_arr(d.map, nb, np.uint16),
-               warpnum=d.warpnum,
+               warpnum=d.warpnum, overflow=bool(d.overflow),
                prefix=_arr(d.prefix, d.warpnum + 1, np.int32))
     lib().orc_dbsr_free(p)
     return out
