@@ -24,18 +24,69 @@ class BspError(RuntimeError):
         self.status = status
 
 
+class CsrOptions(C.Structure):
+    """struct bsp_csr_options (include/bsp.h)."""
+    _fields_ = [("seg_cols", i64), ("reserved", i64 * 7)]
+
+
 class CsrDesc(C.Structure):
     """struct bsp_csr (include/bsp.h)."""
     _fields_ = [("nrows", i64), ("ncols", i64), ("nnz", i64), ("rowptr", vp), ("colidx", vp),
                 ("val", vp), ("dtype", C.c_int)]
 
 
+class BsrView(C.Structure):
+    """struct bsp_bsr_view."""
+    _fields_ = [("nrows", i64), ("ncols", i64), ("nnz", i64), ("brows", i64), ("bcols", i64),
+                ("nblocks", i64), ("tm", i32), ("tk", i32), ("rowptr", vp), ("colidx", vp),
+                ("val", vp), ("map", vp), ("warpnum", i32), ("rowptr_by_warp", vp),
+                ("rowidx_by_warp", vp), ("avg_nnz", f64), ("stand", f64)]
+
+
+class DbsrView(C.Structure):
+    """struct bsp_dbsr_view."""
+    _fields_ = [("nrows", i64), ("ncols", i64), ("nnz", i64), ("brows", i64), ("nblocks", i64),
+                ("recorder", vp), ("blcptr", vp), ("colidx", vp), ("val", vp), ("map", vp),
+                ("warpnum", i32), ("prefix", vp), ("overflow", i32)]
+
+
+_DASP_I32 = ["row_long", "row_block", "row_zero", "short_row_1", "short_row_2", "short_row_3",
+             "short_row_4", "common_13", "fill0_nnz_short13", "fill0_nnz_short34",
+             "fill0_nnz_short22", "fill0_nnz_short", "threadblock13", "threadblock34",
+             "threadblock22", "blocknum", "fill0_nnz_reg", "nnz_irreg", "warp_number",
+             "fill0_nnz_long", "nnz_long"]
+
+
+class DaspView(C.Structure):
+    """struct bsp_dasp_view."""
+    _fields_ = ([("nrows", i64), ("ncols", i64), ("nnz", i64)] + [(k, i32) for k in _DASP_I32] +
+                [(k, vp) for k in ("order_rid", "short_val", "short_cid", "long_rpt", "long_val",
+                                   "long_cid", "rid_by_warp", "block_ptr", "irreg_rpt", "reg_val",
+                                   "reg_cid", "irreg_val", "irreg_cid")])
+
+
 # name -> (restype, argtypes); restype None means bsp_status (checked).
 _SIGS: dict[str, tuple] = {
+    "bsp_dasp_build": (None, [C.POINTER(CsrDesc), f64, i32, vp, C.POINTER(vp)]),
+    "bsp_dasp_view_get": (None, [vp, C.POINTER(DaspView)]),
+    "bsp_dasp_spmv": (None, [vp, vp, vp, C.c_int, vp]),
+    "bsp_dasp_destroy": (None, [vp]),
+    "bsp_copy_to_host": (None, [vp, vp, C.c_size_t]),
+    "bsp_bsr_build": (None, [C.POINTER(CsrDesc), C.c_int, C.c_int, vp, C.POINTER(vp)]),
+    "bsp_bsr_view_get": (None, [vp, C.POINTER(BsrView)]),
+    "bsp_bsr_spmv": (None, [vp, C.c_int, vp, vp, vp]),
+    "bsp_bsr_destroy": (None, [vp]),
+    "bsp_dbsr_build": (None, [C.POINTER(CsrDesc), vp, C.POINTER(vp)]),
+    "bsp_dbsr_view_get": (None, [vp, C.POINTER(DbsrView)]),
+    "bsp_dbsr_spmv": (None, [vp, vp, vp, C.c_int, vp]),
+    "bsp_dbsr_destroy": (None, [vp]),
     "bsp_status_string": (C.c_char_p, [C.c_int]),
     "bsp_last_error": (C.c_char_p, []),
     "bsp_version": (C.c_char_p, []),
     "bsp_csr_plan_create": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(vp)]),
+    "bsp_csr_plan_create_ex": (None, [C.POINTER(CsrDesc), C.c_int, C.c_void_p, vp,
+                                       C.POINTER(vp)]),
+    "bsp_csr_plan_segments": (None, [vp, i64p, i64p]),
     "bsp_csr_spmv": (None, [vp, vp, vp, vp]),
     "bsp_csr_plan_info": (None, [vp, i64p, i64p]),
     "bsp_plan_destroy": (None, [vp]),

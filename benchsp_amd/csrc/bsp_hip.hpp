@@ -69,6 +69,9 @@ bsp_status exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStr
 bsp_status sort_pairs_i32(const int32_t* keys_in, int32_t* keys_out, const int32_t* vals_in,
                           int32_t* vals_out, int64_t n, bool descending, int end_bit,
                           hipStream_t s);
+// Stable ascending sort of (int64 key, int32 value) pairs on bits [0, end_bit).
+bsp_status sort_pairs_i64key(const int64_t* keys_in, int64_t* keys_out, const int32_t* vals_in,
+                             int32_t* vals_out, int64_t n, int end_bit, hipStream_t s);
 // Sum of n int32 into *out (device).
 bsp_status reduce_sum_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s);
 

@@ -1,0 +1,602 @@
+// Block CSR (tm x tk) and DBSR: GPU builders and FP64 SpMV kernels.
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "bsp_hip.hpp"
+
+struct bsp_bsr_s {
+  bsp_bsr_view v{};
+  bsp::DevBuf<int32_t> rowptr, colidx, rpw, riw;
+  bsp::DevBuf<double> val;
+  bsp::DevBuf<uint16_t> map;
+};
+
+struct bsp_dbsr_s {
+  bsp_dbsr_view v{};
+  bsp::DevBuf<int32_t> recorder, blcptr, colidx, prefix;
+  bsp::DevBuf<double> val;
+  bsp::DevBuf<uint16_t> map;
+};
+
+namespace bsp {
+namespace {
+
+constexpr int kTB = 256;
+
+__global__ void row_ids(const int32_t* __restrict__ rowptr, int64_t nrows, int32_t* __restrict__ rid) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) rid[j] = (int32_t)r;
+}
+
+__global__ void bsr_keys(const int32_t* __restrict__ rid, const int32_t* __restrict__ col,
+                         int64_t nnz, int tm, int tk, int64_t bcols, int64_t* __restrict__ key,
+                         int32_t* __restrict__ idx) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  key[j] = (int64_t)(rid[j] / tm) * bcols + col[j] / tk;
+  idx[j] = (int32_t)j;
+}
+
+__global__ void key_flags(const int64_t* __restrict__ skey, int64_t n, int32_t* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (i == 0 || skey[i] != skey[i - 1]) ? 1 : 0;
+}
+
+__global__ void bsr_blocks(const int64_t* __restrict__ skey, const int32_t* __restrict__ flag,
+                           const int32_t* __restrict__ pos, int64_t n, int64_t bcols,
+                           int32_t* __restrict__ colidx, int32_t* __restrict__ bstart,
+                           int32_t* __restrict__ rowcnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const int b = pos[i];
+  colidx[b] = (int32_t)(skey[i] % bcols);
+  bstart[b] = (int32_t)i;
+  atomicAdd(&rowcnt[skey[i] / bcols], 1);
+}
+
+__global__ void bsr_values(const int32_t* __restrict__ sidx, const int32_t* __restrict__ flag,
+                           const int32_t* __restrict__ pos, int64_t n,
+                           const int32_t* __restrict__ rid, const int32_t* __restrict__ col,
+                           const double* __restrict__ val, int tm, int tk,
+                           double* __restrict__ bval) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = sidx[i];
+  const int64_t b = pos[i] + flag[i] - 1;
+  bval[b * tm * tk + (rid[j] % tm) * tk + col[j] % tk] = val[j];
+}
+
+__global__ void bsr_map4(const int32_t* __restrict__ bstart, int64_t nblocks, int64_t n,
+                         const int32_t* __restrict__ sidx, const int32_t* __restrict__ rid,
+                         const int32_t* __restrict__ col, uint16_t* __restrict__ map) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nblocks) return;
+  if (b == nblocks) {
+    map[b] = 0;
+    return;
+  }
+  const int e = b + 1 < nblocks ? bstart[b + 1] : (int)n;
+  unsigned bits = 0;
+  for (int i = bstart[b]; i < e; ++i) {
+    const int j = sidx[i];
+    bits |= 1u << ((rid[j] % 4) * 4 + col[j] % 4);
+  }
+  map[b] = (uint16_t)bits;
+}
+
+__global__ void warp_counts(const int32_t* __restrict__ rowptr, int64_t brows, int cap,
+                            int32_t* __restrict__ cnt) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < brows) cnt[r] = (rowptr[r + 1] - rowptr[r] + cap - 1) / cap;
+  if (r == brows) cnt[r] = 0;
+}
+
+__global__ void warp_rows(const int32_t* __restrict__ rpw, int64_t brows, int32_t* __restrict__ riw) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= brows) return;
+  for (int w = rpw[r]; w < rpw[r + 1]; ++w) riw[w] = (int32_t)r;
+}
+
+// sum over block rows of (len - avg)^2, fixed assignment and tree order.
+__global__ void row_len_sq_dev(const int32_t* __restrict__ rowptr, int64_t brows, double avg,
+                               double* __restrict__ out) {
+  __shared__ double red[kTB];
+  double s = 0.0;
+  for (int64_t r = threadIdx.x; r < brows; r += kTB) {
+    const double d = (double)(rowptr[r + 1] - rowptr[r]) - avg;
+    s += d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kTB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---- SpMV kernels -----------------------------------------------------------
+
+// Wave per block row; lane = slot*tm + r (slots = 64/tm blocks per step).
+// Partials of equal r are added in slot order through LDS.
+__global__ __launch_bounds__(kTB) void bsr_valu_f64(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
+    int64_t brows, int64_t nrows, int64_t ncols, int tm, int tk) {
+  __shared__ double part[kTB];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t br = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int slots = kWave / tm;
+  const int slot = lane / tm, r = lane % tm;
+  double acc = 0.0;
+  if (br < brows && slot < slots) {
+    for (int k = rowptr[br] + slot, e = rowptr[br + 1]; k < e; k += slots) {
+      const int64_t c0 = (int64_t)colidx[k] * tk;
+      const double* p = bval + (int64_t)k * tm * tk + r * tk;
+      const int cmax = (int)min<int64_t>(tk, ncols - c0);
+      for (int c = 0; c < cmax; ++c) acc += ld_stream(&p[c]) * x[c0 + c];
+    }
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (br < brows && lane < tm) {
+    const int base = threadIdx.x - lane;
+    double s = 0.0;
+    for (int q = 0; q < slots; ++q) s += part[base + q * tm + lane];
+    const int64_t row = br * tm + lane;
+    if (row < nrows) y[row] = s;
+  }
+}
+
+// 4x4 blocks on v_mfma_f64_16x16x4f64. Operand layouts (wave64, measured on
+// gfx950 with power-of-two encoded operands):
+//   A (16x4):  lane l holds A[l%16][l/16]
+//   B (4x16):  lane l holds B[l/16][l%16]
+//   C (16x16): lane l holds C[l/16 + 4v][l%16] in acc[v], v = 0..3.
+// Rows 4a..4a+3 of A are block a of the current group of four blocks of the
+// block row; column j of B is the x segment of block j/4. The diagonal 4x4
+// tile a then holds A_a*x_a in every column: its row r sits in acc[a] of
+// lanes 16r + 4a + t. Lane 16r + 4a (t = 0) contributes acc[a]; the four
+// lanes 16r + {0, 4, 8, 12} add up to y row r of the block row.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
+    int64_t brows, int64_t nrows, int64_t ncols) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t br = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (br >= brows) return;  // whole wave exits together
+  const int ai = lane & 15;       // A row / B column
+  const int kk = lane >> 4;       // k index (column within a block)
+  const int a = ai >> 2, r = ai & 3;
+  const int s0 = rowptr[br], e0 = rowptr[br + 1];
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = s0; k0 < e0; k0 += 4) {
+    const int kb = k0 + a;
+    double av = 0.0, bv = 0.0;
+    if (kb < e0) {
+      av = ld_stream(&bval[(int64_t)kb * 16 + r * 4 + kk]);
+      const int64_t c = (int64_t)colidx[kb] * 4 + kk;
+      bv = c < ncols ? x[c] : 0.0;
+    }
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  // lane 16r + 4a + t keeps acc[a]; sum over a (lane bits 2..3)
+  double t = a == 0 ? acc[0] : a == 1 ? acc[1] : a == 2 ? acc[2] : acc[3];
+  t += __shfl_xor(t, 4, 64);
+  t += __shfl_xor(t, 8, 64);
+  if ((lane & 15) == 0) {
+    const int64_t row = br * 4 + (lane >> 4);
+    if (row < nrows) y[row] = t;
+  }
+}
+
+// ---- DBSR -------------------------------------------------------------------
+
+__global__ void row_lengths(const int32_t* __restrict__ rowptr, int64_t nrows,
+                            int32_t* __restrict__ len, int32_t* __restrict__ idx) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  len[r] = rowptr[r + 1] - rowptr[r];
+  idx[r] = (int32_t)r;
+}
+
+__global__ void dbsr_widths(const int32_t* __restrict__ slen, int64_t nrows, int64_t brows,
+                            int32_t* __restrict__ w) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > brows) return;
+  if (g == brows) {
+    w[g] = 0;
+    return;
+  }
+  int m = 0;
+  for (int64_t i = 4 * g; i < 4 * g + 4 && i < nrows; ++i) m = max(m, (slen[i] + 3) / 4);
+  w[g] = m;
+}
+
+__global__ void dbsr_place(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ val, const int32_t* __restrict__ rec,
+                           const int32_t* __restrict__ blcptr, int64_t nrows,
+                           int32_t* __restrict__ ocol, double* __restrict__ oval) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const int o = rec[i];
+  const int64_t b0 = blcptr[i / 4];
+  const int rr = (int)(i % 4) * 4;
+  for (int j = rowptr[o], z = 0, e = rowptr[o + 1]; j < e; ++j, ++z) {
+    const int64_t d = (b0 + z / 4) * 16 + rr + (z % 4);
+    ocol[d] = col[j];
+    oval[d] = val[j];
+  }
+}
+
+__global__ void dbsr_map(const int32_t* __restrict__ slen, const int32_t* __restrict__ blcptr,
+                         int64_t nrows, int64_t brows, uint16_t* __restrict__ map) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= brows) return;
+  for (int b = blcptr[g]; b < blcptr[g + 1]; ++b) {
+    const int z4 = 4 * (b - blcptr[g]);
+    unsigned bits = 0;
+    for (int q = 0; q < 4; ++q) {
+      const int64_t i = 4 * g + q;
+      if (i >= nrows) break;
+      const int cnt = min(4, max(0, slen[i] - z4));
+      bits |= ((1u << cnt) - 1u) << (4 * q);
+    }
+    map[b] = (uint16_t)bits;
+  }
+}
+
+// balanceDbsr (include/dbsr.h:94-169) with WARP_CAPACITY 64, both passes as
+// written. Mode 0 counts (out[0] = pass-1 warps, out[1] = pass-2 warps);
+// mode 1 fills per-warp row counts into cnt.
+__global__ void dbsr_balance(const int32_t* __restrict__ blcptr, int64_t brows, int mode,
+                             int32_t* __restrict__ out, int32_t* __restrict__ cnt) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int C = 64;
+  int now = 0, warpcount = 0;
+  for (int64_t i = 0; i < brows; ++i) {
+    const int len = blcptr[i + 1] - blcptr[i];
+    if (now + len <= C) {
+      now += len;
+    } else {
+      if (now != 0) {
+        warpcount++;
+        now = 0;
+      }
+      if (len <= C)
+        now = len;
+      else
+        warpcount += (len + C - 1) / C;
+    }
+  }
+  if (now) warpcount++;
+  int now_warp = 0;
+  for (int64_t i = 0; i < brows; ++i) {
+    const int len = blcptr[i + 1] - blcptr[i];
+    if (now + len <= C) {
+      if (mode) cnt[now_warp]++;
+      now += len;
+    } else {
+      now = 0;
+      if (len <= C) {
+        now_warp++;
+        if (mode) cnt[now_warp]++;
+      } else {
+        for (int j = 1; j <= (len + C - 1) / C; ++j) {
+          now_warp++;
+          if (mode) cnt[now_warp]++;
+        }
+      }
+    }
+  }
+  if (!mode) {
+    out[0] = warpcount;
+    out[1] = brows > 0 ? now_warp + 1 : 0;
+  }
+}
+
+__global__ void serial_scan_i32(const int32_t* __restrict__ cnt, int n, int32_t* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int s = 0;
+  out[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    s += cnt[i];
+    out[i + 1] = s;
+  }
+}
+
+// Wave per 4-row group; lane = slot*4 + r, 16 blocks per step; each lane
+// gathers x through the per-slot column indices.
+__global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
+    const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
+    double* __restrict__ y, int64_t brows, int64_t nrows, int sorted_order) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (g >= brows) return;
+  const int slot = lane >> 2, r = lane & 3;
+  double acc = 0.0;
+  for (int k = blcptr[g] + slot, e = blcptr[g + 1]; k < e; k += 16) {
+    const int64_t o = (int64_t)k * 16 + r * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc += ld_stream(&bval[o + c]) * x[ld_stream(&colidx[o + c])];
+  }
+  acc += __shfl_xor(acc, 4, 64);
+  acc += __shfl_xor(acc, 8, 64);
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  if (lane < 4) {
+    const int64_t i = 4 * g + lane;
+    if (i < nrows) y[sorted_order ? i : rec[i]] = acc;
+  }
+}
+
+}  // namespace
+
+// ---- builders -----------------------------------------------------------------
+
+static int bits_for(int64_t v) {
+  int b = 1;
+  while (b < 63 && (int64_t(1) << b) <= v) ++b;
+  return b;
+}
+
+static bsp_status build_bsr(const bsp_csr& A, int tm, int tk, hipStream_t s, bsp_bsr_s* h) {
+  const int64_t n = A.nrows, nnz = A.nnz;
+  bsp_bsr_view& v = h->v;
+  v.nrows = n;
+  v.ncols = A.ncols;
+  v.nnz = nnz;
+  v.tm = tm;
+  v.tk = tk;
+  v.brows = cdiv(n, tm);
+  v.bcols = cdiv(A.ncols, tk);
+  const auto* aval = static_cast<const double*>(A.val);
+  DevBuf<int32_t> rid, idx, sidx, flag, pos, bstart;
+  DevBuf<int64_t> key, skey;
+  BSP_TRY(rid.alloc(nnz));
+  BSP_TRY(idx.alloc(nnz));
+  BSP_TRY(sidx.alloc(nnz));
+  BSP_TRY(flag.alloc(nnz));
+  BSP_TRY(pos.alloc(nnz));
+  BSP_TRY(key.alloc(nnz));
+  BSP_TRY(skey.alloc(nnz));
+  int64_t nblocks = 0;
+  if (nnz > 0) {
+    row_ids<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, n, rid.get());
+    bsr_keys<<<cdiv(nnz, kTB), kTB, 0, s>>>(rid.get(), A.colidx, nnz, tm, tk, v.bcols, key.get(),
+                                            idx.get());
+    BSP_HIP(hipGetLastError());
+    BSP_TRY(sort_pairs_i64key(key.get(), skey.get(), idx.get(), sidx.get(), nnz,
+                              bits_for(v.brows * v.bcols), s));
+    key_flags<<<cdiv(nnz, kTB), kTB, 0, s>>>(skey.get(), nnz, flag.get());
+    BSP_HIP(hipGetLastError());
+    BSP_TRY(exclusive_scan_i32(flag.get(), pos.get(), nnz, s));
+    int32_t lp = 0, lf = 0;
+    BSP_TRY(fetch(pos.get() + nnz - 1, &lp, s));
+    BSP_TRY(fetch(flag.get() + nnz - 1, &lf, s));
+    nblocks = (int64_t)lp + lf;
+  }
+  v.nblocks = nblocks;
+  DevBuf<int32_t> rowcnt;
+  BSP_TRY(rowcnt.alloc(v.brows + 1));
+  BSP_TRY(h->rowptr.alloc(v.brows + 1));
+  BSP_TRY(h->colidx.alloc(nblocks));
+  BSP_TRY(h->val.alloc(nblocks * tm * tk));
+  BSP_TRY(bstart.alloc(nblocks + 1));
+  BSP_HIP(hipMemsetAsync(rowcnt.get(), 0, sizeof(int32_t) * (v.brows + 1), s));
+  BSP_HIP(hipMemsetAsync(h->val.get(), 0, sizeof(double) * (nblocks ? nblocks * tm * tk : 1), s));
+  if (nnz > 0) {
+    bsr_blocks<<<cdiv(nnz, kTB), kTB, 0, s>>>(skey.get(), flag.get(), pos.get(), nnz, v.bcols,
+                                              h->colidx.get(), bstart.get(), rowcnt.get());
+    bsr_values<<<cdiv(nnz, kTB), kTB, 0, s>>>(sidx.get(), flag.get(), pos.get(), nnz, rid.get(),
+                                              A.colidx, aval, tm, tk, h->val.get());
+    BSP_HIP(hipGetLastError());
+  }
+  BSP_TRY(exclusive_scan_i32(rowcnt.get(), h->rowptr.get(), v.brows + 1, s));
+  v.rowptr = h->rowptr.get();
+  v.colidx = h->colidx.get();
+  v.val = h->val.get();
+  v.map = nullptr;
+  v.avg_nnz = nblocks ? (double)nnz / (double)nblocks : 0.0;
+  if (tm == 4 && tk == 4) {
+    BSP_TRY(h->map.alloc(nblocks + 1));
+    bsr_map4<<<cdiv(nblocks + 1, kTB), kTB, 0, s>>>(bstart.get(), nblocks, nnz, sidx.get(),
+                                                    rid.get(), A.colidx, h->map.get());
+    BSP_HIP(hipGetLastError());
+    v.map = h->map.get();
+    DevBuf<int32_t> wc;
+    BSP_TRY(wc.alloc(v.brows + 1));
+    BSP_TRY(h->rpw.alloc(v.brows + 1));
+    warp_counts<<<cdiv(v.brows + 1, kTB), kTB, 0, s>>>(h->rowptr.get(), v.brows, 64, wc.get());
+    BSP_HIP(hipGetLastError());
+    BSP_TRY(exclusive_scan_i32(wc.get(), h->rpw.get(), v.brows + 1, s));
+    int32_t wn = 0;
+    BSP_TRY(fetch(h->rpw.get() + v.brows, &wn, s));
+    v.warpnum = wn;
+    BSP_TRY(h->riw.alloc(wn));
+    warp_rows<<<cdiv(v.brows, kTB), kTB, 0, s>>>(h->rpw.get(), v.brows, h->riw.get());
+    BSP_HIP(hipGetLastError());
+    v.rowptr_by_warp = h->rpw.get();
+    v.rowidx_by_warp = h->riw.get();
+    DevBuf<double> sq;
+    BSP_TRY(sq.alloc(1));
+    const double avg_len = v.brows ? (double)nblocks / (double)v.brows : 0.0;
+    row_len_sq_dev<<<1, kTB, 0, s>>>(h->rowptr.get(), v.brows, avg_len, sq.get());
+    BSP_HIP(hipGetLastError());
+    double acc = 0.0;
+    BSP_TRY(fetch(sq.get(), &acc, s));
+    v.stand = v.brows ? std::sqrt(acc / (double)v.brows) : 0.0;
+  }
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
+static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
+  const int64_t n = A.nrows;
+  bsp_dbsr_view& v = h->v;
+  v.nrows = n;
+  v.ncols = A.ncols;
+  v.nnz = A.nnz;
+  v.brows = cdiv(n, 4);
+  DevBuf<int32_t> len, idx, slen, w;
+  BSP_TRY(len.alloc(n));
+  BSP_TRY(idx.alloc(n));
+  BSP_TRY(slen.alloc(n));
+  BSP_TRY(h->recorder.alloc(n));
+  if (n > 0) {
+    row_lengths<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, n, len.get(), idx.get());
+    BSP_HIP(hipGetLastError());
+    // stable radix sort by length keeps equal lengths in index order: the
+    // (length, index) order of dbsr.h:213-216
+    BSP_TRY(sort_pairs_i32(len.get(), slen.get(), idx.get(), h->recorder.get(), n, false, 32, s));
+  }
+  BSP_TRY(w.alloc(v.brows + 1));
+  BSP_TRY(h->blcptr.alloc(v.brows + 1));
+  dbsr_widths<<<cdiv(v.brows + 1, kTB), kTB, 0, s>>>(slen.get(), n, v.brows, w.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(exclusive_scan_i32(w.get(), h->blcptr.get(), v.brows + 1, s));
+  int32_t nb = 0;
+  BSP_TRY(fetch(h->blcptr.get() + v.brows, &nb, s));
+  v.nblocks = nb;
+  BSP_TRY(h->colidx.alloc((int64_t)nb * 16));
+  BSP_TRY(h->val.alloc((int64_t)nb * 16));
+  BSP_TRY(h->map.alloc(nb));
+  BSP_HIP(hipMemsetAsync(h->colidx.get(), 0, sizeof(int32_t) * (nb ? nb * 16 : 1), s));
+  BSP_HIP(hipMemsetAsync(h->val.get(), 0, sizeof(double) * (nb ? nb * 16 : 1), s));
+  if (n > 0) {
+    dbsr_place<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, static_cast<const double*>(A.val),
+                                            h->recorder.get(), h->blcptr.get(), n, h->colidx.get(),
+                                            h->val.get());
+    dbsr_map<<<cdiv(v.brows, kTB), kTB, 0, s>>>(slen.get(), h->blcptr.get(), n, v.brows,
+                                                h->map.get());
+    BSP_HIP(hipGetLastError());
+  }
+  DevBuf<int32_t> counts;
+  BSP_TRY(counts.alloc(2));
+  dbsr_balance<<<1, 1, 0, s>>>(h->blcptr.get(), v.brows, 0, counts.get(), nullptr);
+  BSP_HIP(hipGetLastError());
+  int32_t c[2] = {0, 0};
+  BSP_HIP(hipMemcpyAsync(c, counts.get(), sizeof(c), hipMemcpyDeviceToHost, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  v.overflow = c[1] > c[0] ? 1 : 0;
+  const int32_t warps = c[1] > c[0] ? c[1] : c[0];
+  v.warpnum = warps;
+  DevBuf<int32_t> cnt;
+  BSP_TRY(cnt.alloc(warps + 1));
+  BSP_TRY(h->prefix.alloc(warps + 1));
+  BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * (warps + 1), s));
+  dbsr_balance<<<1, 1, 0, s>>>(h->blcptr.get(), v.brows, 1, nullptr, cnt.get());
+  serial_scan_i32<<<1, 1, 0, s>>>(cnt.get(), warps, h->prefix.get());
+  BSP_HIP(hipGetLastError());
+  BSP_HIP(hipStreamSynchronize(s));
+  v.recorder = h->recorder.get();
+  v.blcptr = h->blcptr.get();
+  v.colidx = h->colidx.get();
+  v.val = h->val.get();
+  v.map = h->map.get();
+  v.prefix = h->prefix.get();
+  return BSP_OK;
+}
+
+}  // namespace bsp
+
+using namespace bsp;
+
+namespace {
+bsp_status check_csr_f64(const bsp_csr* A) {
+  BSP_REQUIRE(A, "null CSR descriptor");
+  BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
+  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
+      A->ncols >= (int64_t(1) << 31))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
+  BSP_REQUIRE(A->nnz == 0 || (A->colidx && A->val), "null colidx/val");
+  BSP_REQUIRE(A->dtype == BSP_F64, "FP64 values required");
+  return BSP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+bsp_status bsp_bsr_build(const bsp_csr* A, int tm, int tk, bsp_stream s, bsp_bsr* out) {
+  BSP_TRY(check_csr_f64(A));
+  BSP_REQUIRE(out, "null output handle");
+  BSP_REQUIRE(tm >= 1 && tk >= 1 && tm <= 64 && tk <= 64, "block size must be in 1..64");
+  auto h = std::make_unique<bsp_bsr_s>();
+  BSP_TRY(build_bsr(*A, tm, tk, (hipStream_t)s, h.get()));
+  if (h->v.nblocks * (int64_t)tm * tk >= (int64_t(1) << 40))
+    return fail(BSP_ERR_TOO_LARGE, "BSR too large");
+  *out = h.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_bsr_view_get(bsp_bsr h, bsp_bsr_view* v) {
+  BSP_REQUIRE(h && v, "null handle");
+  *v = h->v;
+  return BSP_OK;
+}
+
+bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_stream s) {
+  BSP_REQUIRE(h, "null handle");
+  const bsp_bsr_view& v = h->v;
+  BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
+  if (v.brows == 0) return BSP_OK;
+  if (kernel == BSP_BSR_AUTO) kernel = BSP_BSR_VALU;
+  hipStream_t st = (hipStream_t)s;
+  const int64_t grid = cdiv(v.brows * kWave, kTB);
+  if (kernel == BSP_BSR_VALU) {
+    bsr_valu_f64<<<grid, kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows, v.nrows, v.ncols,
+                                       v.tm, v.tk);
+  } else if (kernel == BSP_BSR_MFMA) {
+    if (v.tm != 4 || v.tk != 4) return fail(BSP_ERR_UNSUPPORTED, "MFMA BSR kernel needs 4x4 blocks");
+    bsr4_mfma_f64<<<grid, kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows, v.nrows, v.ncols);
+  } else {
+    return fail(BSP_ERR_INVALID, "unknown BSR kernel");
+  }
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status bsp_bsr_destroy(bsp_bsr h) {
+  delete h;
+  return BSP_OK;
+}
+
+bsp_status bsp_dbsr_build(const bsp_csr* A, bsp_stream s, bsp_dbsr* out) {
+  BSP_TRY(check_csr_f64(A));
+  BSP_REQUIRE(out, "null output handle");
+  auto h = std::make_unique<bsp_dbsr_s>();
+  BSP_TRY(build_dbsr(*A, (hipStream_t)s, h.get()));
+  *out = h.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_dbsr_view_get(bsp_dbsr h, bsp_dbsr_view* v) {
+  BSP_REQUIRE(h && v, "null handle");
+  *v = h->v;
+  return BSP_OK;
+}
+
+bsp_status bsp_dbsr_spmv(bsp_dbsr h, const double* x, double* y, int sorted_order, bsp_stream s) {
+  BSP_REQUIRE(h, "null handle");
+  const bsp_dbsr_view& v = h->v;
+  BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
+  if (v.brows == 0) return BSP_OK;
+  dbsr_spmv_f64<<<cdiv(v.brows * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
+      v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, sorted_order);
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status bsp_dbsr_destroy(bsp_dbsr h) {
+  delete h;
+  return BSP_OK;
+}
+
+}  // extern "C"

@@ -121,17 +121,37 @@ const char* bsp_status_string(bsp_status s) {
 
 const char* bsp_last_error(void) { return g_last_error.c_str(); }
 
+bsp_status bsp_copy_to_host(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return BSP_OK;
+  BSP_REQUIRE(dst && src, "null pointer");
+  BSP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return BSP_OK;
+}
+
 const char* bsp_version(void) { return "bsp 0.1 gfx950"; }
 
-bsp_status bsp_csr_plan_create(const bsp_csr* A, int method, bsp_stream s, bsp_spmv_plan* out) {
+bsp_status bsp_csr_plan_create_ex(const bsp_csr* A, int method, const bsp_csr_options* opt,
+                                  bsp_stream s, bsp_spmv_plan* out) {
   BSP_TRY(check_csr(A));
   BSP_REQUIRE(out, "null output handle");
   BSP_REQUIRE(A->dtype == BSP_F64, "CSR SpMV plan needs FP64 values");
   auto p = std::make_unique<bsp_spmv_plan_s>();
   p->A = *A;
   p->method = method;
+  if (opt) p->opt = *opt;
   BSP_TRY(csr_plan_build(p.get(), (hipStream_t)s));
   *out = p.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_csr_plan_create(const bsp_csr* A, int method, bsp_stream s, bsp_spmv_plan* out) {
+  return bsp_csr_plan_create_ex(A, method, nullptr, s, out);
+}
+
+bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* nseg) {
+  BSP_REQUIRE(p, "null plan");
+  if (seg_cols) *seg_cols = p->seg_width;
+  if (nseg) *nseg = (int64_t)p->seg.size();
   return BSP_OK;
 }
 

@@ -1,15 +1,38 @@
 // Per-matrix CSR SpMV schedule (what bsp_spmv_plan points at).
 #pragma once
 
+#include <vector>
+
 #include "bsp_hip.hpp"
+
+namespace bsp {
+// One column segment of the adaptive layout: a CSR over all rows holding the
+// entries whose column falls in [c0, c0 + width), with its row-block cut.
+struct CsrSegment {
+  const int32_t* rowptr = nullptr;
+  const int32_t* col = nullptr;
+  const double* val = nullptr;
+  int64_t nnz = 0;
+  DevBuf<int32_t> own_rowptr;
+  DevBuf<int32_t> blk;  // row-block starts, nblk+1
+  int64_t nblk = 0;
+  int64_t nlong = 0;
+};
+}  // namespace bsp
 
 struct bsp_spmv_plan_s {
   bsp_csr A{};       // borrowed device pointers
   int method = 0;    // bsp_csr_method
-  int64_t nblk = 0;  // adaptive: blocks; balanced: chunks; split: short rows
-  int64_t nlong = 0; // split: long rows; adaptive: long-row blocks (info only)
+  bsp_csr_options opt{};
+  int64_t nblk = 0;  // balanced: chunks; split: short rows; adaptive: total blocks
+  int64_t nlong = 0; // split: long rows; adaptive: long-row blocks
   bsp::DevBuf<int32_t> a, b;  // method-specific index arrays
   bsp::DevBuf<double> work;   // balanced: chunk partials
+  // adaptive
+  std::vector<bsp::CsrSegment> seg;
+  bsp::DevBuf<int32_t> seg_col;  // entries regrouped by column segment
+  bsp::DevBuf<double> seg_val;
+  int64_t seg_width = 0;         // 0: one segment (the input arrays as they are)
 };
 
 namespace bsp {

@@ -62,6 +62,9 @@ __global__ void scatter_flagged(const int32_t* __restrict__ flags, const int32_t
 
 __global__ void set_value_i32(int32_t* p, int32_t v) { *p = v; }
 
+// ACCUM: y += A_seg*x for a later column segment (rows without entries in
+// the segment are left untouched); otherwise y = A_seg*x.
+template <bool ACCUM>
 __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
@@ -89,7 +92,10 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     double v = group_sum<kWave>((a0 + a1) + (a2 + a3));
     if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
     __syncthreads();
-    if (tid == 0) y[r0] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (tid == 0) {
+      const double t = (red[0] + red[1]) + (red[2] + red[3]);
+      y[r0] = ACCUM ? y[r0] + t : t;
+    }
     return;
   }
 
@@ -130,7 +136,12 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     double s = 0.0;
     for (int i = rs + l; i < re; i += lpr) s += prod[i];
     s = group_sum_dyn(s, lpr);
-    if (l == 0) st_stream(&y[r0 + rr], s);
+    if (l == 0) {
+      if (!ACCUM)
+        st_stream(&y[r0 + rr], s);
+      else if (re > rs)
+        y[r0 + rr] += s;
+    }
   }
 }
 
@@ -261,22 +272,173 @@ static bsp_status stable_partition(const int32_t* flags, int64_t n, DevBuf<int32
   return BSP_OK;
 }
 
+
+// Column-segment support for the adaptive layout ----------------------------
+
+constexpr int64_t kDefaultSegCols = int64_t(1) << 18;  // 2 MB of FP64 x
+
+// Counts entries farther than `half` columns from the (scaled) diagonal and
+// rows whose columns are not strictly ascending.
+__global__ void csr_scatter_stats(const int32_t* __restrict__ rowptr,
+                                  const int32_t* __restrict__ col, int64_t nrows, int64_t ncols,
+                                  int64_t half, unsigned long long* far_count, int* unsorted) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int64_t d = r * ncols / nrows;
+  unsigned long long far = 0;
+  int bad = 0;
+  int prev = -1;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) {
+    const int c = col[j];
+    far += (c > d + half || c < d - half);
+    bad |= c <= prev;
+    prev = c;
+  }
+  if (far) atomicAdd(far_count, far);
+  if (bad) atomicOr(unsorted, 1);
+}
+
+// cnt[s*(nrows+1) + r] = entries of row r in column segment s.
+__global__ void seg_count(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                          int64_t nrows, int64_t width, int32_t* __restrict__ cnt) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j)
+    cnt[(int64_t)(col[j] / width) * (nrows + 1) + r] += 1;
+}
+
+// Copies each row's run of segment-s entries (contiguous: columns ascend)
+// to segment s's sub-CSR.
+__global__ void seg_scatter(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, int64_t nrows, int64_t width,
+                            const int32_t* __restrict__ segrp, const int64_t* __restrict__ base,
+                            int32_t* __restrict__ ocol, double* __restrict__ oval) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int64_t cur = -1, dst = 0;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) {
+    const int c = col[j];
+    const int64_t sgm = c / width;
+    if (sgm != cur) {
+      cur = sgm;
+      dst = base[sgm] + segrp[sgm * (nrows + 1) + r];
+    }
+    ocol[dst] = c;
+    oval[dst] = val[j];
+    ++dst;
+  }
+}
+
+static bsp_status adaptive_partition(CsrSegment& g, int64_t n, hipStream_t s) {
+  DevBuf<int32_t> flags;
+  BSP_TRY(flags.alloc(n));
+  csr_block_flags<<<cdiv(n, kWG), kWG, 0, s>>>(g.rowptr, n, flags.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(stable_partition(flags.get(), n, g.blk, &g.nblk, s));
+  set_value_i32<<<1, 1, 0, s>>>(g.blk.get() + g.nblk, (int32_t)n);
+  BSP_HIP(hipGetLastError());
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
+static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
+  const bsp_csr& A = p->A;
+  const int64_t n = A.nrows;
+  int64_t W = p->opt.seg_cols;
+  if (W == 0 && A.ncols > kDefaultSegCols && A.nnz > 0) {
+    DevBuf<unsigned long long> far;
+    DevBuf<int> unsorted;
+    BSP_TRY(far.alloc(1));
+    BSP_TRY(unsorted.alloc(1));
+    BSP_HIP(hipMemsetAsync(far.get(), 0, sizeof(unsigned long long), s));
+    BSP_HIP(hipMemsetAsync(unsorted.get(), 0, sizeof(int), s));
+    csr_scatter_stats<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, A.ncols,
+                                                   kDefaultSegCols / 2, far.get(), unsorted.get());
+    BSP_HIP(hipGetLastError());
+    unsigned long long nfar = 0;
+    int uns = 0;
+    BSP_TRY(fetch(far.get(), &nfar, s));
+    BSP_TRY(fetch(unsorted.get(), &uns, s));
+    W = (!uns && (int64_t)nfar * 4 > A.nnz) ? kDefaultSegCols : 0;
+  } else if (W > 0 && A.nnz > 0) {
+    DevBuf<unsigned long long> far;
+    DevBuf<int> unsorted;
+    BSP_TRY(far.alloc(1));
+    BSP_TRY(unsorted.alloc(1));
+    BSP_HIP(hipMemsetAsync(far.get(), 0, sizeof(unsigned long long), s));
+    BSP_HIP(hipMemsetAsync(unsorted.get(), 0, sizeof(int), s));
+    csr_scatter_stats<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, A.ncols, W / 2,
+                                                   far.get(), unsorted.get());
+    BSP_HIP(hipGetLastError());
+    int uns = 0;
+    BSP_TRY(fetch(unsorted.get(), &uns, s));
+    if (uns) return fail(BSP_ERR_UNSORTED, "column segmentation needs ascending columns per row");
+  }
+  if (W < 0) W = 0;
+  const int64_t nseg = W > 0 ? cdiv(A.ncols, W) : 1;
+  p->seg.clear();
+  p->seg.resize(nseg);
+  if (nseg == 1) {
+    p->seg_width = 0;
+    CsrSegment& g = p->seg[0];
+    g.rowptr = A.rowptr;
+    g.col = A.colidx;
+    g.val = static_cast<const double*>(A.val);
+    g.nnz = A.nnz;
+  } else {
+    p->seg_width = W;
+    DevBuf<int32_t> cnt;
+    BSP_TRY(cnt.alloc(nseg * (n + 1)));
+    BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * nseg * (n + 1), s));
+    seg_count<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, W, cnt.get());
+    BSP_HIP(hipGetLastError());
+    DevBuf<int32_t> segrp;
+    BSP_TRY(segrp.alloc(nseg * (n + 1)));
+    std::vector<int64_t> base(nseg + 1, 0);
+    for (int64_t k = 0; k < nseg; ++k) {
+      BSP_TRY(exclusive_scan_i32(cnt.get() + k * (n + 1), segrp.get() + k * (n + 1), n + 1, s));
+      int32_t tot = 0;
+      BSP_TRY(fetch(segrp.get() + k * (n + 1) + n, &tot, s));
+      base[k + 1] = base[k] + tot;
+    }
+    DevBuf<int64_t> dbase;
+    BSP_TRY(dbase.alloc(nseg));
+    BSP_HIP(hipMemcpyAsync(dbase.get(), base.data(), sizeof(int64_t) * nseg,
+                           hipMemcpyHostToDevice, s));
+    BSP_TRY(p->seg_col.alloc(A.nnz));
+    BSP_TRY(p->seg_val.alloc(A.nnz));
+    seg_scatter<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, static_cast<const double*>(A.val),
+                                             n, W, segrp.get(), dbase.get(), p->seg_col.get(),
+                                             p->seg_val.get());
+    BSP_HIP(hipGetLastError());
+    for (int64_t k = 0; k < nseg; ++k) {
+      CsrSegment& g = p->seg[k];
+      BSP_TRY(g.own_rowptr.alloc(n + 1));
+      BSP_HIP(hipMemcpyAsync(g.own_rowptr.get(), segrp.get() + k * (n + 1),
+                             sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+      g.rowptr = g.own_rowptr.get();
+      g.col = p->seg_col.get() + base[k];
+      g.val = p->seg_val.get() + base[k];
+      g.nnz = base[k + 1] - base[k];
+    }
+    BSP_HIP(hipStreamSynchronize(s));
+  }
+  p->nblk = 0;
+  for (auto& g : p->seg) {
+    BSP_TRY(adaptive_partition(g, n, s));
+    p->nblk += g.nblk;
+  }
+  return BSP_OK;
+}
+
 bsp_status csr_plan_build(CsrPlan* p, hipStream_t s) {
   const bsp_csr& A = p->A;
   const int64_t n = A.nrows;
   if (n == 0) return BSP_OK;
   switch (p->method) {
-    case BSP_CSR_ADAPTIVE: {
-      DevBuf<int32_t> flags;
-      BSP_TRY(flags.alloc(n));
-      csr_block_flags<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, n, flags.get());
-      BSP_HIP(hipGetLastError());
-      BSP_TRY(stable_partition(flags.get(), n, p->a, &p->nblk, s));
-      set_value_i32<<<1, 1, 0, s>>>(p->a.get() + p->nblk, (int32_t)n);
-      BSP_HIP(hipGetLastError());
-      BSP_HIP(hipStreamSynchronize(s));
+    case BSP_CSR_ADAPTIVE:
+      BSP_TRY(adaptive_build(p, s));
       break;
-    }
     case BSP_CSR_SCALAR:
       p->nblk = cdiv(n, kWG);
       break;
@@ -323,7 +485,16 @@ bsp_status csr_plan_run(const CsrPlan* p, const double* x, double* y, hipStream_
   const auto* v = static_cast<const double*>(A.val);
   switch (p->method) {
     case BSP_CSR_ADAPTIVE:
-      csr_adaptive_f64<<<p->nblk, kWG, 0, s>>>(rp, ci, v, x, y, p->a.get(), p->nblk);
+      for (size_t k = 0; k < p->seg.size(); ++k) {
+        const CsrSegment& g = p->seg[k];
+        if (g.nblk == 0) continue;
+        if (k == 0)
+          csr_adaptive_f64<false><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y,
+                                                         g.blk.get(), g.nblk);
+        else
+          csr_adaptive_f64<true><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y,
+                                                        g.blk.get(), g.nblk);
+      }
       break;
     case BSP_CSR_SCALAR:
       csr_scalar_f64<<<cdiv(n, kWG), kWG, 0, s>>>(rp, ci, v, x, y, n);

@@ -67,6 +67,20 @@ bsp_status sort_pairs_i32(const int32_t* keys_in, int32_t* keys_out, const int32
   return BSP_OK;
 }
 
+bsp_status sort_pairs_i64key(const int64_t* keys_in, int64_t* keys_out, const int32_t* vals_in,
+                             int32_t* vals_out, int64_t n, int end_bit, hipStream_t s) {
+  if (n <= 0) return BSP_OK;
+  size_t bytes = 0;
+  BSP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out,
+                                             (int)n, 0, end_bit, s));
+  Temp t;
+  BSP_TRY(t.get(bytes));
+  BSP_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, bytes, keys_in, keys_out, vals_in, vals_out,
+                                             (int)n, 0, end_bit, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
 bsp_status reduce_sum_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s) {
   size_t bytes = 0;
   BSP_HIP(hipcub::DeviceReduce::Sum(nullptr, bytes, in, out, (int)n, s));

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import CsrDesc, i64, lib
+from ._lib import CsrDesc, CsrOptions, i64, lib
 from .host import Csr
 
 F64, F32, F16, BF16 = 0, 1, 2, 3
@@ -55,13 +55,21 @@ class DeviceCsr:
 class CsrSpmv:
     """y = A*x with a per-matrix schedule built on the GPU (bsp_csr_plan_create)."""
 
-    def __init__(self, A: DeviceCsr, method: str | int = "adaptive"):
+    def __init__(self, A: DeviceCsr, method: str | int = "adaptive", seg_cols: int = 0):
         self.A = A
         self.method = METHODS[method] if isinstance(method, str) else int(method)
         self._desc = A.desc()
+        self._opt = CsrOptions(seg_cols)
         h = C.c_void_p()
-        lib().bsp_csr_plan_create(C.byref(self._desc), self.method, stream_handle(), C.byref(h))
+        lib().bsp_csr_plan_create_ex(C.byref(self._desc), self.method, C.byref(self._opt),
+                                     stream_handle(), C.byref(h))
         self._h = h
+
+    def segments(self) -> tuple[int, int]:
+        """(column-segment width, number of segments); width 0 = unsegmented."""
+        w, k = i64(), i64()
+        lib().bsp_csr_plan_segments(self._h, C.byref(w), C.byref(k))
+        return w.value, k.value
 
     def info(self) -> tuple[int, int]:
         nb, nl = i64(), i64()
@@ -83,5 +91,149 @@ class CsrSpmv:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+def _to_host(ptr_value, n, dtype) -> np.ndarray:
+    out = np.empty(max(int(n), 1), dtype)
+    if n > 0 and ptr_value:
+        lib().bsp_copy_to_host(out.ctypes.data, ptr_value, int(n) * out.itemsize)
+    return out[: int(n)]
+
+
+class Bsr:
+    """Block CSR with tm x tk blocks, built on the GPU (bsp_bsr_build)."""
+
+    VALU, MFMA = 1, 2
+
+    def __init__(self, A: DeviceCsr, tm: int = 4, tk: int = 4):
+        from ._lib import BsrView
+        self.A = A
+        d = A.desc()
+        h = C.c_void_p()
+        lib().bsp_bsr_build(C.byref(d), tm, tk, stream_handle(), C.byref(h))
+        self._h = h
+        self.view = BsrView()
+        lib().bsp_bsr_view_get(self._h, C.byref(self.view))
+
+    def arrays(self) -> dict:
+        v = self.view
+        out = dict(brows=v.brows, bcols=v.bcols, nblocks=v.nblocks, tm=v.tm, tk=v.tk,
+                   rowptr=_to_host(v.rowptr, v.brows + 1, np.int32),
+                   colidx=_to_host(v.colidx, v.nblocks, np.int32),
+                   val=_to_host(v.val, v.nblocks * v.tm * v.tk, np.float64),
+                   avg_nnz=v.avg_nnz, stand=v.stand, warpnum=v.warpnum)
+        if v.map:
+            out["map"] = _to_host(v.map, v.nblocks + 1, np.uint16)
+            out["rowptr_by_warp"] = _to_host(v.rowptr_by_warp, v.brows + 1, np.int32)
+            out["rowidx_by_warp"] = _to_host(v.rowidx_by_warp, v.warpnum, np.int32)
+        return out
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, kernel: int = 1):
+        if y is None:
+            y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
+        lib().bsp_bsr_spmv(self._h, kernel, ptr(x), ptr(y), stream_handle())
+        return y
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_bsr_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class Dbsr:
+    """DBSR (row-sorted, position-packed 4x4 blocks), built on the GPU."""
+
+    def __init__(self, A: DeviceCsr):
+        from ._lib import DbsrView
+        self.A = A
+        d = A.desc()
+        h = C.c_void_p()
+        lib().bsp_dbsr_build(C.byref(d), stream_handle(), C.byref(h))
+        self._h = h
+        self.view = DbsrView()
+        lib().bsp_dbsr_view_get(self._h, C.byref(self.view))
+
+    def arrays(self) -> dict:
+        v = self.view
+        return dict(brows=v.brows, nblocks=v.nblocks, warpnum=v.warpnum,
+                    overflow=bool(v.overflow),
+                    recorder=_to_host(v.recorder, v.nrows, np.int32),
+                    blcptr=_to_host(v.blcptr, v.brows + 1, np.int32),
+                    colidx=_to_host(v.colidx, v.nblocks * 16, np.int32),
+                    val=_to_host(v.val, v.nblocks * 16, np.float64),
+                    map=_to_host(v.map, v.nblocks, np.uint16),
+                    prefix=_to_host(v.prefix, v.warpnum + 1, np.int32))
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, sorted_order=False):
+        if y is None:
+            y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
+        lib().bsp_dbsr_spmv(self._h, ptr(x), ptr(y), int(sorted_order), stream_handle())
+        return y
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_dbsr_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+DASP_INTS = ["row_long", "row_block", "row_zero", "short_row_1", "short_row_2", "short_row_3",
+             "short_row_4", "common_13", "fill0_nnz_short13", "fill0_nnz_short34",
+             "fill0_nnz_short22", "fill0_nnz_short", "threadblock13", "threadblock34",
+             "threadblock22", "blocknum", "fill0_nnz_reg", "nnz_irreg", "warp_number",
+             "fill0_nnz_long", "nnz_long"]
+
+
+class Dasp:
+    """DASP layout built on the GPU (bsp_dasp_build)."""
+
+    def __init__(self, A: DeviceCsr, threshold: float = 0.75, block_longest: int = 256):
+        from ._lib import DaspView
+        self.A = A
+        d = A.desc()
+        h = C.c_void_p()
+        lib().bsp_dasp_build(C.byref(d), float(threshold), int(block_longest), stream_handle(),
+                             C.byref(h))
+        self._h = h
+        self.view = DaspView()
+        lib().bsp_dasp_view_get(self._h, C.byref(self.view))
+
+    def arrays(self) -> dict:
+        v = self.view
+        out = {k: getattr(v, k) for k in DASP_INTS}
+        out.update(
+            order_rid=_to_host(v.order_rid, v.nrows, np.int32),
+            short_val=_to_host(v.short_val, v.fill0_nnz_short, np.float64),
+            short_cid=_to_host(v.short_cid, v.fill0_nnz_short, np.int32),
+            long_rpt=_to_host(v.long_rpt, v.row_long + 1, np.int32),
+            long_val=_to_host(v.long_val, v.fill0_nnz_long, np.float64),
+            long_cid=_to_host(v.long_cid, v.fill0_nnz_long, np.int32),
+            rid_by_warp=_to_host(v.rid_by_warp, v.warp_number, np.int32),
+            block_ptr=_to_host(v.block_ptr, v.blocknum + 1, np.int32),
+            irreg_rpt=_to_host(v.irreg_rpt, v.row_block + 1, np.int32),
+            reg_val=_to_host(v.reg_val, v.fill0_nnz_reg, np.float64),
+            reg_cid=_to_host(v.reg_cid, v.fill0_nnz_reg, np.int32),
+            irreg_val=_to_host(v.irreg_val, v.nnz_irreg, np.float64),
+            irreg_cid=_to_host(v.irreg_cid, v.nnz_irreg, np.int32))
+        return out
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, permuted=False):
+        if y is None:
+            y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
+        lib().bsp_dasp_spmv(self._h, ptr(x), ptr(y), int(permuted), stream_handle())
+        return y
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_dasp_destroy(self._h)
+                self._h = None
         except Exception:
             pass

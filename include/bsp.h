@@ -45,6 +45,8 @@ typedef void* bsp_stream; /* hipStream_t */
 
 const char* bsp_status_string(bsp_status s);
 const char* bsp_last_error(void);
+/* Synchronous device-to-host copy of `bytes` (for reading format views). */
+bsp_status bsp_copy_to_host(void* dst, const void* src, size_t bytes);
 /* Library version and the gfx target its kernels were built for. */
 const char* bsp_version(void);
 
@@ -77,14 +79,146 @@ typedef enum bsp_csr_method {
 
 typedef struct bsp_spmv_plan_s* bsp_spmv_plan;
 
+/* Tuning of the adaptive method (zero-initialise for the defaults).
+ * seg_cols: column-segment width. The adaptive plan regroups the entries of
+ *   a matrix whose columns are scattered into column segments of this many
+ *   columns (8*seg_cols bytes of x stay L2-resident per pass) and runs one
+ *   pass per segment. 0 = automatic (segments of 2^18 columns when more
+ *   than a quarter of the entries lie farther than 2^17 columns from the
+ *   diagonal and rows are sorted), < 0 = never segment. */
+typedef struct bsp_csr_options {
+  int64_t seg_cols;
+  int64_t reserved[7];
+} bsp_csr_options;
+
 /* Build the per-matrix schedule for `method` on the GPU (stream-ordered).
- * The plan keeps the descriptor's device pointers; they must outlive it. */
+ * The plan keeps the descriptor's device pointers; they must outlive it
+ * (a segmented adaptive plan also holds its own regrouped copy of col/val:
+ * call bsp_csr_plan_create again after changing the values). */
 bsp_status bsp_csr_plan_create(const bsp_csr* A, int method, bsp_stream s, bsp_spmv_plan* out);
+bsp_status bsp_csr_plan_create_ex(const bsp_csr* A, int method, const bsp_csr_options* opt,
+                                  bsp_stream s, bsp_spmv_plan* out);
+/* Column-segment width of an adaptive plan (0 = not segmented) and the
+ * number of segments. */
+bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* nseg);
 /* y = A*x. x has ncols, y has nrows doubles (device). Stream-ordered. */
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s);
 /* Number of workgroups one bsp_csr_spmv launch uses (its main kernel). */
 bsp_status bsp_csr_plan_info(bsp_spmv_plan p, int64_t* nblocks, int64_t* nlong);
 bsp_status bsp_plan_destroy(bsp_spmv_plan p);
+
+/* ------------------------------------------------------------------------ */
+/* Block CSR (BSR) with tm x tk dense blocks, built on the GPU.              */
+/* Structure equals csrToBcsr<T,Y> (utils/utils.hpp:9-124): blocks ordered by
+ * block row then ascending block column, values row-major within a block
+ * (offset (r%tm)*tk + c%tk), zero padded. For 4x4 blocks the AmgT extras of
+ * CSR2BSR_GPU / BSR_BALANCED_PREPROCESS_GPU (src/amgT.cu:523-1200) are also
+ * built: the 16-bit occupancy map (bit r*4+c, nblocks+1 entries, last 0), the
+ * 64-blocks-per-warp balance arrays, avg_nnz = nnz/nblocks and
+ * stand = sqrt(mean((blocks per block row - mean)^2)). Duplicate (row, col)
+ * entries are not supported. */
+typedef struct bsp_bsr_s* bsp_bsr;
+typedef struct bsp_bsr_view {
+  int64_t nrows, ncols, nnz;    /* of the source CSR */
+  int64_t brows, bcols, nblocks;
+  int32_t tm, tk;
+  const int32_t* rowptr;        /* brows+1 */
+  const int32_t* colidx;        /* nblocks */
+  const double* val;            /* nblocks*tm*tk */
+  const uint16_t* map;          /* nblocks+1 (4x4 only, else NULL) */
+  int32_t warpnum;              /* 4x4 only */
+  const int32_t* rowptr_by_warp; /* brows+1 */
+  const int32_t* rowidx_by_warp; /* warpnum */
+  double avg_nnz, stand;
+} bsp_bsr_view;
+
+typedef enum bsp_bsr_kernel {
+  BSP_BSR_AUTO = 0,
+  /* Wave per block row, lane per (block, row) slot, VALU FP64. Any tile.
+   * Counterpart of bsr_spmv_cc_fp64 / bcsr_spmv_kernel_column_by_column
+   * (src/amgT.cu:276-322, src/bcsr.cu:161-222). */
+  BSP_BSR_VALU = 1,
+  /* 4x4 only: four blocks of a block row stacked into the 16x4 A operand of
+   * v_mfma_f64_16x16x4f64, x segments as B columns, the block-diagonal of the
+   * 16x16 result accumulated. Counterpart of bsr_spmv_tc_fp64
+   * (src/amgT.cu:210-274, mma.m8n8k4 with 2 blocks). */
+  BSP_BSR_MFMA = 2
+} bsp_bsr_kernel;
+
+bsp_status bsp_bsr_build(const bsp_csr* A, int tm, int tk, bsp_stream s, bsp_bsr* out);
+bsp_status bsp_bsr_view_get(bsp_bsr h, bsp_bsr_view* v);
+/* y = A*x (x: ncols, y: nrows doubles). */
+bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_stream s);
+bsp_status bsp_bsr_destroy(bsp_bsr h);
+
+/* ------------------------------------------------------------------------ */
+/* DBSR (include/dbsr.h): rows sorted by (length, index); each group of four
+ * sorted rows is packed position-wise into 4x4 "blocks" (element z of sorted
+ * row i -> block blcptr[i/4] + z/4, slot (i%4)*4 + z%4) with a column index
+ * per slot; plus the 64-blocks-per-warp schedule of balanceDbsr
+ * (dbsr.h:94-169, reproduced pass for pass). Built on the GPU. */
+typedef struct bsp_dbsr_s* bsp_dbsr;
+typedef struct bsp_dbsr_view {
+  int64_t nrows, ncols, nnz, brows, nblocks;
+  const int32_t* recorder; /* nrows: recorderMap[sorted] = original row */
+  const int32_t* blcptr;   /* brows+1 */
+  const int32_t* colidx;   /* nblocks*16 */
+  const double* val;       /* nblocks*16 */
+  const uint16_t* map;     /* nblocks */
+  int32_t warpnum;
+  const int32_t* prefix;   /* warpnum+1 */
+  /* 1 where the reference's second balance pass would run past the warp
+   * list of its first pass (undefined there); the list is then extended. */
+  int32_t overflow;
+} bsp_dbsr_view;
+
+bsp_status bsp_dbsr_build(const bsp_csr* A, bsp_stream s, bsp_dbsr* out);
+bsp_status bsp_dbsr_view_get(bsp_dbsr h, bsp_dbsr_view* v);
+/* sorted_order != 0: y[i] belongs to sorted row i (the reference's output
+ * order, un-permuted by the caller through recorderMap); otherwise y is in
+ * the original row order. */
+bsp_status bsp_dbsr_spmv(bsp_dbsr h, const double* x, double* y, int sorted_order, bsp_stream s);
+bsp_status bsp_dbsr_destroy(bsp_dbsr h);
+
+/* ------------------------------------------------------------------------ */
+/* DASP (src/dasp.cu:814-1503): rows classified as long (>= block_longest),
+ * row-block (5 .. block_longest-1, sorted longest first, stable), short
+ * (1, 2, 3, 4) and empty; each class packed into its padded layout; y is
+ * produced in order_rid order. Built on the GPU, arrays bit-exact with the
+ * reference's host builder (MMA_M 8, MMA_K 4, rowloop 1). */
+typedef struct bsp_dasp_s* bsp_dasp;
+typedef struct bsp_dasp_view {
+  int64_t nrows, ncols, nnz;
+  int32_t row_long, row_block, row_zero;
+  int32_t short_row_1, short_row_2, short_row_3, short_row_4, common_13;
+  int32_t fill0_nnz_short13, fill0_nnz_short34, fill0_nnz_short22, fill0_nnz_short;
+  int32_t threadblock13, threadblock34, threadblock22;
+  int32_t blocknum, fill0_nnz_reg, nnz_irreg;
+  int32_t warp_number, fill0_nnz_long, nnz_long;
+  const int32_t* order_rid;   /* nrows */
+  const double* short_val;    /* fill0_nnz_short */
+  const int32_t* short_cid;
+  const int32_t* long_rpt;    /* row_long+1, in 64-entry warps */
+  const double* long_val;     /* fill0_nnz_long */
+  const int32_t* long_cid;
+  const int32_t* rid_by_warp; /* warp_number */
+  const int32_t* block_ptr;   /* blocknum+1 */
+  const int32_t* irreg_rpt;   /* row_block+1 */
+  const double* reg_val;      /* fill0_nnz_reg */
+  const int32_t* reg_cid;
+  const double* irreg_val;    /* nnz_irreg */
+  const int32_t* irreg_cid;
+} bsp_dasp_view;
+
+/* threshold in (0, 1]: a 4-column slab of an 8-row block stays regular while
+ * it holds >= threshold*32 entries (the reference passes 0.75, 256). */
+bsp_status bsp_dasp_build(const bsp_csr* A, double threshold, int32_t block_longest, bsp_stream s,
+                          bsp_dasp* out);
+bsp_status bsp_dasp_view_get(bsp_dasp h, bsp_dasp_view* v);
+/* y = A*x. permuted != 0: y[k] is row order_rid[k] (the reference's Y_val);
+ * otherwise y is in the original row order. */
+bsp_status bsp_dasp_spmv(bsp_dasp h, const double* x, double* y, int permuted, bsp_stream s);
+bsp_status bsp_dasp_destroy(bsp_dasp h);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side synthetic matrices (generators; see benchsp_amd/csrc/gen.cpp). */

@@ -1,0 +1,159 @@
+"""GPU format builders bit-exact against the oracle's CPU builders, and the
+blocked SpMV kernels against the oracle's SpMV (through the C-ABI)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from benchsp_amd.host import Csr
+from tests._parity import abs_spmv, assert_spmv_close
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _rand_csr(rng, lens, ncols):
+    rp = np.zeros(len(lens) + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(ncols, L, replace=False)) if L else
+                          np.zeros(0, np.int64) for L in lens]).astype(np.int32)
+    return Csr(len(lens), ncols, rp, col, rng.uniform(-1, 1, int(rp[-1])))
+
+
+def _mats(bsp_host):
+    rng = np.random.default_rng(3)
+    yield "random", _rand_csr(rng, rng.integers(0, 40, 301), 263)
+    yield "stencil27-64", bsp_host.stencil27_64()
+    yield "powerlaw-100k", bsp_host.powerlaw(100_000, 20.0, 1.2, 2000, seed=4)
+    yield "empty-rows", _rand_csr(rng, [0] * 7 + [3] + [0] * 9, 50)
+
+
+def _same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape and np.array_equal(a, b), what
+
+
+@pytest.mark.parametrize("tm,tk", [(4, 4), (3, 3), (16, 16), (2, 5)])
+def test_bsr_builder_bit_exact(oracle, bsp_host, tm, tk):
+    import torch
+    from benchsp_amd.device import Bsr, DeviceCsr
+    for name, A in _mats(bsp_host):
+        if tm == 16 and A.nrows > 200_000:
+            continue
+        print(f"[bsr] {name} {tm}x{tk}", flush=True)
+        g = Bsr(DeviceCsr.from_host(A), tm, tk).arrays()
+        o = oracle.bcsr(A.nrows, A.ncols, A.rowptr, A.col, A.val, tm, tk)
+        assert g["nblocks"] == o["nblocks"], name
+        for k in ("rowptr", "colidx", "val"):
+            _same(g[k], o[k], f"{name} {k}")
+        if tm == 4 and tk == 4:
+            for k in ("map", "rowptr_by_warp", "rowidx_by_warp"):
+                _same(g[k], o[k], f"{name} {k}")
+            assert g["warpnum"] == o["warpnum"]
+            assert g["avg_nnz"] == o["avg_nnz"]
+            assert abs(g["stand"] - o["stand"]) <= 1e-12 * max(1.0, o["stand"])
+        torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+@pytest.mark.parametrize("tile", [4, 3, 16])
+def test_bsr_spmv(oracle, bsp_host, kernel, tile):
+    import torch
+    from benchsp_amd.device import Bsr, DeviceCsr
+    if kernel == "mfma" and tile != 4:
+        pytest.skip("MFMA kernel is 4x4 only")
+    for name, A in _mats(bsp_host):
+        if tile == 16 and A.nrows > 200_000:
+            continue
+        print(f"[bsr-spmv] {name} {tile} {kernel}", flush=True)
+        x = bsp_host.uniform(A.ncols, 21)
+        B = Bsr(DeviceCsr.from_host(A), tile, tile)
+        y = B(torch.from_numpy(x).cuda(), kernel=Bsr.VALU if kernel == "valu" else Bsr.MFMA)
+        torch.cuda.synchronize()
+        assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                          abs_spmv(oracle, A, x), what=f"{name} bsr{tile} {kernel}")
+
+
+def _dbsr_inputs(bsp_host):
+    for p in sorted(glob.glob(os.path.join(GOLDEN, "dbsr_*.npz"))):
+        g = np.load(p)
+        yield os.path.basename(p), Csr(int(g["nrows"]), int(g["ncols"]),
+                                       g["rowptr"].astype(np.int64), g["col"], g["val"])
+    rng = np.random.default_rng(0)
+    yield "overflow-case", _rand_csr(rng, [0, 1, 2, 3, 4, 5, 9, 16, 17, 33, 70, 0, 2, 64, 65, 8, 1], 90)
+    yield "stencil27-64", bsp_host.stencil27_64()
+    yield "powerlaw-1M", bsp_host.powerlaw_1m()
+
+
+def test_dbsr_builder_bit_exact(oracle, bsp_host):
+    from benchsp_amd.device import DeviceCsr, Dbsr
+    for name, A in _dbsr_inputs(bsp_host):
+        print(f"[dbsr] {name}", flush=True)
+        g = Dbsr(DeviceCsr.from_host(A)).arrays()
+        o = oracle.dbsr(A.nrows, A.ncols, A.rowptr, A.col, A.val)
+        assert g["warpnum"] == o["warpnum"] and g["overflow"] == o["overflow"], name
+        for k in ("recorder", "blcptr", "colidx", "val", "map", "prefix"):
+            _same(g[k], o[k], f"{name} {k}")
+
+
+def test_dbsr_spmv(oracle, bsp_host):
+    import torch
+    from benchsp_amd.device import DeviceCsr, Dbsr
+    for name, A in _dbsr_inputs(bsp_host):
+        x = bsp_host.uniform(A.ncols, 5)
+        D = Dbsr(DeviceCsr.from_host(A))
+        xd = torch.from_numpy(x).cuda()
+        y = D(xd).cpu().numpy()
+        ys = D(xd, sorted_order=True).cpu().numpy()
+        y_ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
+        bound = abs_spmv(oracle, A, x)
+        assert_spmv_close(y, y_ref, bound, what=f"{name} dbsr")
+        rec = D.arrays()["recorder"]
+        un = np.empty_like(ys)
+        un[rec] = ys  # the caller's un-permute step (bench/spmvfp64.cpp:163-166)
+        assert np.array_equal(un, y)
+
+
+def _dasp_inputs(bsp_host):
+    rng = np.random.default_rng(5)
+    lens = np.concatenate([np.full(300, 1), np.full(200, 3), np.full(77, 2), np.full(50, 4),
+                           np.full(9, 0), rng.integers(5, 300, 400), [256, 255, 1000, 5000]])
+    rng.shuffle(lens)
+    yield "mixed", _rand_csr(rng, lens, 6000)
+    yield "stencil27-64", bsp_host.stencil27_64()
+    yield "powerlaw-1M", bsp_host.powerlaw_1m()
+    yield "poisson5-256", bsp_host.poisson5_256()
+
+
+DASP_KEYS = ["order_rid", "short_val", "short_cid", "long_rpt", "long_val", "long_cid",
+             "rid_by_warp", "block_ptr", "irreg_rpt", "reg_val", "reg_cid", "irreg_val",
+             "irreg_cid"]
+
+
+def test_dasp_builder_bit_exact(oracle, bsp_host):
+    from benchsp_amd.device import DASP_INTS, Dasp, DeviceCsr
+    for name, A in _dasp_inputs(bsp_host):
+        print(f"[dasp] {name}", flush=True)
+        g = Dasp(DeviceCsr.from_host(A)).arrays()
+        o = oracle.dasp(A.rowptr, A.col, A.val, 0.75, 256)
+        for k in DASP_INTS:
+            assert g[k] == o[k], f"{name} {k}: {g[k]} vs {o[k]}"
+        for k in DASP_KEYS:
+            _same(g[k], o[k], f"{name} {k}")
+
+
+@pytest.mark.parametrize("threshold,longest", [(0.75, 256), (0.5, 64), (1.0, 1000)])
+def test_dasp_spmv(oracle, bsp_host, threshold, longest):
+    import torch
+    from benchsp_amd.device import Dasp, DeviceCsr
+    for name, A in _dasp_inputs(bsp_host):
+        x = bsp_host.uniform(A.ncols, 8)
+        D = Dasp(DeviceCsr.from_host(A), threshold, longest)
+        xd = torch.from_numpy(x).cuda()
+        y = D(xd).cpu().numpy()
+        yp = D(xd, permuted=True).cpu().numpy()
+        y_ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
+        assert_spmv_close(y, y_ref, abs_spmv(oracle, A, x), what=f"{name} dasp")
+        order = D.arrays()["order_rid"]
+        assert np.array_equal(yp, y[order])  # Y_val in order_rid order

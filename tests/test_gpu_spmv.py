@@ -90,6 +90,68 @@ def test_all_empty_rows(method):
     assert np.array_equal(y, np.zeros(1000))
 
 
+def _run_seg(A, x, seg_cols):
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=seg_cols)
+    y = op(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    return y.cpu().numpy(), op.segments()
+
+
+def test_adaptive_segments_powerlaw_auto(oracle, bsp_host):
+    A = bsp_host.powerlaw_1m()
+    x = bsp_host.uniform(A.ncols, seed=13)
+    y, (w, k) = _run_seg(A, x, 0)
+    assert w == 1 << 18 and k == 4  # scattered columns: segmented automatically
+    assert_spmv_close(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x), abs_spmv(oracle, A, x),
+                      what="powerlaw segmented")
+
+
+def test_adaptive_stencil_not_segmented(bsp_host):
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    A = bsp_host.stencil3d(80, 80, 80, 7)  # 512,000 columns, all near the diagonal
+    assert CsrSpmv(DeviceCsr.from_host(A), "adaptive").segments() == (0, 1)
+
+
+@pytest.mark.parametrize("seg_cols", [1000, 4097])
+def test_adaptive_forced_segments(oracle, bsp_host, seg_cols):
+    rng = np.random.default_rng(9)
+    from benchsp_amd.host import Csr
+    lens = np.array([0, 3, 0, 5000, 1, 129, 2047, 2049, 17] + [0] * 40 + [1] * 3000 + [40] * 500)
+    n, ncols = lens.size, 9000
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(ncols, L, replace=False)) for L in lens]).astype(np.int32)
+    A = Csr(n, ncols, rp, col, rng.uniform(-1, 1, rp[-1]))
+    x = rng.uniform(-1, 1, ncols)
+    y, (w, k) = _run_seg(A, x, seg_cols)
+    assert w == seg_cols and k == -(-ncols // seg_cols)
+    assert_spmv_close(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x), abs_spmv(oracle, A, x),
+                      what=f"forced segments {seg_cols}")
+    S = bsp_host.stencil3d(20, 21, 22, 27, seed=5)
+    xs = bsp_host.uniform(S.ncols, 3)
+    y, _ = _run_seg(S, xs, seg_cols)
+    assert_spmv_close(y, oracle.spmv_f64(S.rowptr, S.col, S.val, xs), abs_spmv(oracle, S, xs),
+                      what="stencil forced segments")
+
+
+def test_adaptive_segments_reject_unsorted():
+    import torch
+    from benchsp_amd import BspError
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    from benchsp_amd.host import Csr
+    A = Csr(2, 100, np.array([0, 2, 3]), np.array([50, 7, 3], np.int32), np.ones(3))
+    with pytest.raises(BspError) as e:
+        CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=10)
+    assert e.value.status == 4
+    # unsorted rows are still fine unsegmented
+    y = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=-1)(torch.ones(100, dtype=torch.float64,
+                                                                             device="cuda"))
+    assert y.cpu().tolist() == [2.0, 1.0]
+
+
 def test_reference_surface_wrappers(oracle, bsp_host):
     import ctypes as C
     from benchsp_amd._lib import lib
