@@ -14,7 +14,7 @@ struct CsrSegment {
   const double* val = nullptr;
   int64_t nnz = 0;
   DevBuf<int32_t> own_rowptr;
-  DevBuf<int32_t> blk;  // row-block starts, nblk+1
+  DevBuf<int32_t> blk;  // nblk+1 {first row, first entry} pairs
   int64_t nblk = 0;
   int64_t nlong = 0;
 };
@@ -33,6 +33,7 @@ struct bsp_spmv_plan_s {
   bsp::DevBuf<int32_t> seg_col;  // entries regrouped by column segment
   bsp::DevBuf<double> seg_val;
   int64_t seg_width = 0;         // 0: one segment (the input arrays as they are)
+  int chunk = 1024;              // adaptive block window (rows + entries)
 };
 
 namespace bsp {

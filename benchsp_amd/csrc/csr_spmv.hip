@@ -13,10 +13,10 @@ namespace bsp {
 // Adaptive kernel.
 //
 // Rows are cut into blocks on the merge-path coordinate u(r) = rowptr[r] + r:
-// rows whose start coordinate falls in the same window of kChunk units share
-// a block, and every row longer than kLongRow gets a block of its own. A
-// stream block therefore has at most kChunk rows and fewer than
-// kChunk + kLongRow nonzeros, all of which fit in LDS.
+// rows whose start coordinate falls in the same window of CHUNK units share a
+// block, and every row longer than CHUNK/2 gets a block of its own. A stream
+// block therefore has at most CHUNK rows and fewer than 1.5*CHUNK nonzeros,
+// all of which fit in LDS.
 //
 // Stream block: every lane streams values/column indices (coalesced,
 // non-temporal), gathers x and parks the product in LDS; then groups of
@@ -24,30 +24,35 @@ namespace bsp {
 // row each. With lpr == 1 a row is summed left to right from separately
 // rounded products, which is the reference CPU loop's arithmetic exactly.
 // Long block: the whole workgroup strides over the one row.
+//
 constexpr int kWG = 256;
-constexpr int kChunk = 2048;
-constexpr int kLongRow = 1024;
-constexpr int kMaxBlockNnz = kChunk + kLongRow;
-constexpr int kLoadIters = kMaxBlockNnz / kWG;
+
+template <int CHUNK>
+struct Ad {
+  static constexpr int LONG = CHUNK / 2;
+  static constexpr int MAXNNZ = CHUNK + LONG;
+  static constexpr int IT = MAXNNZ / kWG;  // value/index loads per lane
+};
 
 // Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range of
-// logical blocks so neighbouring rows (and their x entries) share an L2.
+// logical work so neighbouring rows (and their x entries) share an L2.
 __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
   constexpr int64_t X = 8;
   int64_t q = nb / X, rem = nb % X, xcd = bid % X, k = bid / X;
   return xcd * q + (xcd < rem ? xcd : rem) + k;
 }
 
-__global__ void csr_block_flags(const int32_t* __restrict__ rowptr, int64_t nrows,
+__global__ void csr_block_flags(const int32_t* __restrict__ rowptr, int64_t nrows, int chunk,
                                 int32_t* __restrict__ flags) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
+  const int longrow = chunk / 2;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int f = 1;
   if (r > 0) {
     int32_t sp = rowptr[r - 1];
-    bool lng = (e - s) > kLongRow, lngp = (s - sp) > kLongRow;
-    int64_t key = ((int64_t)s + r) / kChunk, keyp = ((int64_t)sp + r - 1) / kChunk;
+    bool lng = (e - s) > longrow, lngp = (s - sp) > longrow;
+    int64_t key = ((int64_t)s + r) / chunk, keyp = ((int64_t)sp + r - 1) / chunk;
     f = (key != keyp) || lng || lngp;
   }
   flags[r] = f;
@@ -62,24 +67,26 @@ __global__ void scatter_flagged(const int32_t* __restrict__ flags, const int32_t
 
 __global__ void set_value_i32(int32_t* p, int32_t v) { *p = v; }
 
-// ACCUM: y += A_seg*x for a later column segment (rows without entries in
-// the segment are left untouched); otherwise y = A_seg*x.
-template <bool ACCUM>
+// One block per workgroup. The block's first row and first entry come from
+// one 8-byte load each ({row, entry offset} pairs built with the plan), so a
+// workgroup's stream loads start after a single round trip.
+template <int CHUNK, bool ACCUM>
 __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
-    const int32_t* __restrict__ blk_row, int64_t nblk) {
-  __shared__ double prod[kMaxBlockNnz];
-  __shared__ int32_t rptr[kChunk + 1];
+    const int2* __restrict__ blk, int64_t nblk) {
+  using C = Ad<CHUNK>;
+  __shared__ double prod[C::MAXNNZ];
+  __shared__ int32_t rptr[CHUNK + 1];
+  __shared__ double yb[ACCUM ? CHUNK : 1];
   __shared__ double red[kWG / kWave];
-  const int64_t b = xcd_remap(blockIdx.x, nblk);
   const int tid = threadIdx.x;
-  const int r0 = blk_row[b], r1 = blk_row[b + 1];
-  const int p0 = rowptr[r0], p1 = rowptr[r1];
-  const int nnz = p1 - p0;
-  const int nrows = r1 - r0;
+  const int64_t b = xcd_remap(blockIdx.x, nblk);
+  const int2 s0 = blk[b], s1 = blk[b + 1];
+  const int r0 = s0.x, p0 = s0.y, r1 = s1.x, p1 = s1.y;
+  const int nnz = p1 - p0, nrows = r1 - r0;
 
-  if (nrows == 1 && nnz > kLongRow) {
+  if (nrows == 1 && nnz > C::LONG) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int i = p0 + tid;
     for (; i + 3 * kWG < p1; i += 4 * kWG) {
@@ -89,7 +96,7 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
       a3 += ld_stream(&val[i + 3 * kWG]) * x[ld_stream(&col[i + 3 * kWG])];
     }
     for (; i < p1; i += kWG) a0 += ld_stream(&val[i]) * x[ld_stream(&col[i])];
-    double v = group_sum<kWave>((a0 + a1) + (a2 + a3));
+    const double v = group_sum<kWave>((a0 + a1) + (a2 + a3));
     if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
     __syncthreads();
     if (tid == 0) {
@@ -99,33 +106,36 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     return;
   }
 
-  // Issue every index/value load of the block before the first dependent x
-  // gather, so a workgroup pays two memory round trips, not one per
-  // iteration. Iterations past the block end re-read its last entry (same
-  // cache line for the whole wave) instead of branching per lane.
+  // Stream loads (lanes past the block end re-read its last entry), the
+  // block's row pointers and, when accumulating, its y entries: all
+  // independent, all in flight together.
+  int ci[C::IT];
+  double vv[C::IT];
+  const int iters = (nnz + kWG - 1) / kWG;
   if (nnz > 0) {
-    const int iters = (nnz + kWG - 1) / kWG;
-    const int last = p1 - 1;
-    int ci[kLoadIters];
-    double vv[kLoadIters], xv[kLoadIters];
 #pragma unroll
-    for (int k = 0; k < kLoadIters; ++k) {
+    for (int k = 0; k < C::IT; ++k) {
       if (k < iters) {
-        const int j = min(p0 + tid + k * kWG, last);
+        const int j = min(p0 + tid + k * kWG, p1 - 1);
         ci[k] = ld_stream(&col[j]);
         vv[k] = ld_stream(&val[j]);
       }
     }
+  }
+  for (int i = tid; i <= nrows; i += kWG) rptr[i] = rowptr[r0 + i] - p0;
+  if (ACCUM)
+    for (int i = tid; i < nrows; i += kWG) yb[i] = y[r0 + i];
+  if (nnz > 0) {
+    double xv[C::IT];
 #pragma unroll
-    for (int k = 0; k < kLoadIters; ++k)
+    for (int k = 0; k < C::IT; ++k)
       if (k < iters) xv[k] = x[ci[k]];
 #pragma unroll
-    for (int k = 0; k < kLoadIters; ++k) {
+    for (int k = 0; k < C::IT; ++k) {
       const int i = tid + k * kWG;
       if (k < iters && i < nnz) prod[i] = vv[k] * xv[k];
     }
   }
-  for (int i = tid; i <= nrows; i += kWG) rptr[i] = rowptr[r0 + i] - p0;
   __syncthreads();
 
   int lpr = 1;
@@ -140,9 +150,36 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
       if (!ACCUM)
         st_stream(&y[r0 + rr], s);
       else if (re > rs)
-        y[r0 + rr] += s;
+        st_stream(&y[r0 + rr], yb[rr] + s);
     }
   }
+}
+
+template <int CHUNK>
+static void launch_adaptive_t(const CsrSegment& g, bool accum, const double* x, double* y,
+                              hipStream_t s) {
+  const int2* blk = reinterpret_cast<const int2*>(g.blk.get());
+  if (accum)
+    csr_adaptive_f64<CHUNK, true><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y, blk, g.nblk);
+  else
+    csr_adaptive_f64<CHUNK, false><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y, blk, g.nblk);
+}
+
+static void launch_adaptive(const CsrSegment& g, int chunk, bool accum, const double* x, double* y,
+                            hipStream_t s) {
+  if (chunk == 2048)
+    launch_adaptive_t<2048>(g, accum, x, y, s);
+  else
+    launch_adaptive_t<1024>(g, accum, x, y, s);
+}
+
+// {first row, first entry} of every block; blk[nblk] = {nrows, nnz}.
+__global__ void blk_pairs(const int32_t* __restrict__ starts, const int32_t* __restrict__ rowptr,
+                          int64_t nblk, int64_t nrows, int2* __restrict__ blk) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nblk) return;
+  const int r = b == nblk ? (int)nrows : starts[b];
+  blk[b] = make_int2(r, rowptr[r]);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,7 +312,7 @@ static bsp_status stable_partition(const int32_t* flags, int64_t n, DevBuf<int32
 
 // Column-segment support for the adaptive layout ----------------------------
 
-constexpr int64_t kDefaultSegCols = int64_t(1) << 18;  // 2 MB of FP64 x
+constexpr int64_t kDefaultSegCols = int64_t(1) << 19;  // 4 MB of FP64 x (one XCD's L2)
 
 // Counts entries farther than `half` columns from the (scaled) diagonal and
 // rows whose columns are not strictly ascending.
@@ -329,13 +366,16 @@ __global__ void seg_scatter(const int32_t* __restrict__ rowptr, const int32_t* _
   }
 }
 
-static bsp_status adaptive_partition(CsrSegment& g, int64_t n, hipStream_t s) {
+static bsp_status adaptive_partition(CsrSegment& g, int64_t n, int chunk, hipStream_t s) {
   DevBuf<int32_t> flags;
   BSP_TRY(flags.alloc(n));
-  csr_block_flags<<<cdiv(n, kWG), kWG, 0, s>>>(g.rowptr, n, flags.get());
+  csr_block_flags<<<cdiv(n, kWG), kWG, 0, s>>>(g.rowptr, n, chunk, flags.get());
   BSP_HIP(hipGetLastError());
-  BSP_TRY(stable_partition(flags.get(), n, g.blk, &g.nblk, s));
-  set_value_i32<<<1, 1, 0, s>>>(g.blk.get() + g.nblk, (int32_t)n);
+  DevBuf<int32_t> starts;
+  BSP_TRY(stable_partition(flags.get(), n, starts, &g.nblk, s));
+  BSP_TRY(g.blk.alloc(2 * (g.nblk + 1)));
+  blk_pairs<<<cdiv(g.nblk + 1, kWG), kWG, 0, s>>>(starts.get(), g.rowptr, g.nblk, n,
+                                                  reinterpret_cast<int2*>(g.blk.get()));
   BSP_HIP(hipGetLastError());
   BSP_HIP(hipStreamSynchronize(s));
   return BSP_OK;
@@ -423,9 +463,10 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     }
     BSP_HIP(hipStreamSynchronize(s));
   }
+  p->chunk = p->opt.chunk == 2048 ? 2048 : 1024;
   p->nblk = 0;
   for (auto& g : p->seg) {
-    BSP_TRY(adaptive_partition(g, n, s));
+    BSP_TRY(adaptive_partition(g, n, p->chunk, s));
     p->nblk += g.nblk;
   }
   return BSP_OK;
@@ -488,12 +529,7 @@ bsp_status csr_plan_run(const CsrPlan* p, const double* x, double* y, hipStream_
       for (size_t k = 0; k < p->seg.size(); ++k) {
         const CsrSegment& g = p->seg[k];
         if (g.nblk == 0) continue;
-        if (k == 0)
-          csr_adaptive_f64<false><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y,
-                                                         g.blk.get(), g.nblk);
-        else
-          csr_adaptive_f64<true><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y,
-                                                        g.blk.get(), g.nblk);
+        launch_adaptive(g, p->chunk, k > 0, x, y, s);
       }
       break;
     case BSP_CSR_SCALAR:

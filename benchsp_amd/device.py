@@ -55,11 +55,12 @@ class DeviceCsr:
 class CsrSpmv:
     """y = A*x with a per-matrix schedule built on the GPU (bsp_csr_plan_create)."""
 
-    def __init__(self, A: DeviceCsr, method: str | int = "adaptive", seg_cols: int = 0):
+    def __init__(self, A: DeviceCsr, method: str | int = "adaptive", seg_cols: int = 0,
+                 chunk: int = 0):
         self.A = A
         self.method = METHODS[method] if isinstance(method, str) else int(method)
         self._desc = A.desc()
-        self._opt = CsrOptions(seg_cols)
+        self._opt = CsrOptions(seg_cols, chunk)
         h = C.c_void_p()
         lib().bsp_csr_plan_create_ex(C.byref(self._desc), self.method, C.byref(self._opt),
                                      stream_handle(), C.byref(h))

@@ -83,12 +83,15 @@ typedef struct bsp_spmv_plan_s* bsp_spmv_plan;
  * seg_cols: column-segment width. The adaptive plan regroups the entries of
  *   a matrix whose columns are scattered into column segments of this many
  *   columns (8*seg_cols bytes of x stay L2-resident per pass) and runs one
- *   pass per segment. 0 = automatic (segments of 2^18 columns when more
- *   than a quarter of the entries lie farther than 2^17 columns from the
+ *   pass per segment. 0 = automatic (segments of 2^19 columns when more
+ *   than a quarter of the entries lie farther than 2^18 columns from the
  *   diagonal and rows are sorted), < 0 = never segment. */
 typedef struct bsp_csr_options {
   int64_t seg_cols;
-  int64_t reserved[7];
+  /* adaptive block window in rows+entries: 1024 or 2048 (0 = 1024) */
+  int32_t chunk;
+  int32_t reserved0;
+  int64_t reserved[6];
 } bsp_csr_options;
 
 /* Build the per-matrix schedule for `method` on the GPU (stream-ordered).
