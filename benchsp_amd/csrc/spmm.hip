@@ -1,0 +1,429 @@
+// CSR SpMM with dense row-major operands, its transpose-free transpose product
+// and SDDMM on the pattern: the forward and backward of Y = A*X.
+//
+// Work list: every row is cut into chunks of at most kChunk entries; a wave
+// owns one chunk. Rows with one chunk write their output row directly; longer
+// rows (graph hubs) write per-chunk partials that a second pass adds up in
+// chunk order, so results are deterministic and free of atomics.
+//
+// A^T * dY reuses the same kernel over the pattern's CSC (column pointers,
+// row indices and the CSR position of every entry, built once with the
+// plan); values are read through that permutation, never copied.
+#include <memory>
+
+#include "bsp_hip.hpp"
+
+struct bsp_spmm_plan_s {
+  bsp_csr A{};
+  // forward (CSR) and transposed (CSC) traversals
+  struct Side {
+    const int32_t* ptr = nullptr;
+    const int32_t* idx = nullptr;
+    const int32_t* perm = nullptr;  // CSC only: CSR position of each entry
+    int64_t nrows = 0;
+    bsp::DevBuf<int4> items;        // {row, begin, end, partial slot or -1}
+    int64_t nitems = 0;
+    bsp::DevBuf<int4> combine;      // {row, first slot, slot count, 0}
+    int64_t ncombine = 0;
+    int64_t nslots = 0;
+  } fwd, bwd;
+  bsp::DevBuf<int32_t> colptr, rowidx, perm;
+  bsp::DevBuf<float> partial;       // max slots * F floats, grown on demand
+  int64_t partial_cap = 0;
+};
+
+namespace bsp {
+namespace {
+
+constexpr int kTB = 256;
+constexpr int kChunk = 512;
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <class T>
+__device__ __forceinline__ float to_f32(T v);
+template <>
+__device__ __forceinline__ float to_f32<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float to_f32<uint16_t>(uint16_t v) { return bf16_to_f32(v); }
+
+// VEC consecutive features of a row of X starting at p, as floats.
+template <class T, int VEC>
+__device__ __forceinline__ void load_feats(const T* p, float (&f)[VEC]);
+template <>
+__device__ __forceinline__ void load_feats<float, 1>(const float* p, float (&f)[1]) { f[0] = *p; }
+template <>
+__device__ __forceinline__ void load_feats<float, 2>(const float* p, float (&f)[2]) {
+  const float2 v = *reinterpret_cast<const float2*>(p);
+  f[0] = v.x; f[1] = v.y;
+}
+template <>
+__device__ __forceinline__ void load_feats<float, 4>(const float* p, float (&f)[4]) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  f[0] = v.x; f[1] = v.y; f[2] = v.z; f[3] = v.w;
+}
+template <>
+__device__ __forceinline__ void load_feats<uint16_t, 1>(const uint16_t* p, float (&f)[1]) {
+  f[0] = bf16_to_f32(*p);
+}
+template <>
+__device__ __forceinline__ void load_feats<uint16_t, 2>(const uint16_t* p, float (&f)[2]) {
+  const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+  f[0] = __uint_as_float(v << 16); f[1] = __uint_as_float(v & 0xffff0000u);
+}
+template <>
+__device__ __forceinline__ void load_feats<uint16_t, 4>(const uint16_t* p, float (&f)[4]) {
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  f[0] = __uint_as_float(v.x << 16); f[1] = __uint_as_float(v.x & 0xffff0000u);
+  f[2] = __uint_as_float(v.y << 16); f[3] = __uint_as_float(v.y & 0xffff0000u);
+}
+
+__device__ __forceinline__ int rdlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rdlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Y[row, :] (or a partial) = sum over the chunk of a * X[col, :]; wave per
+// work item, lane l owns features [l*VEC, l*VEC+VEC) (F = 64*VEC). The
+// chunk's (col, a) pairs are loaded 64 at a time, one per lane, and
+// broadcast with readlane.
+template <class TV, class TX, int VEC, bool GATHER>
+__global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ idx,
+                                                  const TV* __restrict__ val,
+                                                  const int32_t* __restrict__ perm,
+                                                  const TX* __restrict__ X, float* __restrict__ Y,
+                                                  const int4* __restrict__ items, int64_t nitems,
+                                                  float* __restrict__ partial) {
+  constexpr int F = kWave * VEC;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (w >= nitems) return;
+  const int4 it = items[w];
+  float acc[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) acc[v] = 0.f;
+  for (int base = it.y; base < it.z; base += kWave) {
+    const int j = base + lane;
+    int c = 0;
+    float a = 0.f;
+    if (j < it.z) {
+      c = ld_stream(&idx[j]);
+      a = to_f32<TV>(GATHER ? val[perm[j]] : ld_stream(&val[j]));
+    }
+    const int n = min(kWave, it.z - base);
+    int t = 0;
+    for (; t + 4 <= n; t += 4) {
+      float x0[VEC], x1[VEC], x2[VEC], x3[VEC];
+      const int c0 = rdlane_i(c, t), c1 = rdlane_i(c, t + 1), c2 = rdlane_i(c, t + 2),
+                c3 = rdlane_i(c, t + 3);
+      load_feats<TX, VEC>(X + (int64_t)c0 * F + lane * VEC, x0);
+      load_feats<TX, VEC>(X + (int64_t)c1 * F + lane * VEC, x1);
+      load_feats<TX, VEC>(X + (int64_t)c2 * F + lane * VEC, x2);
+      load_feats<TX, VEC>(X + (int64_t)c3 * F + lane * VEC, x3);
+      const float a0 = rdlane_f(a, t), a1 = rdlane_f(a, t + 1), a2 = rdlane_f(a, t + 2),
+                  a3 = rdlane_f(a, t + 3);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc[v] += a0 * x0[v] + a1 * x1[v] + a2 * x2[v] + a3 * x3[v];
+    }
+    for (; t < n; ++t) {
+      float x0[VEC];
+      load_feats<TX, VEC>(X + (int64_t)rdlane_i(c, t) * F + lane * VEC, x0);
+      const float a0 = rdlane_f(a, t);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc[v] += a0 * x0[v];
+    }
+  }
+  float* dst = it.w < 0 ? Y + (int64_t)it.x * F : partial + (int64_t)it.w * F;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) dst[lane * VEC + v] = acc[v];
+}
+
+// Y[row, :] = sum of its chunk partials, in chunk order.
+__global__ __launch_bounds__(kTB) void spmm_combine(const int4* __restrict__ comb, int64_t ncomb,
+                                                    const float* __restrict__ partial, int64_t F,
+                                                    float* __restrict__ Y) {
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (w >= ncomb) return;
+  const int4 c = comb[w];
+  for (int64_t f = lane; f < F; f += kWave) {
+    float s = 0.f;
+    for (int k = 0; k < c.z; ++k) s += partial[(int64_t)(c.y + k) * F + f];
+    Y[(int64_t)c.x * F + f] = s;
+  }
+}
+
+// dvals[j] = dot(dY[row_j, :], X[col_j, :]); wave per work item, G = F/4
+// lanes per entry (float4 each), 64/G entries per step; the item's dY row is
+// loaded once and reused for all its entries.
+template <class TX, int F>
+__global__ __launch_bounds__(kTB) void sddmm_items(const int32_t* __restrict__ col,
+                                                   const TX* __restrict__ dY,
+                                                   const TX* __restrict__ X,
+                                                   const int4* __restrict__ items, int64_t nitems,
+                                                   float* __restrict__ dvals) {
+  constexpr int G = F / 4, NZ = kWave / G;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (w >= nitems) return;
+  const int4 it = items[w];
+  const int q = lane % G, sub = lane / G;
+  float dy[4];
+  load_feats<TX, 4>(dY + (int64_t)it.x * F + q * 4, dy);
+  for (int j0 = it.y; j0 < it.z; j0 += NZ) {
+    const int j = j0 + sub;
+    const bool ok = j < it.z;
+    const int c = ok ? col[j] : 0;
+    float xv[4];
+    load_feats<TX, 4>(X + (int64_t)c * F + q * 4, xv);
+    float p = dy[0] * xv[0] + dy[1] * xv[1] + dy[2] * xv[2] + dy[3] * xv[3];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+    if (q == 0 && ok) dvals[j] = p;
+  }
+}
+
+// ---- work-list construction ----------------------------------------------
+
+__global__ void chunk_counts(const int32_t* __restrict__ ptr, int64_t n, int32_t* __restrict__ nch,
+                             int32_t* __restrict__ nslot, int32_t* __restrict__ multi) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  if (r == n) {
+    nch[r] = 0;
+    nslot[r] = 0;
+    multi[r] = 0;
+    return;
+  }
+  const int len = ptr[r + 1] - ptr[r];
+  const int c = len <= kChunk ? 1 : (len + kChunk - 1) / kChunk;
+  nch[r] = c;
+  nslot[r] = c > 1 ? c : 0;
+  multi[r] = c > 1;
+}
+
+__global__ void fill_items(const int32_t* __restrict__ ptr, int64_t n,
+                           const int32_t* __restrict__ itemptr, const int32_t* __restrict__ slotptr,
+                           const int32_t* __restrict__ combpos, const int32_t* __restrict__ multi,
+                           int4* __restrict__ items, int4* __restrict__ comb) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = ptr[r], e = ptr[r + 1];
+  const int c = itemptr[r + 1] - itemptr[r];
+  for (int k = 0; k < c; ++k)
+    items[itemptr[r] + k] = make_int4((int)r, s + k * kChunk, min(e, s + (k + 1) * kChunk),
+                                      c > 1 ? slotptr[r] + k : -1);
+  if (multi[r]) comb[combpos[r]] = make_int4((int)r, slotptr[r], c, 0);
+}
+
+__global__ void csc_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                         int64_t n, int64_t ncols, int64_t* __restrict__ key,
+                         int32_t* __restrict__ idx) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+    key[j] = (int64_t)col[j] * n + r;
+    idx[j] = j;
+  }
+}
+
+__global__ void csc_fill(const int64_t* __restrict__ skey, int64_t nnz, int64_t n,
+                         int32_t* __restrict__ rowidx, int32_t* __restrict__ colcnt) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  rowidx[k] = (int32_t)(skey[k] % n);
+  atomicAdd(&colcnt[skey[k] / n], 1);
+}
+
+}  // namespace
+
+static bsp_status build_items(bsp_spmm_plan_s::Side& sd, hipStream_t s) {
+  const int64_t n = sd.nrows;
+  DevBuf<int32_t> nch, nslot, multi, itemptr, slotptr, combpos;
+  BSP_TRY(nch.alloc(n + 1));
+  BSP_TRY(nslot.alloc(n + 1));
+  BSP_TRY(multi.alloc(n + 1));
+  BSP_TRY(itemptr.alloc(n + 1));
+  BSP_TRY(slotptr.alloc(n + 1));
+  BSP_TRY(combpos.alloc(n + 1));
+  chunk_counts<<<cdiv(n + 1, kTB), kTB, 0, s>>>(sd.ptr, n, nch.get(), nslot.get(), multi.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(exclusive_scan_i32(nch.get(), itemptr.get(), n + 1, s));
+  BSP_TRY(exclusive_scan_i32(nslot.get(), slotptr.get(), n + 1, s));
+  BSP_TRY(exclusive_scan_i32(multi.get(), combpos.get(), n + 1, s));
+  int32_t ni = 0, ns = 0, nc = 0;
+  BSP_TRY(fetch(itemptr.get() + n, &ni, s));
+  BSP_TRY(fetch(slotptr.get() + n, &ns, s));
+  BSP_TRY(fetch(combpos.get() + n, &nc, s));
+  sd.nitems = ni;
+  sd.nslots = ns;
+  sd.ncombine = nc;
+  BSP_TRY(sd.items.alloc(ni));
+  BSP_TRY(sd.combine.alloc(nc));
+  if (n > 0)
+    fill_items<<<cdiv(n, kTB), kTB, 0, s>>>(sd.ptr, n, itemptr.get(), slotptr.get(), combpos.get(),
+                                            multi.get(), sd.items.get(), sd.combine.get());
+  BSP_HIP(hipGetLastError());
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
+static bsp_status build_spmm_plan(bsp_spmm_plan_s* p, hipStream_t s) {
+  const bsp_csr& A = p->A;
+  p->fwd.ptr = A.rowptr;
+  p->fwd.idx = A.colidx;
+  p->fwd.nrows = A.nrows;
+  BSP_TRY(build_items(p->fwd, s));
+  // CSC of the pattern: stable sort of (col, row) keys
+  const int64_t nnz = A.nnz;
+  DevBuf<int64_t> key, skey;
+  DevBuf<int32_t> idx, colcnt;
+  BSP_TRY(key.alloc(nnz));
+  BSP_TRY(skey.alloc(nnz));
+  BSP_TRY(idx.alloc(nnz));
+  BSP_TRY(p->perm.alloc(nnz));
+  BSP_TRY(p->rowidx.alloc(nnz));
+  BSP_TRY(colcnt.alloc(A.ncols + 1));
+  BSP_TRY(p->colptr.alloc(A.ncols + 1));
+  BSP_HIP(hipMemsetAsync(colcnt.get(), 0, sizeof(int32_t) * (A.ncols + 1), s));
+  if (nnz > 0) {
+    csc_keys<<<cdiv(A.nrows, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, A.nrows, A.ncols, key.get(),
+                                                idx.get());
+    BSP_HIP(hipGetLastError());
+    int bits = 1;
+    while (bits < 63 && (int64_t(1) << bits) <= A.ncols * A.nrows) ++bits;
+    BSP_TRY(sort_pairs_i64key(key.get(), skey.get(), idx.get(), p->perm.get(), nnz, bits, s));
+    csc_fill<<<cdiv(nnz, kTB), kTB, 0, s>>>(skey.get(), nnz, A.nrows, p->rowidx.get(), colcnt.get());
+    BSP_HIP(hipGetLastError());
+  }
+  BSP_TRY(exclusive_scan_i32(colcnt.get(), p->colptr.get(), A.ncols + 1, s));
+  p->bwd.ptr = p->colptr.get();
+  p->bwd.idx = p->rowidx.get();
+  p->bwd.perm = p->perm.get();
+  p->bwd.nrows = A.ncols;
+  BSP_TRY(build_items(p->bwd, s));
+  return BSP_OK;
+}
+
+template <class TV, class TX, bool GATHER>
+static bsp_status run_spmm_t(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd, const void* V,
+                             const void* X,
+                             int64_t F, float* Y, hipStream_t s) {
+  const auto* val = static_cast<const TV*>(V);
+  const auto* x = static_cast<const TX*>(X);
+  const int64_t grid = cdiv(sd.nitems * kWave, kTB);
+  if (sd.nslots * F > p->partial_cap) {
+    BSP_HIP(hipStreamSynchronize(s));
+    BSP_TRY(p->partial.alloc(sd.nslots * F));
+    p->partial_cap = sd.nslots * F;
+  }
+  if (sd.nitems == 0) return BSP_OK;
+  switch (F) {
+    case 64:
+      spmm_items<TV, TX, 1, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
+                                                         sd.nitems, p->partial.get());
+      break;
+    case 128:
+      spmm_items<TV, TX, 2, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
+                                                         sd.nitems, p->partial.get());
+      break;
+    case 256:
+      spmm_items<TV, TX, 4, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
+                                                         sd.nitems, p->partial.get());
+      break;
+    default:
+      return fail(BSP_ERR_UNSUPPORTED, "SpMM feature width must be 64, 128 or 256");
+  }
+  BSP_HIP(hipGetLastError());
+  if (sd.ncombine > 0) {
+    spmm_combine<<<cdiv(sd.ncombine * kWave, kTB), kTB, 0, s>>>(sd.combine.get(), sd.ncombine,
+                                                                 p->partial.get(), F, Y);
+    BSP_HIP(hipGetLastError());
+  }
+  return BSP_OK;
+}
+
+template <bool GATHER>
+static bsp_status run_spmm(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd, const void* V,
+                           bsp_dtype vt, const void* X,
+                           bsp_dtype xt, int64_t F, float* Y, hipStream_t s) {
+  const bool vb = vt == BSP_BF16, xb = xt == BSP_BF16;
+  if (vb && xb) return run_spmm_t<uint16_t, uint16_t, GATHER>(p, sd, V, X, F, Y, s);
+  if (vb) return run_spmm_t<uint16_t, float, GATHER>(p, sd, V, X, F, Y, s);
+  if (xb) return run_spmm_t<float, uint16_t, GATHER>(p, sd, V, X, F, Y, s);
+  return run_spmm_t<float, float, GATHER>(p, sd, V, X, F, Y, s);
+}
+
+}  // namespace bsp
+
+using namespace bsp;
+
+extern "C" {
+
+bsp_status bsp_spmm_plan_create(const bsp_csr* A, bsp_stream s, bsp_spmm_plan* out) {
+  BSP_REQUIRE(A && out, "null argument");
+  BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
+  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
+      A->ncols >= (int64_t(1) << 31))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
+  BSP_REQUIRE(A->nnz == 0 || A->colidx, "null colidx");
+  auto p = std::make_unique<bsp_spmm_plan_s>();
+  p->A = *A;
+  BSP_TRY(build_spmm_plan(p.get(), (hipStream_t)s));
+  *out = p.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_spmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* X,
+                    bsp_dtype xdtype, int64_t F, float* Y, bsp_stream s) {
+  BSP_REQUIRE(p && (X || p->A.ncols == 0) && (Y || p->A.nrows == 0), "null argument");
+  BSP_REQUIRE(val || p->A.nnz == 0, "null values");
+  BSP_REQUIRE(xdtype == BSP_F32 || xdtype == BSP_BF16, "X must be FP32 or BF16");
+  BSP_REQUIRE(vdtype == BSP_F32 || vdtype == BSP_BF16, "values must be FP32 or BF16");
+  return run_spmm<false>(p, p->fwd, val, vdtype, X, xdtype, F, Y, (hipStream_t)s);
+}
+
+bsp_status bsp_spmm_t(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* dY,
+                      bsp_dtype dtype, int64_t F, float* dX, bsp_stream s) {
+  BSP_REQUIRE(p && (dY || p->A.nrows == 0) && (dX || p->A.ncols == 0), "null argument");
+  BSP_REQUIRE(val || p->A.nnz == 0, "null values");
+  BSP_REQUIRE(dtype == BSP_F32 || dtype == BSP_BF16, "dY must be FP32 or BF16");
+  BSP_REQUIRE(vdtype == BSP_F32 || vdtype == BSP_BF16, "values must be FP32 or BF16");
+  return run_spmm<true>(p, p->bwd, val, vdtype, dY, dtype, F, dX, (hipStream_t)s);
+}
+
+bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype dtype, int64_t F,
+                     float* dvals, bsp_stream s) {
+  BSP_REQUIRE(p && dvals, "null argument");
+  BSP_REQUIRE(dtype == BSP_F32 || dtype == BSP_BF16, "operands must be FP32 or BF16");
+  const auto& sd = p->fwd;
+  if (sd.nitems == 0 || p->A.nnz == 0) return BSP_OK;
+  const int64_t grid = cdiv(sd.nitems * kWave, kTB);
+  hipStream_t st = (hipStream_t)s;
+  if (F == 64 && dtype == BSP_F32)
+    sddmm_items<float, 64><<<grid, kTB, 0, st>>>(p->A.colidx, (const float*)dY, (const float*)X,
+                                                 sd.items.get(), sd.nitems, dvals);
+  else if (F == 128 && dtype == BSP_F32)
+    sddmm_items<float, 128><<<grid, kTB, 0, st>>>(p->A.colidx, (const float*)dY, (const float*)X,
+                                                  sd.items.get(), sd.nitems, dvals);
+  else if (F == 64)
+    sddmm_items<uint16_t, 64><<<grid, kTB, 0, st>>>(p->A.colidx, (const uint16_t*)dY,
+                                                    (const uint16_t*)X, sd.items.get(), sd.nitems,
+                                                    dvals);
+  else if (F == 128)
+    sddmm_items<uint16_t, 128><<<grid, kTB, 0, st>>>(p->A.colidx, (const uint16_t*)dY,
+                                                     (const uint16_t*)X, sd.items.get(), sd.nitems,
+                                                     dvals);
+  else
+    return fail(BSP_ERR_UNSUPPORTED, "SDDMM feature width must be 64 or 128");
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status bsp_spmm_plan_destroy(bsp_spmm_plan p) {
+  delete p;
+  return BSP_OK;
+}
+
+}  // extern "C"

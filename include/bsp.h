@@ -224,6 +224,30 @@ bsp_status bsp_dasp_spmv(bsp_dasp h, const double* x, double* y, int permuted, b
 bsp_status bsp_dasp_destroy(bsp_dasp h);
 
 /* ------------------------------------------------------------------------ */
+/* Differentiable CSR SpMM: Y = A*X with dense row-major X (ncols x F) and Y
+ * (nrows x F, FP32), FP32 accumulation. A's values are FP32 or BF16, dense
+ * operands FP32 or BF16; F is 64, 128 or 256. No counterpart in the
+ * reference (its only SpMM is the block kernel of src/smat.cu); this is the
+ * forward and backward of a GNN-style layer:
+ *   bsp_spmm    Y  = A * X
+ *   bsp_spmm_t  dX = A^T * dY   (over the pattern's CSC, values read through
+ *                                the CSR permutation: no transposed copy)
+ *   bsp_sddmm   dvals[j] = dY[row_j, :] . X[col_j, :]   (A's pattern, CSR order)
+ * The plan (work lists, CSC permutation) is built on the GPU from the
+ * pattern alone (A.val may be NULL); the values (nnz entries in CSR order,
+ * FP32 or BF16) are passed to every call. Results are deterministic (no
+ * atomics). */
+typedef struct bsp_spmm_plan_s* bsp_spmm_plan;
+bsp_status bsp_spmm_plan_create(const bsp_csr* A, bsp_stream s, bsp_spmm_plan* out);
+bsp_status bsp_spmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* X,
+                    bsp_dtype xdtype, int64_t F, float* Y, bsp_stream s);
+bsp_status bsp_spmm_t(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* dY,
+                      bsp_dtype dtype, int64_t F, float* dX, bsp_stream s);
+bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype dtype, int64_t F,
+                     float* dvals, bsp_stream s);
+bsp_status bsp_spmm_plan_destroy(bsp_spmm_plan p);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side synthetic matrices (generators; see benchsp_amd/csrc/gen.cpp). */
 typedef struct bsp_host_csr_s* bsp_host_csr;
 bsp_status bsp_gen_uniform(int64_t n, uint64_t seed, double* out);

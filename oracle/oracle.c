@@ -768,3 +768,67 @@ void orc_dasp_free(orc_dasp* D) {
   free(D->irreg_cid);
   free(D);
 }
+
+/* ------------------------------------------------------------------------ */
+/* FP64 oracles for the SpMM path (no counterpart in the reference's CPU
+ * code: the reference checks nothing for SpMM, bench/spmmfp16.cpp:75-79).
+ * Textbook definitions, row-major dense operands with F columns. */
+
+/* Y = A X: Y[i,:] = sum_j A_ij X[j,:], entries in CSR order. */
+void orc_csr_spmm_f64(int64_t nrows, const int64_t* rowptr, const int32_t* col, const double* val,
+                      const double* X, int64_t F, double* Y) {
+  for (int64_t i = 0; i < nrows; i++) {
+    double* y = Y + i * F;
+    for (int64_t f = 0; f < F; f++) y[f] = 0.0;
+    for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++) {
+      const double a = val[j];
+      const double* x = X + (int64_t)col[j] * F;
+      for (int64_t f = 0; f < F; f++) y[f] += a * x[f];
+    }
+  }
+}
+
+/* dX = A^T dY: dX[c,:] = sum over entries (i,c) of A_ic dY[i,:]. */
+void orc_csr_spmm_t_f64(int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* col,
+                        const double* val, const double* dY, int64_t F, double* dX) {
+  for (int64_t k = 0; k < ncols * F; k++) dX[k] = 0.0;
+  for (int64_t i = 0; i < nrows; i++)
+    for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++) {
+      const double a = val[j];
+      double* d = dX + (int64_t)col[j] * F;
+      const double* g = dY + i * F;
+      for (int64_t f = 0; f < F; f++) d[f] += a * g[f];
+    }
+}
+
+/* dvals[j] = dY[row_j,:] . X[col_j,:] for every entry (CSR order). */
+void orc_sddmm_f64(int64_t nrows, const int64_t* rowptr, const int32_t* col, const double* dY,
+                   const double* X, int64_t F, double* dvals) {
+  for (int64_t i = 0; i < nrows; i++)
+    for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++) {
+      const double* g = dY + i * F;
+      const double* x = X + (int64_t)col[j] * F;
+      double s = 0.0;
+      for (int64_t f = 0; f < F; f++) s += g[f] * x[f];
+      dvals[j] = s;
+    }
+}
+
+/* Block-sparse C = A B with A in BSR (bs x bs dense blocks, row-major,
+ * `brows` block rows) and dense B (ncols x N, row-major): FP64 block GEMM. */
+void orc_bsr_spmm_f64(int64_t brows, int32_t bs, const int32_t* browptr, const int32_t* bcol,
+                      const double* bval, const double* B, int64_t N, double* Cm) {
+  for (int64_t k = 0; k < brows * bs * N; k++) Cm[k] = 0.0;
+  for (int64_t br = 0; br < brows; br++)
+    for (int32_t b = browptr[br]; b < browptr[br + 1]; b++) {
+      const double* a = bval + (int64_t)b * bs * bs;
+      for (int32_t i = 0; i < bs; i++) {
+        double* c = Cm + (br * bs + i) * N;
+        for (int32_t k = 0; k < bs; k++) {
+          const double av = a[i * bs + k];
+          const double* brow = B + ((int64_t)bcol[b] * bs + k) * N;
+          for (int64_t n = 0; n < N; n++) c[n] += av * brow[n];
+        }
+      }
+    }
+}

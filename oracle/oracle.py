@@ -262,3 +262,58 @@ def ref_dbsr(nrows, ncols, rowptr, col, val):
                 val=_arr(o.blcVal, nb * 16, np.float64),
                 map=_arr(o.blcMap, nb, np.uint16),
                 prefix=_arr(o.prefix, o.warpnum + 1, np.int32))
+
+
+# --------------------------------------------------------------------------
+# FP64 SpMM / SDDMM / block SpMM
+
+def _spmm_lib():
+    L = lib()
+    if not getattr(L, "_spmm_declared", False):
+        L.orc_csr_spmm_f64.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_int64, _f64p]
+        L.orc_csr_spmm_t_f64.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _f64p, _f64p,
+                                         C.c_int64, _f64p]
+        L.orc_sddmm_f64.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_int64, _f64p]
+        L.orc_bsr_spmm_f64.argtypes = [C.c_int64, C.c_int32, _i32p, _i32p, _f64p, _f64p,
+                                       C.c_int64, _f64p]
+        L._spmm_declared = True
+    return L
+
+
+def csr_spmm_f64(rowptr, col, val, X):
+    """Y = A X in FP64 (X row-major, nrows(X) = ncols(A))."""
+    rp = _c64(rowptr)
+    X = np.ascontiguousarray(X, np.float64)
+    Y = np.zeros((rp.shape[0] - 1, X.shape[1]))
+    _spmm_lib().orc_csr_spmm_f64(rp.shape[0] - 1, rp, _c32(col), _cf64(val), X, X.shape[1], Y)
+    return Y
+
+
+def csr_spmm_t_f64(rowptr, col, val, dY, ncols):
+    """dX = A^T dY in FP64."""
+    rp = _c64(rowptr)
+    dY = np.ascontiguousarray(dY, np.float64)
+    dX = np.zeros((int(ncols), dY.shape[1]))
+    _spmm_lib().orc_csr_spmm_t_f64(rp.shape[0] - 1, int(ncols), rp, _c32(col), _cf64(val), dY,
+                                   dY.shape[1], dX)
+    return dX
+
+
+def sddmm_f64(rowptr, col, dY, X):
+    """dvals[j] = dY[row_j] . X[col_j] in FP64."""
+    rp = _c64(rowptr)
+    dY = np.ascontiguousarray(dY, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    out = np.zeros(int(rp[-1]))
+    _spmm_lib().orc_sddmm_f64(rp.shape[0] - 1, rp, _c32(col), dY, X, X.shape[1], out)
+    return out
+
+
+def bsr_spmm_f64(browptr, bcol, bval, bs, B):
+    """C = A B for A in BSR (bs x bs row-major blocks), FP64."""
+    bp = _c32(browptr)
+    B = np.ascontiguousarray(B, np.float64)
+    Cm = np.zeros(((bp.shape[0] - 1) * bs, B.shape[1]))
+    _spmm_lib().orc_bsr_spmm_f64(bp.shape[0] - 1, int(bs), bp, _c32(bcol), _cf64(bval), B,
+                                 B.shape[1], Cm)
+    return Cm
