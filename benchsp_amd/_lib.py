@@ -67,6 +67,11 @@ class DaspView(C.Structure):
 
 # name -> (restype, argtypes); restype None means bsp_status (checked).
 _SIGS: dict[str, tuple] = {
+    "bsp_bspmm_build": (None, [vp, C.c_int, vp, C.POINTER(vp)]),
+    "bsp_bspmm_view_get": (None, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i64p]),
+    "bsp_bspmm_fwd": (None, [vp, vp, i64, vp, vp]),
+    "bsp_bspmm_bwd": (None, [vp, vp, vp, i64, vp, vp, vp]),
+    "bsp_bspmm_destroy": (None, [vp]),
     "bsp_spmm_plan_create": (None, [C.POINTER(CsrDesc), vp, C.POINTER(vp)]),
     "bsp_spmm": (None, [vp, vp, C.c_int, vp, C.c_int, i64, vp, vp]),
     "bsp_spmm_t": (None, [vp, vp, C.c_int, vp, C.c_int, i64, vp, vp]),

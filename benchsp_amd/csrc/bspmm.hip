@@ -1,0 +1,422 @@
+// Block-sparse SpMM on FP16/BF16 MFMA with FP32 accumulation (the SMaT
+// kernel of src/smat.cu:22-91 re-designed for gfx950), with its backward.
+//
+// Format (built on the GPU from a 16x16 BSR with FP64 values):
+//   scale[b]  = 2^e with max|A_b| in [2^(e-1), 2^e): a power of two computed
+//               in FP64, so A_b / scale[b] is exact before the one rounding
+//               to FP16/BF16 and the values fill (-1, 1);
+//   hv[b]     = A_b / scale[b], 16x16 row-major, FP16 or BF16;
+//   hvt[b]    = the same block transposed (for A^T products);
+//   a transposed block index (blocks of each block column).
+//
+// MFMA operand layouts (v_mfma_f32_16x16x16{f16,bf16_1k}, x32 for K = 32):
+//   A (16xK): lane l holds A[l%16][E*(l/16) .. +E-1]   (E = K/4)
+//   B (Kx16): lane l holds B[E*(l/16) .. +E-1][l%16]
+//   D (16x16 FP32): lane l holds D[4*(l/16)+v][l%16] in acc[v]
+// Both operands are read k-contiguous: A from hv (row-major), B from a
+// transposed copy of the dense operand (B^T is N x K row-major), so every
+// lane issues one 8- or 16-byte load per operand.
+//
+//   forward  C[16 rows of br, :] = sum_b scale_b * hv_b * B[16 rows of bc_b, :]
+//   dB       dB[16 rows of bc, :] = sum_b scale_b * hvt_b * dC[16 rows of br_b, :]
+//   dvals    dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T   (K = N)
+// One workgroup per block row (block column for dB); its four waves take
+// every fourth block and their FP32 partials are added in wave order through
+// LDS, so results are deterministic.
+#include <cmath>
+#include <memory>
+
+#include "bsp_hip.hpp"
+
+struct bsp_bspmm_s {
+  int64_t M = 0, K = 0, brows = 0, bcols = 0, nblocks = 0;
+  bsp_dtype ht = BSP_F16;
+  const int32_t* browptr = nullptr;  // borrowed from the BSR
+  const int32_t* bcol = nullptr;
+  bsp::DevBuf<float> scale;
+  bsp::DevBuf<uint16_t> hv, hvt;
+  bsp::DevBuf<int32_t> tcolptr, tperm, tbrow;
+  bsp::DevBuf<uint16_t> work;  // transposed dense operand
+  int64_t work_cap = 0;
+};
+
+namespace bsp {
+namespace {
+
+constexpr int kTB = 256;
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+template <bool BF>
+__device__ __forceinline__ f4 mma16(u2 a, u2 b, f4 c) {
+  if constexpr (BF)
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a),
+                                                     __builtin_bit_cast(s4, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(h4, a),
+                                                 __builtin_bit_cast(h4, b), c, 0, 0, 0);
+}
+
+template <bool BF>
+__device__ __forceinline__ f4 mma32(u4 a, u4 b, f4 c) {
+  if constexpr (BF)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+                                                   __builtin_bit_cast(b8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                  __builtin_bit_cast(h8, b), c, 0, 0, 0);
+}
+
+template <bool BF>
+__device__ __forceinline__ uint16_t to_half(double v) {
+  if constexpr (BF)
+    return __builtin_bit_cast(uint16_t, (__bf16)v);
+  else
+    return __builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+// wave per block: FP64 max|a|, power-of-two scale, one rounding to half
+template <bool BF>
+__global__ __launch_bounds__(kTB) void block_scale_convert(const double* __restrict__ bval,
+                                                           int64_t nblocks, float* __restrict__ scale,
+                                                           uint16_t* __restrict__ hv,
+                                                           uint16_t* __restrict__ hvt) {
+  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (b >= nblocks) return;
+  const double* a = bval + b * 256;
+  double v[4], m = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[q] = a[lane + 64 * q];
+    m = fmax(m, fabs(v[q]));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  int e = 0;
+  if (m > 0.0) frexp(m, &e);
+  const double s = ldexp(1.0, e);
+  if (lane == 0) scale[b] = (float)s;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = lane + 64 * q, i = idx / 16, j = idx % 16;
+    const uint16_t h = to_half<BF>(v[q] / s);
+    hv[b * 256 + idx] = h;
+    hvt[b * 256 + j * 16 + i] = h;
+  }
+}
+
+__global__ void block_rows_of(const int32_t* __restrict__ browptr, int64_t brows,
+                              int32_t* __restrict__ brow) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= brows) return;
+  for (int b = browptr[r]; b < browptr[r + 1]; ++b) brow[b] = (int32_t)r;
+}
+
+__global__ void tkeys(const int32_t* __restrict__ bcol, const int32_t* __restrict__ brow,
+                      int64_t nblocks, int64_t brows, int64_t* __restrict__ key,
+                      int32_t* __restrict__ idx, int32_t* __restrict__ cnt) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  key[b] = (int64_t)bcol[b] * brows + brow[b];
+  idx[b] = (int32_t)b;
+  atomicAdd(&cnt[bcol[b]], 1);
+}
+
+__global__ void gather_i32(const int32_t* __restrict__ src, const int32_t* __restrict__ perm,
+                           int64_t n, int32_t* __restrict__ dst) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[perm[k]];
+}
+
+// dst[c][r] = src[r][c] for r < R, c < Cn; dst has Rp >= R columns, the
+// extra ones zero. 64x64 tiles through LDS.
+__global__ __launch_bounds__(kTB) void transpose_u16(const uint16_t* __restrict__ src, int64_t R,
+                                                     int64_t Cn, uint16_t* __restrict__ dst,
+                                                     int64_t Rp) {
+  __shared__ uint16_t t[64][65];
+  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < 64; k += 4) {
+    const int64_t r = r0 + k, c = c0 + tx;
+    t[k][tx] = (r < R && c < Cn) ? src[r * Cn + c] : (uint16_t)0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const int64_t c = c0 + k, r = r0 + tx;
+    if (c < Cn && r < Rp) dst[c * Rp + r] = t[tx][k];
+  }
+}
+
+// Adds the four waves' NT tiles in wave order (through LDS) and stores rows
+// [row0, row0+16) of a row-major FP32 matrix with N = 16*NT columns.
+template <int NT>
+__device__ __forceinline__ void reduce_store(f4 (&acc)[NT], float* red, float* __restrict__ out,
+                                             int64_t row0, int64_t rows, int64_t N) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (w > 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[((w - 1) * NT * 4 + t * 4 + v) * kWave + lane] = acc[t][v];
+  }
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float s = acc[t][v];
+        for (int q = 0; q < 3; ++q) s += red[(q * NT * 4 + t * 4 + v) * kWave + lane];
+        const int64_t row = row0 + 4 * (lane / 16) + v;
+        if (row < rows) out[row * N + t * 16 + (lane & 15)] = s;
+      }
+  }
+}
+
+// forward (TRANS = false) and dB (TRANS = true)
+template <int NT, bool BF, bool TRANS>
+__global__ __launch_bounds__(kTB) void bspmm_rows(const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ perm,
+                                                  const int32_t* __restrict__ other,
+                                                  const uint16_t* __restrict__ blocks,
+                                                  const float* __restrict__ scale,
+                                                  const uint16_t* __restrict__ opT, int64_t ldT,
+                                                  float* __restrict__ out, int64_t out_rows,
+                                                  int64_t nouter) {
+  extern __shared__ float red[];
+  const int64_t r = blockIdx.x;
+  if (r >= nouter) return;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int i = lane & 15, kg = lane >> 4;
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int k = ptr[r] + w; k < ptr[r + 1]; k += 4) {
+    const int b = TRANS ? perm[k] : k;
+    const int o = other[k];  // block column (forward) / block row (dB)
+    const u2 a = *reinterpret_cast<const u2*>(blocks + (int64_t)b * 256 + i * 16 + 4 * kg);
+    const float s = scale[b];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const u2 bo =
+          *reinterpret_cast<const u2*>(opT + (int64_t)(t * 16 + i) * ldT + (int64_t)o * 16 + 4 * kg);
+      const f4 p = mma16<BF>(a, bo, f4{0.f, 0.f, 0.f, 0.f});
+      acc[t] += s * p;
+    }
+  }
+  reduce_store<NT>(acc, red, out, r * 16, out_rows, NT * 16);
+}
+
+// dvals: dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T (K = N)
+template <int NT, bool BF>
+__global__ __launch_bounds__(kTB) void bspmm_dvals(const int32_t* __restrict__ browptr,
+                                                   const int32_t* __restrict__ bcol,
+                                                   const uint16_t* __restrict__ dC, int64_t M,
+                                                   const uint16_t* __restrict__ B, int64_t K,
+                                                   float* __restrict__ dvals, int64_t brows) {
+  constexpr int N = NT * 16, S = N / 32;
+  const int64_t br = blockIdx.x;
+  if (br >= brows) return;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int i = lane & 15, kg = lane >> 4;
+  u4 a[S];
+  const int64_t arow = br * 16 + i;
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    a[s] = arow < M ? *reinterpret_cast<const u4*>(dC + arow * N + 32 * s + 8 * kg) : u4{0, 0, 0, 0};
+  for (int b = browptr[br] + w; b < browptr[br + 1]; b += 4) {
+    const int64_t brow = (int64_t)bcol[b] * 16 + i;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const u4 bo =
+          brow < K ? *reinterpret_cast<const u4*>(B + brow * N + 32 * s + 8 * kg) : u4{0, 0, 0, 0};
+      acc = mma32<BF>(a[s], bo, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dvals[(int64_t)b * 256 + (4 * kg + v) * 16 + i] = acc[v];
+  }
+}
+
+template <bool BF, bool TRANS>
+static bsp_status launch_rows(bsp_bspmm_s* h, int64_t N, const uint16_t* opT, int64_t ldT,
+                              float* out, int64_t out_rows, hipStream_t s) {
+  const int64_t nouter = TRANS ? h->bcols : h->brows;
+  const int32_t* ptr = TRANS ? h->tcolptr.get() : h->browptr;
+  const int32_t* perm = TRANS ? h->tperm.get() : nullptr;
+  const int32_t* other = TRANS ? h->tbrow.get() : h->bcol;
+  const uint16_t* blocks = TRANS ? h->hvt.get() : h->hv.get();
+  if (nouter == 0) return BSP_OK;
+#define BSPMM_CASE(NT)                                                                           \
+  case NT:                                                                                       \
+    bspmm_rows<NT, BF, TRANS><<<nouter, kTB, 3 * NT * 4 * kWave * sizeof(float), s>>>(          \
+        ptr, perm, other, blocks, h->scale.get(), opT, ldT, out, out_rows, nouter);              \
+    break;
+  switch (N / 16) {
+    BSPMM_CASE(2)
+    BSPMM_CASE(4)
+    BSPMM_CASE(8)
+    BSPMM_CASE(16)
+    default:
+      return fail(BSP_ERR_UNSUPPORTED, "block SpMM needs N in {32, 64, 128, 256}");
+  }
+#undef BSPMM_CASE
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+template <bool BF>
+static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, const uint16_t* B,
+                               float* dvals, hipStream_t s) {
+  if (h->brows == 0) return BSP_OK;
+#define DV_CASE(NT)                                                                             \
+  case NT:                                                                                      \
+    bspmm_dvals<NT, BF><<<h->brows, kTB, 0, s>>>(h->browptr, h->bcol, dC, h->M, B, h->K, dvals, \
+                                                 h->brows);                                     \
+    break;
+  switch (N / 16) {
+    DV_CASE(2)
+    DV_CASE(4)
+    DV_CASE(8)
+    DV_CASE(16)
+    default:
+      return fail(BSP_ERR_UNSUPPORTED, "block SpMM needs N in {32, 64, 128, 256}");
+  }
+#undef DV_CASE
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+static bsp_status transposed(bsp_bspmm_s* h, const uint16_t* src, int64_t R, int64_t Cn,
+                             int64_t Rp, hipStream_t s) {
+  if (Cn * Rp > h->work_cap) {
+    BSP_HIP(hipStreamSynchronize(s));
+    BSP_TRY(h->work.alloc(Cn * Rp));
+    h->work_cap = Cn * Rp;
+  }
+  dim3 grid((unsigned)cdiv(Cn, 64), (unsigned)cdiv(Rp, 64));
+  transpose_u16<<<grid, kTB, 0, s>>>(src, R, Cn, h->work.get(), Rp);
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+}  // namespace
+}  // namespace bsp
+
+using namespace bsp;
+
+extern "C" {
+
+bsp_status bsp_bspmm_build(bsp_bsr A, bsp_dtype half_type, bsp_stream s, bsp_bspmm* out) {
+  BSP_REQUIRE(A && out, "null argument");
+  BSP_REQUIRE(half_type == BSP_F16 || half_type == BSP_BF16, "half type must be FP16 or BF16");
+  bsp_bsr_view v;
+  BSP_TRY(bsp_bsr_view_get(A, &v));
+  BSP_REQUIRE(v.tm == 16 && v.tk == 16, "block SpMM needs 16x16 blocks");
+  hipStream_t st = (hipStream_t)s;
+  auto h = std::make_unique<bsp_bspmm_s>();
+  h->M = v.nrows;
+  h->K = v.ncols;
+  h->brows = v.brows;
+  h->bcols = v.bcols;
+  h->nblocks = v.nblocks;
+  h->ht = half_type;
+  h->browptr = v.rowptr;
+  h->bcol = v.colidx;
+  const int64_t nb = v.nblocks;
+  BSP_TRY(h->scale.alloc(nb));
+  BSP_TRY(h->hv.alloc(nb * 256));
+  BSP_TRY(h->hvt.alloc(nb * 256));
+  if (nb > 0) {
+    if (half_type == BSP_BF16)
+      block_scale_convert<true><<<cdiv(nb * kWave, kTB), kTB, 0, st>>>(v.val, nb, h->scale.get(),
+                                                                      h->hv.get(), h->hvt.get());
+    else
+      block_scale_convert<false><<<cdiv(nb * kWave, kTB), kTB, 0, st>>>(v.val, nb, h->scale.get(),
+                                                                       h->hv.get(), h->hvt.get());
+    BSP_HIP(hipGetLastError());
+  }
+  // transposed block index: blocks ordered by (block column, block row)
+  DevBuf<int32_t> brow, idx, cnt;
+  DevBuf<int64_t> key, skey;
+  BSP_TRY(brow.alloc(nb));
+  BSP_TRY(idx.alloc(nb));
+  BSP_TRY(key.alloc(nb));
+  BSP_TRY(skey.alloc(nb));
+  BSP_TRY(cnt.alloc(v.bcols + 1));
+  BSP_TRY(h->tperm.alloc(nb));
+  BSP_TRY(h->tbrow.alloc(nb));
+  BSP_TRY(h->tcolptr.alloc(v.bcols + 1));
+  BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * (v.bcols + 1), st));
+  if (nb > 0) {
+    block_rows_of<<<cdiv(v.brows, kTB), kTB, 0, st>>>(v.rowptr, v.brows, brow.get());
+    tkeys<<<cdiv(nb, kTB), kTB, 0, st>>>(v.colidx, brow.get(), nb, v.brows, key.get(), idx.get(),
+                                         cnt.get());
+    BSP_HIP(hipGetLastError());
+    int bits = 1;
+    while (bits < 62 && (int64_t(1) << bits) <= v.bcols * v.brows) ++bits;
+    BSP_TRY(sort_pairs_i64key(key.get(), skey.get(), idx.get(), h->tperm.get(), nb, bits, st));
+    gather_i32<<<cdiv(nb, kTB), kTB, 0, st>>>(brow.get(), h->tperm.get(), nb, h->tbrow.get());
+    BSP_HIP(hipGetLastError());
+  }
+  BSP_TRY(exclusive_scan_i32(cnt.get(), h->tcolptr.get(), v.bcols + 1, st));
+  BSP_HIP(hipStreamSynchronize(st));
+  *out = h.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_bspmm_view_get(bsp_bspmm h, const float** scale, const uint16_t** hv,
+                              const uint16_t** hvt, int64_t* nblocks) {
+  BSP_REQUIRE(h, "null handle");
+  if (scale) *scale = h->scale.get();
+  if (hv) *hv = h->hv.get();
+  if (hvt) *hvt = h->hvt.get();
+  if (nblocks) *nblocks = h->nblocks;
+  return BSP_OK;
+}
+
+bsp_status bsp_bspmm_fwd(bsp_bspmm h, const void* B, int64_t N, float* Cout, bsp_stream s) {
+  BSP_REQUIRE(h && B && Cout, "null argument");
+  hipStream_t st = (hipStream_t)s;
+  const int64_t Kp = h->bcols * 16;
+  // B^T (N x Kp), k-contiguous for the MFMA B operand
+  BSP_TRY(transposed(h, static_cast<const uint16_t*>(B), h->K, N, Kp, st));
+  if (h->ht == BSP_BF16)
+    return launch_rows<true, false>(h, N, h->work.get(), Kp, Cout, h->M, st);
+  return launch_rows<false, false>(h, N, h->work.get(), Kp, Cout, h->M, st);
+}
+
+bsp_status bsp_bspmm_bwd(bsp_bspmm h, const void* B, const void* dC, int64_t N, float* dB,
+                         float* dvals, bsp_stream s) {
+  BSP_REQUIRE(h && B && dC, "null argument");
+  hipStream_t st = (hipStream_t)s;
+  const bool bf = h->ht == BSP_BF16;
+  if (dvals) {
+    if (bf)
+      BSP_TRY(launch_dvals<true>(h, N, (const uint16_t*)dC, (const uint16_t*)B, dvals, st));
+    else
+      BSP_TRY(launch_dvals<false>(h, N, (const uint16_t*)dC, (const uint16_t*)B, dvals, st));
+  }
+  if (dB) {
+    const int64_t Mp = h->brows * 16;
+    BSP_TRY(transposed(h, static_cast<const uint16_t*>(dC), h->M, N, Mp, st));
+    if (bf)
+      BSP_TRY((launch_rows<true, true>(h, N, h->work.get(), Mp, dB, h->K, st)));
+    else
+      BSP_TRY((launch_rows<false, true>(h, N, h->work.get(), Mp, dB, h->K, st)));
+  }
+  return BSP_OK;
+}
+
+bsp_status bsp_bspmm_destroy(bsp_bspmm h) {
+  delete h;
+  return BSP_OK;
+}
+
+}  // extern "C"

@@ -238,3 +238,47 @@ class Dasp:
                 self._h = None
         except Exception:
             pass
+
+
+class BlockSpmm:
+    """Block-sparse SpMM on FP16/BF16 MFMA over a 16x16 Bsr (bsp_bspmm_*)."""
+
+    def __init__(self, bsr: "Bsr", half: torch.dtype = torch.float16):
+        assert bsr.view.tm == 16 and bsr.view.tk == 16
+        self.bsr = bsr
+        self.half = half
+        self.M, self.K = bsr.view.nrows, bsr.view.ncols
+        self.nblocks = bsr.view.nblocks
+        h = C.c_void_p()
+        lib().bsp_bspmm_build(bsr._h, BF16 if half == torch.bfloat16 else F16, stream_handle(),
+                              C.byref(h))
+        self._h = h
+
+    def arrays(self) -> dict:
+        sc, hv, hvt, nb = C.c_void_p(), C.c_void_p(), C.c_void_p(), i64()
+        lib().bsp_bspmm_view_get(self._h, C.byref(sc), C.byref(hv), C.byref(hvt), C.byref(nb))
+        n = nb.value
+        return dict(scale=_to_host(sc.value, n, np.float32), hv=_to_host(hv.value, n * 256, np.uint16),
+                    hvt=_to_host(hvt.value, n * 256, np.uint16))
+
+    def fwd(self, B: torch.Tensor) -> torch.Tensor:
+        assert B.dtype == self.half and B.is_contiguous() and B.shape[0] == self.K
+        Cm = torch.empty(self.M, B.shape[1], dtype=torch.float32, device=B.device)
+        lib().bsp_bspmm_fwd(self._h, ptr(B), B.shape[1], ptr(Cm), stream_handle())
+        return Cm
+
+    def bwd(self, B: torch.Tensor, dC: torch.Tensor):
+        assert B.dtype == self.half and dC.dtype == self.half
+        N = B.shape[1]
+        dB = torch.empty(self.K, N, dtype=torch.float32, device=B.device)
+        dv = torch.empty(self.nblocks, 16, 16, dtype=torch.float32, device=B.device)
+        lib().bsp_bspmm_bwd(self._h, ptr(B), ptr(dC), N, ptr(dB), ptr(dv), stream_handle())
+        return dB, dv
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_bspmm_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -248,6 +248,27 @@ bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype d
 bsp_status bsp_spmm_plan_destroy(bsp_spmm_plan p);
 
 /* ------------------------------------------------------------------------ */
+/* Block-sparse SpMM on FP16/BF16 MFMA, FP32 accumulation. Replaces the
+ * kernel of smatSpmmfp16 (src/smat.cu:22-91, 103-202): 16x16 dense blocks,
+ * each scaled by a power of two computed in FP64 on the GPU (values fill
+ * (-1, 1) before the one rounding to FP16/BF16), plus the backward.
+ * Dense operands are row-major FP16/BF16 (the handle's type): B is K x N,
+ * dC is M x N; outputs are FP32 row-major (C: M x N, dB: K x N) and dvals
+ * (nblocks x 16 x 16, the gradient with respect to the unscaled block
+ * values, block order of the BSR). N is 32, 64, 128 or 256. */
+typedef struct bsp_bspmm_s* bsp_bspmm;
+/* A: a 16x16 BSR (bsp_bsr_build with tm = tk = 16) that must outlive h. */
+bsp_status bsp_bspmm_build(bsp_bsr A, bsp_dtype half_type, bsp_stream s, bsp_bspmm* out);
+bsp_status bsp_bspmm_view_get(bsp_bspmm h, const float** scale, const uint16_t** hv,
+                              const uint16_t** hvt, int64_t* nblocks);
+/* C = A * B */
+bsp_status bsp_bspmm_fwd(bsp_bspmm h, const void* B, int64_t N, float* C, bsp_stream s);
+/* dB = A^T * dC and dvals = dC * B^T on A's blocks (either output may be NULL) */
+bsp_status bsp_bspmm_bwd(bsp_bspmm h, const void* B, const void* dC, int64_t N, float* dB,
+                         float* dvals, bsp_stream s);
+bsp_status bsp_bspmm_destroy(bsp_bspmm h);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side synthetic matrices (generators; see benchsp_amd/csrc/gen.cpp). */
 typedef struct bsp_host_csr_s* bsp_host_csr;
 bsp_status bsp_gen_uniform(int64_t n, uint64_t seed, double* out);
