@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: FP64 SpMV effective HBM bandwidth on one MI355X.
+"""Benchmark on one MI355X: FP64 SpMV effective HBM bandwidth (headline
+value) and FP16 block-SpMM forward+backward GFLOP/s against a plain FP32 CSR
+SpMM (secondary leg).
 
-Workload (one step): y = A*x with the library's adaptive CSR kernel on
+SpMV step: y = A*x with the library's adaptive CSR kernel on
   (a) the 27-point stencil on a 64^3 grid (n=262,144, nnz=6,859,000) and
   (b) the power-law matrix (n=2^20, lognormal row lengths mean 20, sigma 1.2,
       cap 5000, ~21M nnz),
 synthetic data from the library's deterministic generators, resident in HBM.
-Each matrix is replicated so that the copies together exceed 512 MB and every
+Each matrix is replicated so the copies together exceed 512 MB and every
 launch reads a copy that is not in the 256 MB Infinity Cache (cold HBM).
-
 value = sum of CSR-equivalent algorithmic bytes (12*nnz + 4*(n+1) + 8*ncols
 + 8*n per SpMV) over the timed steps / timed wall time, in GB/s.
+
+SpMM leg: A = 16384^2 with 52,429 dense 16x16 blocks (5%), B and dC
+16384x128 N(0,1). One step = forward C = A*B + backward (dB = A^T*dC and
+dvals = dC*B^T on A's blocks) = 3 * 2*nnz*N FLOP, on FP16 (and BF16) MFMA
+with FP32 accumulation, against the same three products with the library's
+CSR SpMM in FP32.
 
 Prints one JSON line. `--gpus N` with N > 1 is refused: this build measures
 one GPU.
@@ -39,16 +46,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--method", default="adaptive")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-spmm", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--warm", action="store_true", help="single copy (cache-warm) timing")
     return ap.parse_args()
 
 
-def _copies(A, warm: bool) -> int:
-    if warm:
-        return 1
-    return max(2, math.ceil(2 * MALL_BYTES / A.csr_bytes()))
+def _copies(nbytes: int, warm: bool) -> int:
+    return 1 if warm else max(2, math.ceil(2 * MALL_BYTES / nbytes))
 
 
 def cpu_baseline(mats, xs, seconds: float):
@@ -66,9 +72,142 @@ def cpu_baseline(mats, xs, seconds: float):
             t_total += time.perf_counter() - t0
             nbytes += A.csr_bytes()
         reps += 1
-    return {"value": nbytes / t_total / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} x (SpMV 27pt-64^3 + SpMV powerlaw-1M), {t_total:.1f} s, "
-                      f"{threads} threads"}
+    return {"value": round(nbytes / t_total / 1e9, 2), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{reps} x (SpMV 27pt-64^3 + SpMV powerlaw-1M), {t_total:.1f} s on "
+                      f"{threads} threads, same CSR-equivalent bytes as the GPU value"}
+
+
+def spmv_leg(args, dev):
+    import numpy as np
+    import torch
+    from benchsp_amd import host
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+
+    mats = [("27pt-64^3", host.stencil27_64()), ("powerlaw-1M", host.powerlaw_1m())]
+    xs = [host.uniform(A.ncols, seed=100 + i) for i, (_, A) in enumerate(mats)]
+    work = []
+    for (name, A), x in zip(mats, xs):
+        ncopy = _copies(A.csr_bytes(), args.warm)
+        ops = [CsrSpmv(DeviceCsr.from_host(A, dev), args.method) for _ in range(ncopy)]
+        work.append(dict(name=name, A=A, ops=ops, x=torch.from_numpy(x).to(dev),
+                         y=torch.empty(A.nrows, dtype=torch.float64, device=dev),
+                         bytes=A.csr_bytes(), ncopy=ncopy, seg=ops[0].segments()))
+    torch.cuda.synchronize()
+
+    def step(i, evs=None):
+        for wi, w in enumerate(work):
+            if evs is not None:
+                evs[wi][0].record()
+            w["ops"][i % w["ncopy"]](w["x"], w["y"])
+            if evs is not None:
+                evs[wi][1].record()
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # per-launch kernel durations, HIP events on the launch (current) stream
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in work]
+          for _ in range(args.steps)]
+    for i in range(args.steps):
+        step(args.warmup + i, ev[i])
+    torch.cuda.synchronize()
+    per, tot_b, tot_t = {}, 0.0, 0.0
+    for wi, w in enumerate(work):
+        ms = float(np.mean([ev[i][wi][0].elapsed_time(ev[i][wi][1]) for i in range(args.steps)]))
+        tot_b += w["bytes"]
+        tot_t += ms
+        gbs = w["bytes"] / (ms * 1e-3) / 1e9
+        per[w["name"]] = {"nrows": w["A"].nrows, "nnz": w["A"].nnz, "bytes": w["bytes"],
+                          "us": round(ms * 1e3, 2), "eff_GBs": round(gbs, 1),
+                          "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
+                          "copies": w["ncopy"], "column_segments": w["seg"][1]}
+    step_bytes = sum(w["bytes"] for w in work)
+    return dict(value=step_bytes * args.steps / wall / 1e9, wall=wall, per=per,
+                achieved=tot_b / (tot_t * 1e-3) / 1e9,
+                mats=[A for _, A in mats], xs=xs)
+
+
+def spmm_leg(args, dev):
+    import numpy as np
+    import torch
+    from benchsp_amd import host
+    from benchsp_amd.autograd import SparsePattern
+    from benchsp_amd.device import Bsr, BlockSpmm, DeviceCsr
+
+    A = host.blocksparse_16k()
+    N = 128
+    flop = 2.0 * A.nnz * N  # per product
+    rng_b = host.normal(A.ncols * N, seed=200).reshape(A.ncols, N).astype(np.float32)
+    rng_d = host.normal(A.nrows * N, seed=201).reshape(A.nrows, N).astype(np.float32)
+    out = {"workload": "A 16384^2, 52,429 dense 16x16 blocks (5%), B/dC 16384x128 N(0,1); "
+                       "step = C=A*B, dB=A^T*dC, dvals=dC*B^T",
+           "flop_per_step": 3 * flop}
+
+    def timed(step_fn, n_ops):
+        for i in range(args.warmup):
+            step_fn(i)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(args.steps):
+            step_fn(args.warmup + i)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps  # ms per step
+
+    dA = DeviceCsr.from_host(A, dev)
+    bsr_bytes = A.nnz * 2 * 2  # hv + hvt per copy
+    for half in ("float16", "bfloat16"):
+        td = getattr(torch, half)
+        ncopy = _copies(bsr_bytes, args.warm)
+        bsrs = [Bsr(dA, 16, 16) for _ in range(ncopy)]
+        ops = [BlockSpmm(b, td) for b in bsrs]
+        B = torch.from_numpy(rng_b).to(dev).to(td)
+        dC = torch.from_numpy(rng_d).to(dev).to(td)
+
+        def step(i):
+            op = ops[i % ncopy]
+            op.fwd(B)
+            op.bwd(B, dC)
+        ms = timed(step, 3)
+        out[f"block_{half}_GFLOPs"] = round(3 * flop / (ms * 1e-3) / 1e9, 1)
+        out[f"block_{half}_ms_per_step"] = round(ms, 4)
+        del ops, bsrs
+    # plain FP32 CSR SpMM on the same matrix and operands
+    ncopy = _copies(A.nnz * 8, args.warm)
+    pats = [SparsePattern.from_host(A, dev) for _ in range(ncopy)]
+    vals = [torch.from_numpy(A.val.astype(np.float32)).to(dev) for _ in range(ncopy)]
+    X = torch.from_numpy(rng_b).to(dev)
+    dY = torch.from_numpy(rng_d).to(dev)
+
+    def step32(i):
+        p, v = pats[i % ncopy], vals[i % ncopy]
+        p.spmm(v, X)
+        p.spmm_t(v, dY)
+        p.sddmm(dY, X)
+    ms = timed(step32, 3)
+    out["csr_fp32_GFLOPs"] = round(3 * flop / (ms * 1e-3) / 1e9, 1)
+    out["csr_fp32_ms_per_step"] = round(ms, 4)
+    out["fp16_over_fp32"] = round(out["block_float16_GFLOPs"] / out["csr_fp32_GFLOPs"], 2)
+    return out
+
+
+def measured_traffic():
+    """HBM bytes the adaptive kernel moves per step (27-pt launch + both
+    power-law segment launches), from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement scaled by a plain copy (profiles/)."""
+    p = os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return int(sum(v["hbm_bytes"] for v in d["launches"].values()))
 
 
 def main():
@@ -77,77 +216,19 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU sharding plan is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
-    from benchsp_amd import host
-    from benchsp_amd.device import CsrSpmv, DeviceCsr
-
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    mats = [("27pt-64^3", host.stencil27_64()), ("powerlaw-1M", host.powerlaw_1m())]
-    xs = [host.uniform(A.ncols, seed=100 + i) for i, (_, A) in enumerate(mats)]
 
-    work = []  # per matrix: list of (op), x, y, bytes
-    for (name, A), x in zip(mats, xs):
-        ncopy = _copies(A, args.warm)
-        ops = [CsrSpmv(DeviceCsr.from_host(A, dev), args.method) for _ in range(ncopy)]
-        xd = torch.from_numpy(x).to(dev)
-        yd = torch.empty(A.nrows, dtype=torch.float64, device=dev)
-        work.append(dict(name=name, A=A, ops=ops, x=xd, y=yd, bytes=A.csr_bytes(), ncopy=ncopy))
-    torch.cuda.synchronize()
-
-    def step(i, evs=None):
-        for wi, w in enumerate(work):
-            op = w["ops"][i % w["ncopy"]]
-            if evs is not None:
-                evs[wi][0].record()
-            op(w["x"], w["y"])
-            if evs is not None:
-                evs[wi][1].record()
-
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
-
-    # timed region: wall clock over K steps (bracketed by synchronize)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(args.warmup + i)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-
-    # per-launch kernel durations with HIP events on the launch stream
-    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in work]
-          for _ in range(args.steps)]
-    for i in range(args.steps):
-        step(args.warmup + i, ev[i])
-    torch.cuda.synchronize()
-    kern_ms = [[ev[i][wi][0].elapsed_time(ev[i][wi][1]) for i in range(args.steps)]
-               for wi in range(len(work))]
-
-    step_bytes = sum(w["bytes"] for w in work)
-    value = step_bytes * args.steps / wall / 1e9
-    per = {}
-    tot_b, tot_t = 0.0, 0.0
-    for wi, w in enumerate(work):
-        ms = float(np.mean(kern_ms[wi]))
-        tot_b += w["bytes"]
-        tot_t += ms
-        per[w["name"]] = {"nrows": w["A"].nrows, "nnz": w["A"].nnz, "bytes": w["bytes"],
-                          "kernel_us": round(ms * 1e3, 2),
-                          "eff_GBs": round(w["bytes"] / (ms * 1e-3) / 1e9, 1),
-                          "pct_hbm_peak": round(100 * w["bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
-                          "copies": w["ncopy"]}
-    achieved = tot_b / (tot_t * 1e-3) / 1e9
-
+    sp = spmv_leg(args, dev)
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": round(sp["value"], 1),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(sp["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -156,14 +237,20 @@ def main():
         "config": {"workload": "FP64 CSR SpMV: 27-pt stencil 64^3 + power-law n=2^20 (lognormal "
                                "mean 20, sigma 1.2, cap 5000)",
                    "method": args.method, "cache": "warm" if args.warm else "cold (rotated copies)",
-                   "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 1),
-                   "matrices": per},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                   "pct_hbm_peak": round(100 * sp["value"] / HBM_PEAK_GBS, 1),
+                   "matrices": sp["per"]},
+        "roofline": {"bound": "hbm", "achieved": round(sp["achieved"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(sp["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic() if args.method == "adaptive" else None,
+                     "traffic_basis": "HBM bytes per step (all launches of the kernel), "
+                                      "algorithmic bytes per step = "
+                                      f"{sum(v['bytes'] for v in sp['per'].values())}",
                      "kernel": "csr_adaptive_f64" if args.method == "adaptive" else args.method},
     }
+    if not args.no_spmm:
+        out["spmm_fwd_bwd"] = spmm_leg(args, dev)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline([A for _, A in mats], xs, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(sp["mats"], sp["xs"], args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
