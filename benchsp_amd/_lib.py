@@ -26,7 +26,7 @@ class BspError(RuntimeError):
 
 class CsrOptions(C.Structure):
     """struct bsp_csr_options (include/bsp.h)."""
-    _fields_ = [("seg_cols", i64), ("chunk", i32), ("reserved0", i32), ("reserved", i64 * 6)]
+    _fields_ = [("seg_cols", i64), ("chunk", i32), ("seg_mode", i32), ("reserved", i64 * 6)]
 
 
 class CsrDesc(C.Structure):

@@ -34,6 +34,10 @@ struct bsp_spmv_plan_s {
   bsp::DevBuf<double> seg_val;
   int64_t seg_width = 0;         // 0: one segment (the input arrays as they are)
   int chunk = 1024;              // adaptive block window (rows + entries)
+  bool xcd_mode = false;         // all segments in one XCD-pinned launch
+  int64_t xcd_grid = 0;
+  bsp::DevBuf<double> part;      // per-segment partial y (xcd_mode)
+  bsp::DevBuf<char> segtab;      // device table of segment descriptors (xcd_mode)
 };
 
 namespace bsp {

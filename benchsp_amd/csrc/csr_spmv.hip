@@ -71,17 +71,17 @@ __global__ void set_value_i32(int32_t* p, int32_t v) { *p = v; }
 // one 8-byte load each ({row, entry offset} pairs built with the plan), so a
 // workgroup's stream loads start after a single round trip.
 template <int CHUNK, bool ACCUM>
-__global__ __launch_bounds__(kWG) void csr_adaptive_f64(
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-    const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
-    const int2* __restrict__ blk, int64_t nblk) {
+__device__ __forceinline__ void adaptive_block(const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ col,
+                                               const double* __restrict__ val,
+                                               const double* __restrict__ x, double* __restrict__ y,
+                                               const int2* __restrict__ blk, int64_t b) {
   using C = Ad<CHUNK>;
   __shared__ double prod[C::MAXNNZ];
   __shared__ int32_t rptr[CHUNK + 1];
   __shared__ double yb[ACCUM ? CHUNK : 1];
   __shared__ double red[kWG / kWave];
   const int tid = threadIdx.x;
-  const int64_t b = xcd_remap(blockIdx.x, nblk);
   const int2 s0 = blk[b], s1 = blk[b + 1];
   const int r0 = s0.x, p0 = s0.y, r1 = s1.x, p1 = s1.y;
   const int nnz = p1 - p0, nrows = r1 - r0;
@@ -153,6 +153,49 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
         st_stream(&y[r0 + rr], yb[rr] + s);
     }
   }
+}
+
+template <int CHUNK, bool ACCUM>
+__global__ __launch_bounds__(kWG) void csr_adaptive_f64(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
+    const int2* __restrict__ blk, int64_t nblk) {
+  adaptive_block<CHUNK, ACCUM>(rowptr, col, val, x, y, blk, xcd_remap(blockIdx.x, nblk));
+}
+
+// All column segments in one launch, each pinned to a set of XCDs:
+// workgroup i runs on XCD i % 8, XCD k serves segment k % nseg, so an XCD
+// only ever gathers from its own segment's slice of x, which stays in its
+// L2. Each segment writes its own partial y; sum_partials adds them in
+// segment order.
+struct SegDesc {
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* val;
+  const int2* blk;
+  int64_t nblk;
+  double* out;
+};
+
+template <int CHUNK>
+__global__ __launch_bounds__(kWG) void csr_adaptive_xcd_f64(const SegDesc* __restrict__ segs,
+                                                            int nseg, const double* __restrict__ x) {
+  const int xcd = blockIdx.x % 8;
+  const int64_t local = blockIdx.x / 8;
+  const int s = xcd % nseg, q = xcd / nseg, reps = 8 / nseg;
+  const SegDesc d = segs[s];
+  const int64_t b = local * reps + q;
+  if (b >= d.nblk) return;
+  adaptive_block<CHUNK, false>(d.rowptr, d.col, d.val, x, d.out, d.blk, b);
+}
+
+__global__ void sum_partials(const double* __restrict__ part, int nseg, int64_t n,
+                             double* __restrict__ y) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < nseg; ++k) s += ld_stream(&part[k * n + r]);
+  y[r] = s;
 }
 
 template <int CHUNK>
@@ -313,6 +356,7 @@ static bsp_status stable_partition(const int32_t* flags, int64_t n, DevBuf<int32
 // Column-segment support for the adaptive layout ----------------------------
 
 constexpr int64_t kDefaultSegCols = int64_t(1) << 19;  // 4 MB of FP64 x (one XCD's L2)
+constexpr int64_t kXcdSegCols = int64_t(1) << 18;      // 2 MB slices when XCDs are pinned
 
 // Counts entries farther than `half` columns from the (scaled) diagonal and
 // rows whose columns are not strictly ascending.
@@ -400,6 +444,13 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     BSP_TRY(fetch(far.get(), &nfar, s));
     BSP_TRY(fetch(unsorted.get(), &uns, s));
     W = (!uns && (int64_t)nfar * 4 > A.nnz) ? kDefaultSegCols : 0;
+    if (W > 0 && p->opt.seg_mode == 0) {
+      // one launch, one segment per XCD group: the smallest power-of-two
+      // segment count (<= 8) whose x slices are at most 2 MB
+      int64_t S = 1;
+      while (S < 8 && cdiv(A.ncols, S) > kXcdSegCols) S *= 2;
+      if (cdiv(A.ncols, S) <= kXcdSegCols) W = cdiv(A.ncols, S);
+    }
   } else if (W > 0 && A.nnz > 0) {
     DevBuf<unsigned long long> far;
     DevBuf<int> unsorted;
@@ -463,11 +514,34 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     }
     BSP_HIP(hipStreamSynchronize(s));
   }
-  p->chunk = p->opt.chunk == 2048 ? 2048 : 1024;
+  // measured on gfx950: 1024-unit blocks for one segment, 2048 for segmented
+  p->chunk = p->opt.chunk == 2048 || p->opt.chunk == 1024 ? p->opt.chunk : (nseg > 1 ? 2048 : 1024);
   p->nblk = 0;
   for (auto& g : p->seg) {
     BSP_TRY(adaptive_partition(g, n, p->chunk, s));
     p->nblk += g.nblk;
+  }
+  // XCD-pinned single launch for 2, 4 or 8 segments (unless sequential passes
+  // were asked for)
+  p->xcd_mode = false;
+  if (nseg == 2 || nseg == 4 || nseg == 8) {
+    if (p->opt.seg_mode == 0) {
+      p->xcd_mode = true;
+      BSP_TRY(p->part.alloc(nseg * n));
+      std::vector<SegDesc> tab(nseg);
+      int64_t most = 0;
+      for (int64_t k = 0; k < nseg; ++k) {
+        const CsrSegment& g = p->seg[k];
+        tab[k] = SegDesc{g.rowptr, g.col, g.val, reinterpret_cast<const int2*>(g.blk.get()),
+                         g.nblk, p->part.get() + k * n};
+        most = std::max(most, cdiv(g.nblk, 8 / nseg));
+      }
+      p->xcd_grid = 8 * most;
+      BSP_TRY(p->segtab.alloc(nseg * sizeof(SegDesc)));
+      BSP_HIP(hipMemcpyAsync(p->segtab.get(), tab.data(), nseg * sizeof(SegDesc),
+                             hipMemcpyHostToDevice, s));
+      BSP_HIP(hipStreamSynchronize(s));
+    }
   }
   return BSP_OK;
 }
@@ -526,6 +600,17 @@ bsp_status csr_plan_run(const CsrPlan* p, const double* x, double* y, hipStream_
   const auto* v = static_cast<const double*>(A.val);
   switch (p->method) {
     case BSP_CSR_ADAPTIVE:
+      if (p->xcd_mode) {
+        const int nseg = (int)p->seg.size();
+        const auto* tab = reinterpret_cast<const SegDesc*>(p->segtab.get());
+        if (p->chunk == 2048)
+          csr_adaptive_xcd_f64<2048><<<p->xcd_grid, kWG, 0, s>>>(tab, nseg, x);
+        else
+          csr_adaptive_xcd_f64<1024><<<p->xcd_grid, kWG, 0, s>>>(tab, nseg, x);
+        BSP_HIP(hipGetLastError());
+        sum_partials<<<cdiv(n, kWG), kWG, 0, s>>>(p->part.get(), nseg, n, y);
+        break;
+      }
       for (size_t k = 0; k < p->seg.size(); ++k) {
         const CsrSegment& g = p->seg[k];
         if (g.nblk == 0) continue;

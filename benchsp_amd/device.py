@@ -56,11 +56,11 @@ class CsrSpmv:
     """y = A*x with a per-matrix schedule built on the GPU (bsp_csr_plan_create)."""
 
     def __init__(self, A: DeviceCsr, method: str | int = "adaptive", seg_cols: int = 0,
-                 chunk: int = 0):
+                 chunk: int = 0, seg_mode: int = 0):
         self.A = A
         self.method = METHODS[method] if isinstance(method, str) else int(method)
         self._desc = A.desc()
-        self._opt = CsrOptions(seg_cols, chunk)
+        self._opt = CsrOptions(seg_cols, chunk, seg_mode)
         h = C.c_void_p()
         lib().bsp_csr_plan_create_ex(C.byref(self._desc), self.method, C.byref(self._opt),
                                      stream_handle(), C.byref(h))

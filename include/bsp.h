@@ -88,9 +88,14 @@ typedef struct bsp_spmv_plan_s* bsp_spmv_plan;
  *   diagonal and rows are sorted), < 0 = never segment. */
 typedef struct bsp_csr_options {
   int64_t seg_cols;
-  /* adaptive block window in rows+entries: 1024 or 2048 (0 = 1024) */
+  /* adaptive block window in rows+entries: 1024 or 2048 (0 = 1024 for an
+   * unsegmented plan, 2048 for a segmented one) */
   int32_t chunk;
-  int32_t reserved0;
+  /* segmented plans: 0 = automatic (2, 4 or 8 segments run in one launch,
+   * each pinned to a group of XCDs, partial results added by a second
+   * kernel; automatic width then targets 2 MB of x per segment),
+   * 1 = one launch per segment, later ones accumulating into y */
+  int32_t seg_mode;
   int64_t reserved[6];
 } bsp_csr_options;
 

@@ -90,10 +90,10 @@ def test_all_empty_rows(method):
     assert np.array_equal(y, np.zeros(1000))
 
 
-def _run_seg(A, x, seg_cols):
+def _run_seg(A, x, seg_cols, seg_mode=0):
     import torch
     from benchsp_amd.device import CsrSpmv, DeviceCsr
-    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=seg_cols)
+    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=seg_cols, seg_mode=seg_mode)
     y = op(torch.from_numpy(x).cuda())
     torch.cuda.synchronize()
     return y.cpu().numpy(), op.segments()
@@ -103,7 +103,7 @@ def test_adaptive_segments_powerlaw_auto(oracle, bsp_host):
     A = bsp_host.powerlaw_1m()
     x = bsp_host.uniform(A.ncols, seed=13)
     y, (w, k) = _run_seg(A, x, 0)
-    assert w == 1 << 19 and k == 2  # scattered columns: segmented automatically
+    assert w == 1 << 18 and k == 4  # scattered columns: 4 XCD-pinned segments of 2 MB of x
     assert_spmv_close(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x), abs_spmv(oracle, A, x),
                       what="powerlaw segmented")
 
@@ -115,8 +115,9 @@ def test_adaptive_stencil_not_segmented(bsp_host):
     assert CsrSpmv(DeviceCsr.from_host(A), "adaptive").segments() == (0, 1)
 
 
-@pytest.mark.parametrize("seg_cols", [1000, 4097])
-def test_adaptive_forced_segments(oracle, bsp_host, seg_cols):
+@pytest.mark.parametrize("seg_cols,seg_mode", [(1000, 0), (4097, 0), (4500, 0), (2250, 0),
+                                               (1125, 0), (4500, 1), (2250, 1)])
+def test_adaptive_forced_segments(oracle, bsp_host, seg_cols, seg_mode):
     rng = np.random.default_rng(9)
     from benchsp_amd.host import Csr
     lens = np.array([0, 3, 0, 5000, 1, 129, 2047, 2049, 17] + [0] * 40 + [1] * 3000 + [40] * 500)
@@ -126,13 +127,13 @@ def test_adaptive_forced_segments(oracle, bsp_host, seg_cols):
     col = np.concatenate([np.sort(rng.choice(ncols, L, replace=False)) for L in lens]).astype(np.int32)
     A = Csr(n, ncols, rp, col, rng.uniform(-1, 1, rp[-1]))
     x = rng.uniform(-1, 1, ncols)
-    y, (w, k) = _run_seg(A, x, seg_cols)
+    y, (w, k) = _run_seg(A, x, seg_cols, seg_mode)
     assert w == seg_cols and k == -(-ncols // seg_cols)
     assert_spmv_close(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x), abs_spmv(oracle, A, x),
                       what=f"forced segments {seg_cols}")
     S = bsp_host.stencil3d(20, 21, 22, 27, seed=5)
     xs = bsp_host.uniform(S.ncols, 3)
-    y, _ = _run_seg(S, xs, seg_cols)
+    y, _ = _run_seg(S, xs, seg_cols, seg_mode)
     assert_spmv_close(y, oracle.spmv_f64(S.rowptr, S.col, S.val, xs), abs_spmv(oracle, S, xs),
                       what="stencil forced segments")
 
