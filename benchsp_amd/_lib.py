@@ -35,6 +35,21 @@ class CsrDesc(C.Structure):
                 ("val", vp), ("dtype", C.c_int)]
 
 
+class CSRFormatF64(C.Structure):
+    """struct bsp_CSRFormat_f64: CSRFormat<double, int32_t> (include/matrixFormat.hpp)."""
+    _fields_ = [("rowPtr", vp), ("colIdx", vp), ("values", vp), ("row", i32), ("col", i32),
+                ("nnz", i32)]
+
+
+class DbsrMat(C.Structure):
+    """struct bsp_dbsrMat: dbsrMat (include/matrixFormat.hpp:86-106)."""
+    _fields_ = [("row", C.c_int), ("col", C.c_int), ("nnz", C.c_int), ("blc_row", C.c_int),
+                ("blc_col", C.c_int), ("blc_num", C.c_int), ("stand", f64), ("avg_nnz", f64),
+                ("blcPtr", vp), ("colIdx", vp), ("blcVal", vp), ("blcMap", vp),
+                ("blocknnz", i32), ("warpnum", i32), ("rowPtrbyWarp", vp), ("rowIdxbyWarp", vp),
+                ("recorderMap", vp), ("prefix", vp)]
+
+
 class BsrView(C.Structure):
     """struct bsp_bsr_view."""
     _fields_ = [("nrows", i64), ("ncols", i64), ("nnz", i64), ("brows", i64), ("bcols", i64),
@@ -65,8 +80,27 @@ class DaspView(C.Structure):
                                    "reg_cid", "irreg_val", "irreg_cid")])
 
 
+VOID = "void"  # restype marker for the reference's void functions
+
 # name -> (restype, argtypes); restype None means bsp_status (checked).
 _SIGS: dict[str, tuple] = {
+    "bsp_csr_spmv_f16": (None, [vp, vp, vp, vp]),
+    "bsp_rocsparse_plan_create": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(vp)]),
+    "bsp_rocsparse_spmv": (None, [vp, vp, vp, vp]),
+    "bsp_rocsparse_plan_destroy": (None, [vp]),
+    "bsp_dbsr_from_host": (None, [C.POINTER(DbsrMat), vp, C.POINTER(vp)]),
+    "bsp_csrTodbsrWithRecorder": (C.c_int, [C.POINTER(CSRFormatF64), C.POINTER(DbsrMat)]),
+    "bsp_dbsrMat_free": (VOID, [C.POINTER(DbsrMat)]),
+    "bsp_amgT_spmv_fp64_dbsr": (C.c_int, [C.POINTER(DbsrMat), vp, vp, i32, i32, f64p]),
+    "bsp_amgT_spmv_fp64_dbsr_balance": (C.c_int, [C.POINTER(DbsrMat), vp, vp, i32, i32, f64p]),
+    "bsp_amgT_spmv_fp64": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, C.c_int, f64p]),
+    "bsp_bcsr_spmv_fp64": (VOID, [vp, vp, vp, vp, vp, i64, i64, i64, C.c_int]),
+    "bsp_spmv_all": (VOID, [C.c_char_p, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                            C.c_int, f64, C.c_int, f64p]),
+    "bsp_MySpmvWithPrefp64": (VOID, [vp, vp, i64, i64, i64, vp, vp, vp]),
+    "bsp_MySpmvWithPrefp16": (VOID, [vp, vp, i32, i32, i32, vp, vp, vp]),
+    "bsp_smatSpmmfp16": (VOID, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
+    "bsp_cusparse_spmv_fp64": (C.c_int, [vp, vp, vp, vp, vp, i64, i64, i64, C.c_int, f64p]),
     "bsp_bspmm_build": (None, [vp, C.c_int, vp, C.POINTER(vp)]),
     "bsp_bspmm_view_get": (None, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i64p]),
     "bsp_bspmm_fwd": (None, [vp, vp, i64, vp, vp]),
@@ -137,7 +171,7 @@ class _Lib:
         for name, (res, args) in _SIGS.items():
             fn = getattr(cdll, name)
             fn.argtypes = args
-            fn.restype = C.c_int if res is None else res
+            fn.restype = C.c_int if res is None else (None if res is VOID else res)
             setattr(self, name, _Checked(cdll, name, fn) if res is None else fn)
 
     def declare(self, name, restype, argtypes):

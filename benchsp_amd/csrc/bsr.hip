@@ -599,4 +599,54 @@ bsp_status bsp_dbsr_destroy(bsp_dbsr h) {
   return BSP_OK;
 }
 
+bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out) {
+  BSP_REQUIRE(m && out, "null argument");
+  BSP_REQUIRE(m->row >= 0 && m->blc_row == (m->row + 3) / 4 && m->blocknnz >= 0 &&
+                  m->blocknnz % 16 == 0 && m->warpnum >= 0,
+              "inconsistent dbsrMat sizes");
+  BSP_REQUIRE(m->blcPtr && (m->row == 0 || m->recorderMap) &&
+                  (m->blocknnz == 0 || (m->colIdx && m->blcVal && m->blcMap)),
+              "null dbsrMat array");
+  hipStream_t st = (hipStream_t)s;
+  auto h = std::make_unique<bsp_dbsr_s>();
+  bsp_dbsr_view& v = h->v;
+  v.nrows = m->row;
+  v.ncols = m->col;
+  v.nnz = m->nnz;
+  v.brows = m->blc_row;
+  v.nblocks = m->blocknnz / 16;
+  v.warpnum = m->warpnum;
+  BSP_TRY(h->recorder.alloc(v.nrows));
+  BSP_TRY(h->blcptr.alloc(v.brows + 1));
+  BSP_TRY(h->colidx.alloc(v.nblocks * 16));
+  BSP_TRY(h->val.alloc(v.nblocks * 16));
+  BSP_TRY(h->map.alloc(v.nblocks));
+  BSP_TRY(h->prefix.alloc(v.warpnum + 1));
+  if (v.nrows)
+    BSP_HIP(hipMemcpyAsync(h->recorder.get(), m->recorderMap, sizeof(int32_t) * v.nrows,
+                           hipMemcpyHostToDevice, st));
+  BSP_HIP(hipMemcpyAsync(h->blcptr.get(), m->blcPtr, sizeof(int32_t) * (v.brows + 1),
+                         hipMemcpyHostToDevice, st));
+  if (v.nblocks) {
+    BSP_HIP(hipMemcpyAsync(h->colidx.get(), m->colIdx, sizeof(int32_t) * v.nblocks * 16,
+                           hipMemcpyHostToDevice, st));
+    BSP_HIP(hipMemcpyAsync(h->val.get(), m->blcVal, sizeof(double) * v.nblocks * 16,
+                           hipMemcpyHostToDevice, st));
+    BSP_HIP(hipMemcpyAsync(h->map.get(), m->blcMap, sizeof(uint16_t) * v.nblocks,
+                           hipMemcpyHostToDevice, st));
+  }
+  if (m->prefix)
+    BSP_HIP(hipMemcpyAsync(h->prefix.get(), m->prefix, sizeof(int32_t) * (v.warpnum + 1),
+                           hipMemcpyHostToDevice, st));
+  BSP_HIP(hipStreamSynchronize(st));
+  v.recorder = h->recorder.get();
+  v.blcptr = h->blcptr.get();
+  v.colidx = h->colidx.get();
+  v.val = h->val.get();
+  v.map = h->map.get();
+  v.prefix = h->prefix.get();
+  *out = h.release();
+  return BSP_OK;
+}
+
 }  // extern "C"

@@ -134,7 +134,8 @@ bsp_status bsp_csr_plan_create_ex(const bsp_csr* A, int method, const bsp_csr_op
                                   bsp_stream s, bsp_spmv_plan* out) {
   BSP_TRY(check_csr(A));
   BSP_REQUIRE(out, "null output handle");
-  BSP_REQUIRE(A->dtype == BSP_F64, "CSR SpMV plan needs FP64 values");
+  BSP_REQUIRE(A->dtype == BSP_F64 || (A->dtype == BSP_F16 && method == BSP_CSR_SPLIT),
+              "CSR SpMV plans need FP64 values (FP16 with the split method)");
   auto p = std::make_unique<bsp_spmv_plan_s>();
   p->A = *A;
   p->method = method;
@@ -157,8 +158,16 @@ bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* ns
 
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s) {
   BSP_REQUIRE(p, "null plan");
+  BSP_REQUIRE(p->A.dtype == BSP_F64, "plan has FP16 values: use bsp_csr_spmv_f16");
   BSP_REQUIRE((p->A.ncols == 0 || x) && (p->A.nrows == 0 || y), "null x/y");
   return csr_plan_run(p, x, y, (hipStream_t)s);
+}
+
+bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_stream s) {
+  BSP_REQUIRE(p, "null plan");
+  BSP_REQUIRE(p->A.dtype == BSP_F16, "plan has no FP16 values");
+  BSP_REQUIRE((p->A.ncols == 0 || x) && (p->A.nrows == 0 || y), "null x/y");
+  return csr_plan_run_f16(p, x, y, (hipStream_t)s);
 }
 
 bsp_status bsp_csr_plan_info(bsp_spmv_plan p, int64_t* nblocks, int64_t* nlong) {

@@ -334,6 +334,58 @@ __global__ __launch_bounds__(kWG) void split_long_f64(
   if (lane == 0) y[r] = s;
 }
 
+// FP16 split SpMV (src/spmv.cu:13-27, 44-75): half values and x, products
+// and sums in FP32 (the reference accumulates in half), y = half(sum)
+// returned as float, the rounding of ref_spmv_fp16 (src/ref.cpp:3-13).
+__device__ __forceinline__ float h2f(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+__device__ __forceinline__ float round_half(float v) { return (float)(_Float16)v; }
+
+__global__ __launch_bounds__(kWG) void split_short_f16(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const uint16_t* __restrict__ val, const uint16_t* __restrict__ x, float* __restrict__ y,
+    const int32_t* __restrict__ map, int64_t n) {
+#pragma clang fp contract(off)
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int r = map[t];
+  float s = 0.f;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) s += h2f(val[j]) * h2f(x[col[j]]);
+  y[r] = round_half(s);
+}
+
+__global__ __launch_bounds__(kWG) void split_long_f16(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const uint16_t* __restrict__ val, const uint16_t* __restrict__ x, float* __restrict__ y,
+    const int32_t* __restrict__ map, int64_t n) {
+  int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (w >= n) return;
+  const int r = map[w];
+  float s = 0.f;
+  for (int j = rowptr[r] + lane, e = rowptr[r + 1]; j < e; j += kWave)
+    s += h2f(val[j]) * h2f(x[col[j]]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) y[r] = round_half(s);
+}
+
+bsp_status csr_plan_run_f16(const CsrPlan* p, const uint16_t* x, float* y, hipStream_t s) {
+  const bsp_csr& A = p->A;
+  if (p->method != BSP_CSR_SPLIT) return fail(BSP_ERR_UNSUPPORTED, "FP16 SpMV uses the split plan");
+  if (A.nrows == 0) return BSP_OK;
+  const auto* v = static_cast<const uint16_t*>(A.val);
+  if (p->nblk > 0)
+    split_short_f16<<<cdiv(p->nblk, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, v, x, y, p->a.get(),
+                                                       p->nblk);
+  if (p->nlong > 0)
+    split_long_f16<<<cdiv(p->nlong * kWave, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, v, x, y,
+                                                                p->b.get(), p->nlong);
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Plan construction.
 

@@ -1,0 +1,429 @@
+// Reference call surface: host-pointer wrappers with the semantics of the
+// reference's host functions (upload, build on the GPU, warmup + timed
+// launches each followed by a synchronisation, download), plus the rocSPARSE
+// comparator. Only public C-ABI entry points of the library are used here.
+#include <hip/hip_runtime.h>
+#include <rocsparse/rocsparse.h>
+
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "bsp_hip.hpp"
+
+using namespace bsp;
+
+namespace {
+
+constexpr int kTB = 256;
+
+__global__ void i64_to_i32(const int64_t* __restrict__ a, int64_t n, int32_t* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (int32_t)a[i];
+}
+__global__ void f32_to_f64(const float* __restrict__ a, int64_t n, double* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+__global__ void f32_to_f16(const float* __restrict__ a, int64_t n, uint16_t* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = __builtin_bit_cast(uint16_t, (_Float16)a[i]);
+}
+__global__ void fill_u16(uint16_t* __restrict__ a, int64_t n, uint16_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+template <class T>
+bsp_status up(DevBuf<T>& d, const T* h, int64_t n) {
+  BSP_TRY(d.alloc(n));
+  if (n > 0) BSP_HIP(hipMemcpy(d.get(), h, sizeof(T) * n, hipMemcpyHostToDevice));
+  return BSP_OK;
+}
+
+template <class T>
+bsp_status down(T* h, const T* d, int64_t n) {
+  if (n > 0) BSP_HIP(hipMemcpy(h, d, sizeof(T) * n, hipMemcpyDeviceToHost));
+  return BSP_OK;
+}
+
+// host-clock timing of launch+sync pairs, whole microseconds per launch
+template <class F>
+bsp_status timed(int warmup, int repeat, double* avg_time, F body) {
+  for (int i = 0; i < warmup; ++i) {
+    BSP_TRY(body());
+    BSP_HIP(hipDeviceSynchronize());
+  }
+  auto t0 = std::chrono::high_resolution_clock::now();
+  for (int i = 0; i < repeat; ++i) {
+    BSP_TRY(body());
+    BSP_HIP(hipDeviceSynchronize());
+  }
+  auto t1 = std::chrono::high_resolution_clock::now();
+  if (avg_time) {
+    long long us = std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
+    *avg_time = repeat > 0 ? (double)(us / repeat) : 0.0;
+  }
+  return BSP_OK;
+}
+
+struct DevCsr {
+  DevBuf<int32_t> rp, ci;
+  DevBuf<double> v;
+  bsp_csr A{};
+};
+
+bsp_status upload_i32(DevCsr& c, const int32_t* rp, const int32_t* ci, const double* v,
+                      int64_t rows, int64_t cols, int64_t nnz) {
+  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && nnz < (int64_t(1) << 31), "bad sizes");
+  BSP_REQUIRE(rp && (nnz == 0 || (ci && v)), "null host pointer");
+  BSP_TRY(up(c.rp, rp, rows + 1));
+  BSP_TRY(up(c.ci, ci, nnz));
+  BSP_TRY(up(c.v, v, nnz));
+  c.A = bsp_csr{rows, cols, nnz, c.rp.get(), c.ci.get(), c.v.get(), BSP_F64};
+  return BSP_OK;
+}
+
+bsp_status upload_i64(DevCsr& c, const int64_t* rp, const int64_t* ci, const double* v,
+                      int64_t rows, int64_t cols, int64_t nnz) {
+  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && nnz < (int64_t(1) << 31) &&
+                  rows < (int64_t(1) << 31) && cols < (int64_t(1) << 31),
+              "sizes must fit 32-bit indices");
+  BSP_REQUIRE(rp && (nnz == 0 || (ci && v)), "null host pointer");
+  DevBuf<int64_t> t;
+  BSP_TRY(up(t, rp, rows + 1));
+  BSP_TRY(c.rp.alloc(rows + 1));
+  i64_to_i32<<<cdiv(rows + 1, kTB), kTB>>>(t.get(), rows + 1, c.rp.get());
+  BSP_TRY(up(t, ci, nnz));
+  BSP_TRY(c.ci.alloc(nnz));
+  if (nnz) i64_to_i32<<<cdiv(nnz, kTB), kTB>>>(t.get(), nnz, c.ci.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(up(c.v, v, nnz));
+  BSP_HIP(hipDeviceSynchronize());
+  c.A = bsp_csr{rows, cols, nnz, c.rp.get(), c.ci.get(), c.v.get(), BSP_F64};
+  return BSP_OK;
+}
+
+template <class T>
+T* host_copy(const T* d, int64_t n) {
+  T* h = (T*)std::malloc(sizeof(T) * (n > 0 ? n : 1));
+  if (h && n > 0 && hipMemcpy(h, d, sizeof(T) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+    std::free(h);
+    return nullptr;
+  }
+  return h;
+}
+
+bsp_status dbsr_spmv_host(bsp_dbsrMat* m, double* hX, double* hY, int warmup, int repeat,
+                          double* avg_time) {
+  BSP_REQUIRE(m && hX && hY, "null argument");
+  bsp_dbsr h = nullptr;
+  BSP_TRY(bsp_dbsr_from_host(m, nullptr, &h));
+  std::unique_ptr<bsp_dbsr_s, bsp_status (*)(bsp_dbsr)> own(h, bsp_dbsr_destroy);
+  DevBuf<double> x, y;
+  BSP_TRY(up(x, hX, m->col));
+  BSP_TRY(y.alloc(m->row));
+  BSP_TRY(timed(warmup, repeat, avg_time,
+                [&] { return bsp_dbsr_spmv(h, x.get(), y.get(), 1, nullptr); }));
+  return down(hY, y.get(), m->row);
+}
+
+}  // namespace
+
+struct bsp_rocsparse_plan_s {
+  rocsparse_handle handle = nullptr;
+  rocsparse_spmat_descr mat = nullptr;
+  rocsparse_dnvec_descr x = nullptr, y = nullptr;
+  rocsparse_spmv_alg alg = rocsparse_spmv_alg_default;
+  DevBuf<char> buffer;
+  size_t bytes = 0;
+  bsp_csr A{};
+  ~bsp_rocsparse_plan_s() {
+    if (x) rocsparse_destroy_dnvec_descr(x);
+    if (y) rocsparse_destroy_dnvec_descr(y);
+    if (mat) rocsparse_destroy_spmat_descr(mat);
+    if (handle) rocsparse_destroy_handle(handle);
+  }
+};
+
+#define BSP_RS(call)                                                                         \
+  do {                                                                                       \
+    rocsparse_status e_ = (call);                                                            \
+    if (e_ != rocsparse_status_success)                                                      \
+      return fail(BSP_ERR_HIP, std::string(#call) + ": rocsparse status " + std::to_string(e_)); \
+  } while (0)
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+
+extern "C" {
+
+bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
+                                     bsp_rocsparse_plan* out) {
+  BSP_REQUIRE(A && out && A->dtype == BSP_F64, "FP64 CSR required");
+  auto p = std::make_unique<bsp_rocsparse_plan_s>();
+  p->A = *A;
+  p->alg = (rocsparse_spmv_alg)alg;
+  BSP_RS(rocsparse_create_handle(&p->handle));
+  BSP_RS(rocsparse_set_stream(p->handle, (hipStream_t)s));
+  BSP_RS(rocsparse_create_csr_descr(&p->mat, A->nrows, A->ncols, A->nnz, (void*)A->rowptr,
+                                    (void*)A->colidx, (void*)A->val, rocsparse_indextype_i32,
+                                    rocsparse_indextype_i32, rocsparse_index_base_zero,
+                                    rocsparse_datatype_f64_r));
+  DevBuf<double> xd, yd;
+  BSP_TRY(xd.alloc(A->ncols));
+  BSP_TRY(yd.alloc(A->nrows));
+  BSP_RS(rocsparse_create_dnvec_descr(&p->x, A->ncols, xd.get(), rocsparse_datatype_f64_r));
+  BSP_RS(rocsparse_create_dnvec_descr(&p->y, A->nrows, yd.get(), rocsparse_datatype_f64_r));
+  const double one = 1.0, zero = 0.0;
+  BSP_RS(rocsparse_spmv(p->handle, rocsparse_operation_none, &one, p->mat, p->x, &zero, p->y,
+                        rocsparse_datatype_f64_r, p->alg, rocsparse_spmv_stage_buffer_size,
+                        &p->bytes, nullptr));
+  BSP_TRY(p->buffer.alloc(p->bytes));
+  BSP_RS(rocsparse_spmv(p->handle, rocsparse_operation_none, &one, p->mat, p->x, &zero, p->y,
+                        rocsparse_datatype_f64_r, p->alg, rocsparse_spmv_stage_preprocess,
+                        &p->bytes, p->buffer.get()));
+  BSP_HIP(hipStreamSynchronize((hipStream_t)s));
+  *out = p.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const double* x, double* y, bsp_stream s) {
+  BSP_REQUIRE(p && x && y, "null argument");
+  BSP_RS(rocsparse_set_stream(p->handle, (hipStream_t)s));
+  BSP_RS(rocsparse_dnvec_set_values(p->x, (void*)x));
+  BSP_RS(rocsparse_dnvec_set_values(p->y, (void*)y));
+  const double one = 1.0, zero = 0.0;
+  BSP_RS(rocsparse_spmv(p->handle, rocsparse_operation_none, &one, p->mat, p->x, &zero, p->y,
+                        rocsparse_datatype_f64_r, p->alg, rocsparse_spmv_stage_compute, &p->bytes,
+                        p->buffer.get()));
+  return BSP_OK;
+}
+
+bsp_status bsp_rocsparse_plan_destroy(bsp_rocsparse_plan p) {
+  delete p;
+  return BSP_OK;
+}
+
+int bsp_cusparse_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* hA_values,
+                           double* hX, double* hY, int64_t A_num_rows, int64_t A_num_cols,
+                           int64_t A_nnz, int repeat, double* avg_time) {
+  DevCsr c;
+  if (upload_i64(c, hA_csrOffsets, hA_columns, hA_values, A_num_rows, A_num_cols, A_nnz) != BSP_OK)
+    return 1;
+  if (!hX || !hY) return 1;
+  DevBuf<double> x, y;
+  if (up(x, hX, A_num_cols) != BSP_OK || y.alloc(A_num_rows) != BSP_OK) return 1;
+  bsp_rocsparse_plan p = nullptr;
+  if (bsp_rocsparse_plan_create(&c.A, 0, nullptr, &p) != BSP_OK) return 1;
+  std::unique_ptr<bsp_rocsparse_plan_s> own(p);
+  if (timed(10, repeat, avg_time, [&] { return bsp_rocsparse_spmv(p, x.get(), y.get(), nullptr); }) !=
+      BSP_OK)
+    return 1;
+  return down(hY, y.get(), A_num_rows) == BSP_OK ? 0 : 1;
+}
+
+int bsp_csrTodbsrWithRecorder(const bsp_CSRFormat_f64* csr, bsp_dbsrMat* out) {
+  if (!csr || !out) return 1;
+  DevCsr c;
+  if (upload_i32(c, csr->rowPtr, csr->colIdx, csr->values, csr->row, csr->col, csr->nnz) != BSP_OK)
+    return 1;
+  bsp_dbsr h = nullptr;
+  if (bsp_dbsr_build(&c.A, nullptr, &h) != BSP_OK) return 1;
+  std::unique_ptr<bsp_dbsr_s, bsp_status (*)(bsp_dbsr)> own(h, bsp_dbsr_destroy);
+  bsp_dbsr_view v;
+  bsp_dbsr_view_get(h, &v);
+  std::memset(out, 0, sizeof(*out));
+  out->row = (int)v.nrows;
+  out->col = (int)v.ncols;
+  out->nnz = (int)v.nnz;
+  out->blc_row = (int)v.brows;
+  out->blc_num = (int)v.nblocks;
+  out->avg_nnz = v.nblocks ? (double)v.nnz / (double)v.nblocks : 0.0;
+  out->blocknnz = (int32_t)(v.nblocks * 16);
+  out->warpnum = v.warpnum;
+  out->blcPtr = host_copy(v.blcptr, v.brows + 1);
+  out->colIdx = host_copy(v.colidx, v.nblocks * 16);
+  out->blcVal = host_copy(v.val, v.nblocks * 16);
+  out->blcMap = host_copy(v.map, v.nblocks);
+  out->recorderMap = host_copy(v.recorder, v.nrows);
+  out->prefix = host_copy(v.prefix, (int64_t)v.warpnum + 1);
+  if (!out->blcPtr || !out->colIdx || !out->blcVal || !out->blcMap || !out->recorderMap ||
+      !out->prefix) {
+    bsp_dbsrMat_free(out);
+    return 1;
+  }
+  return 0;
+}
+
+void bsp_dbsrMat_free(bsp_dbsrMat* m) {
+  if (!m) return;
+  std::free(m->blcPtr);
+  std::free(m->colIdx);
+  std::free(m->blcVal);
+  std::free(m->blcMap);
+  std::free(m->recorderMap);
+  std::free(m->prefix);
+  m->blcPtr = m->colIdx = m->recorderMap = m->prefix = nullptr;
+  m->blcVal = nullptr;
+  m->blcMap = nullptr;
+}
+
+int bsp_amgT_spmv_fp64_dbsr(bsp_dbsrMat* dbsr, double* hX, double* hY, int32_t warmup,
+                            int32_t repeat, double* avg_time) {
+  return dbsr_spmv_host(dbsr, hX, hY, warmup, repeat, avg_time) == BSP_OK ? 0 : 1;
+}
+
+int bsp_amgT_spmv_fp64_dbsr_balance(bsp_dbsrMat* dbsr, double* hX, double* hY, int32_t warmup,
+                                    int32_t repeat, double* avg_time) {
+  return dbsr_spmv_host(dbsr, hX, hY, warmup, repeat, avg_time) == BSP_OK ? 0 : 1;
+}
+
+int bsp_amgT_spmv_fp64(int32_t* hA_csrOffsets, int32_t* hA_columns, double* hA_values, double* hX,
+                       double* hY, int32_t A_num_rows, int32_t A_num_cols, int32_t A_nnz,
+                       int repeat, double* avg_time) {
+  DevCsr c;
+  if (upload_i32(c, hA_csrOffsets, hA_columns, hA_values, A_num_rows, A_num_cols, A_nnz) != BSP_OK)
+    return 1;
+  if (!hX || !hY) return 1;
+  bsp_bsr b = nullptr;
+  if (bsp_bsr_build(&c.A, 4, 4, nullptr, &b) != BSP_OK) return 1;
+  std::unique_ptr<bsp_bsr_s, bsp_status (*)(bsp_bsr)> own(b, bsp_bsr_destroy);
+  bsp_bsr_view v;
+  bsp_bsr_view_get(b, &v);
+  const int kernel = v.avg_nnz >= 10.0 ? BSP_BSR_MFMA : BSP_BSR_VALU;
+  DevBuf<double> x, y;
+  if (up(x, hX, A_num_cols) != BSP_OK || y.alloc(A_num_rows) != BSP_OK) return 1;
+  if (timed(5, repeat, avg_time, [&] { return bsp_bsr_spmv(b, kernel, x.get(), y.get(), nullptr); }) !=
+      BSP_OK)
+    return 1;
+  return down(hY, y.get(), A_num_rows) == BSP_OK ? 0 : 1;
+}
+
+void bsp_bcsr_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* hA_values,
+                        double* hX, double* hY, int64_t A_num_rows, int64_t A_num_cols,
+                        int64_t A_nnz, int repeat) {
+  DevCsr c;
+  if (upload_i64(c, hA_csrOffsets, hA_columns, hA_values, A_num_rows, A_num_cols, A_nnz) != BSP_OK)
+    return;
+  if (!hX || !hY) return;
+  bsp_bsr b = nullptr;
+  if (bsp_bsr_build(&c.A, 3, 3, nullptr, &b) != BSP_OK) return;
+  std::unique_ptr<bsp_bsr_s, bsp_status (*)(bsp_bsr)> own(b, bsp_bsr_destroy);
+  DevBuf<double> x, y;
+  if (up(x, hX, A_num_cols) != BSP_OK || y.alloc(A_num_rows) != BSP_OK) return;
+  double t = 0;
+  if (timed(0, repeat > 0 ? repeat : 1, &t,
+            [&] { return bsp_bsr_spmv(b, BSP_BSR_VALU, x.get(), y.get(), nullptr); }) != BSP_OK)
+    return;
+  down(hY, y.get(), A_num_rows);
+}
+
+void bsp_spmv_all(char* filename, double* csrValA, int* csrRowPtrA, int* csrColIdxA, double* X_val,
+                  double* Y_val, int* order_rid, int rowA, int colA, int nnzA, int NUM,
+                  double threshold, int block_longest, double* avg_time) {
+  (void)filename;
+  (void)NUM;
+  DevCsr c;
+  if (upload_i32(c, csrRowPtrA, csrColIdxA, csrValA, rowA, colA, nnzA) != BSP_OK) return;
+  if (!X_val || !Y_val || !order_rid) return;
+  bsp_dasp d = nullptr;
+  if (bsp_dasp_build(&c.A, threshold, block_longest, nullptr, &d) != BSP_OK) return;
+  std::unique_ptr<bsp_dasp_s, bsp_status (*)(bsp_dasp)> own(d, bsp_dasp_destroy);
+  DevBuf<double> x, y;
+  if (up(x, X_val, colA) != BSP_OK || y.alloc(rowA) != BSP_OK) return;
+  if (timed(100, 10, avg_time, [&] { return bsp_dasp_spmv(d, x.get(), y.get(), 1, nullptr); }) !=
+      BSP_OK)
+    return;
+  bsp_dasp_view v;
+  bsp_dasp_view_get(d, &v);
+  down(Y_val, y.get(), rowA);
+  down(order_rid, v.order_rid, rowA);
+}
+
+void bsp_MySpmvWithPrefp64(int64_t* rowPtr, int64_t* colIdx, int64_t rows, int64_t cols,
+                           int64_t nnz, double* values, double* x, double* outvalues) {
+  DevCsr c;
+  if (upload_i64(c, rowPtr, colIdx, values, rows, cols, nnz) != BSP_OK) return;
+  if (!x || !outvalues) return;
+  bsp_spmv_plan p = nullptr;
+  if (bsp_csr_plan_create(&c.A, BSP_CSR_SPLIT, nullptr, &p) != BSP_OK) return;
+  std::unique_ptr<bsp_spmv_plan_s, bsp_status (*)(bsp_spmv_plan)> own(p, bsp_plan_destroy);
+  DevBuf<double> xd, yd;
+  if (up(xd, x, cols) != BSP_OK || yd.alloc(rows) != BSP_OK) return;
+  double t = 0;
+  if (timed(0, 10, &t, [&] { return bsp_csr_spmv(p, xd.get(), yd.get(), nullptr); }) != BSP_OK)
+    return;
+  down(outvalues, yd.get(), rows);
+}
+
+void bsp_MySpmvWithPrefp16(int32_t* rowPtr, int32_t* colIdx, int32_t rows, int32_t cols,
+                           int32_t nnz, float* values, float* x, float* outvalues) {
+  if (!rowPtr || rows < 0 || cols < 0 || nnz < 0 || (nnz && (!colIdx || !values)) || !x ||
+      !outvalues)
+    return;
+  DevBuf<int32_t> rp, ci;
+  DevBuf<float> v32, x32;
+  DevBuf<uint16_t> v16, x16;
+  DevBuf<float> y;
+  if (up(rp, rowPtr, rows + 1) != BSP_OK || up(ci, colIdx, nnz) != BSP_OK ||
+      up(v32, values, nnz) != BSP_OK || up(x32, x, cols) != BSP_OK || v16.alloc(nnz) != BSP_OK ||
+      x16.alloc(cols) != BSP_OK || y.alloc(rows) != BSP_OK)
+    return;
+  if (nnz) f32_to_f16<<<cdiv(nnz, kTB), kTB>>>(v32.get(), nnz, v16.get());
+  if (cols) f32_to_f16<<<cdiv(cols, kTB), kTB>>>(x32.get(), cols, x16.get());
+  if (hipDeviceSynchronize() != hipSuccess) return;
+  bsp_csr A{rows, cols, nnz, rp.get(), ci.get(), v16.get(), BSP_F16};
+  bsp_spmv_plan p = nullptr;
+  if (bsp_csr_plan_create(&A, BSP_CSR_SPLIT, nullptr, &p) != BSP_OK) return;
+  std::unique_ptr<bsp_spmv_plan_s, bsp_status (*)(bsp_spmv_plan)> own(p, bsp_plan_destroy);
+  double t = 0;
+  if (timed(0, 10, &t, [&] { return bsp_csr_spmv_f16(p, x16.get(), y.get(), nullptr); }) != BSP_OK)
+    return;
+  down(outvalues, y.get(), rows);
+}
+
+void bsp_smatSpmmfp16(int* rowPtr, int* colIdx, int row, int col, int n, int nnz, float* values,
+                      float* outvalues) {
+  if (!rowPtr || row < 0 || col < 0 || nnz < 0 || n <= 0 || n % 32 != 0 || !outvalues ||
+      (nnz && (!colIdx || !values)))
+    return;
+  DevCsr c;
+  DevBuf<float> v32;
+  if (up(c.rp, rowPtr, (int64_t)row + 1) != BSP_OK || up(c.ci, colIdx, nnz) != BSP_OK ||
+      up(v32, values, nnz) != BSP_OK || c.v.alloc(nnz) != BSP_OK)
+    return;
+  if (nnz) f32_to_f64<<<cdiv(nnz, kTB), kTB>>>(v32.get(), nnz, c.v.get());
+  c.A = bsp_csr{row, col, nnz, c.rp.get(), c.ci.get(), c.v.get(), BSP_F64};
+  bsp_bsr b = nullptr;
+  if (bsp_bsr_build(&c.A, 16, 16, nullptr, &b) != BSP_OK) return;
+  std::unique_ptr<bsp_bsr_s, bsp_status (*)(bsp_bsr)> ownb(b, bsp_bsr_destroy);
+  bsp_bspmm h = nullptr;
+  if (bsp_bspmm_build(b, BSP_F16, nullptr, &h) != BSP_OK) return;
+  std::unique_ptr<bsp_bspmm_s, bsp_status (*)(bsp_bspmm)> ownh(h, bsp_bspmm_destroy);
+  // B = 1 (col x n) as in the reference (src/smat.cu:142-144), in column
+  // chunks of at most 256
+  const int chunk = n % 256 == 0 ? 256 : n % 128 == 0 ? 128 : n % 64 == 0 ? 64 : 32;
+  DevBuf<uint16_t> B;
+  DevBuf<float> Cc, Cfull;
+  if (B.alloc((int64_t)col * chunk) != BSP_OK || Cc.alloc((int64_t)row * chunk) != BSP_OK ||
+      Cfull.alloc((int64_t)row * n) != BSP_OK)
+    return;
+  const uint16_t one = __builtin_bit_cast(uint16_t, (_Float16)1.0f);
+  fill_u16<<<cdiv((int64_t)col * chunk, kTB), kTB>>>(B.get(), (int64_t)col * chunk, one);
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    if (bsp_bspmm_fwd(h, B.get(), chunk, Cc.get(), nullptr) != BSP_OK) return;
+    if (row > 0 &&
+        hipMemcpy2D(Cfull.get() + c0, sizeof(float) * n, Cc.get(), sizeof(float) * chunk,
+                    sizeof(float) * chunk, row, hipMemcpyDeviceToDevice) != hipSuccess)
+      return;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return;
+  down(outvalues, Cfull.get(), (int64_t)row * n);
+}
+
+}  // extern "C"
+
+#pragma clang diagnostic pop

@@ -293,6 +293,112 @@ void bsp_host_csr_free(bsp_host_csr h);
 
 /* ------------------------------------------------------------------------ */
 /* Reference call surface (host pointers).                                   */
+/* Every function uploads its inputs, builds what the reference builds (on the
+ * GPU), times `repeat` launches each followed by a stream synchronisation
+ * (host clock, as the reference does) and downloads the result. avg_time
+ * receives whole microseconds per launch like the reference. Results are
+ * y = A*x exactly once per call: the reference's accumulation into an
+ * un-zeroed output (atomicAdd across repeats) is not reproduced.          */
+
+/* Layout of CSRFormat<double, int32_t> (include/matrixFormat.hpp:4-12). */
+typedef struct bsp_CSRFormat_f64 {
+  int32_t* rowPtr;
+  int32_t* colIdx;
+  double* values;
+  int32_t row, col;
+  int32_t nnz;
+} bsp_CSRFormat_f64;
+
+/* Layout of dbsrMat (include/matrixFormat.hpp:86-106). */
+typedef struct bsp_dbsrMat {
+  int row, col, nnz, blc_row, blc_col, blc_num;
+  double stand, avg_nnz;
+  int32_t* blcPtr;
+  int32_t* colIdx;
+  double* blcVal;
+  unsigned short* blcMap;
+  int32_t blocknnz;
+  int32_t warpnum;
+  int32_t* rowPtrbyWarp;
+  int32_t* rowIdxbyWarp;
+  int32_t* recorderMap;
+  int32_t* prefix;
+} bsp_dbsrMat;
+
+/* Replaces csrTodbsrWithRecorder (include/dbsr.h:208-253, with csrTodbsr
+ * :10-93 and balanceDbsr :94-169): builds on the GPU and fills `out` with
+ * malloc'ed host arrays (blcPtr, colIdx, blcVal, blcMap, recorderMap,
+ * prefix) bit-identical to the reference builder's; free them with
+ * bsp_dbsrMat_free. blc_num is set to the number of blocks, blc_col to 0.
+ * Returns 0 on success. */
+int bsp_csrTodbsrWithRecorder(const bsp_CSRFormat_f64* csr, bsp_dbsrMat* out);
+void bsp_dbsrMat_free(bsp_dbsrMat* m);
+/* Uploads a host DBSR (as filled above) into a device handle. */
+bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out);
+
+/* Replaces int amgT_spmv_fp64_dbsr(dbsrMat*, double* hX, double* hY, int32_t
+ * warmup, int32_t repeat, double& avg_time) (include/amgT.h:4,
+ * src/amgT.cu:1202-1290): y in sorted-row order (the caller un-permutes
+ * through recorderMap). */
+int bsp_amgT_spmv_fp64_dbsr(bsp_dbsrMat* dbsr, double* hX, double* hY, int32_t warmup,
+                            int32_t repeat, double* avg_time);
+/* Replaces amgT_spmv_fp64_dbsr_balance (include/amgT.h:6, src/amgT.cu:1292-1353);
+ * the same kernel (it balances by block rows itself, not by `prefix`). */
+int bsp_amgT_spmv_fp64_dbsr_balance(bsp_dbsrMat* dbsr, double* hX, double* hY, int32_t warmup,
+                                    int32_t repeat, double* avg_time);
+/* Replaces int amgT_spmv_fp64(...) (include/amgT.h:3, src/amgT.cu:1355-1488):
+ * CSR -> 4x4 BSR on the GPU, then the MFMA kernel when nnz/blocks >= 10 and
+ * the VALU kernel otherwise (the reference's avgnz cut, amgT.cu:1394-1424),
+ * 5 warmups, `repeat` timed launches. */
+int bsp_amgT_spmv_fp64(int32_t* hA_csrOffsets, int32_t* hA_columns, double* hA_values, double* hX,
+                       double* hY, int32_t A_num_rows, int32_t A_num_cols, int32_t A_nnz,
+                       int repeat, double* avg_time);
+/* Replaces void bcsr_spmv_fp64(...) (include/bcsr.h:3, src/bcsr.cu:224-302):
+ * 3x3 BCSR built on the GPU, `repeat` launches. */
+void bsp_bcsr_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* hA_values,
+                        double* hX, double* hY, int64_t A_num_rows, int64_t A_num_cols,
+                        int64_t A_nnz, int repeat);
+/* Replaces void spmv_all(...) (include/dasp.h:1-2, src/dasp.cu:814-1823): DASP
+ * layout built on the GPU, 100 warmups + 10 timed launches as the reference
+ * (it ignores `repeat`); Y in order_rid order and order_rid (rowA entries)
+ * returned. filename and NUM are unused, as in the reference. */
+void bsp_spmv_all(char* filename, double* csrValA, int* csrRowPtrA, int* csrColIdxA, double* X_val,
+                  double* Y_val, int* order_rid, int rowA, int colA, int nnzA, int NUM,
+                  double threshold, int block_longest, double* avg_time);
+/* Replaces void MySpmvWithPrefp64(...) (include/spmv.h:3, src/spmv.cu:323-397):
+ * short/long split at 128; outvalues keeps full FP64 (the reference truncates
+ * to float). */
+void bsp_MySpmvWithPrefp64(int64_t* rowPtr, int64_t* colIdx, int64_t rows, int64_t cols,
+                           int64_t nnz, double* values, double* x, double* outvalues);
+/* Replaces void MySpmvWithPrefp16(...) (include/spmv.h:2, src/spmv.cu:197-272):
+ * values and x rounded to FP16, products and sums in FP32 (the reference
+ * sums in half), y rounded to FP16 and returned as float. */
+void bsp_MySpmvWithPrefp16(int32_t* rowPtr, int32_t* colIdx, int32_t rows, int32_t cols,
+                           int32_t nnz, float* values, float* x, float* outvalues);
+/* Replaces void smatSpmmfp16(...) (include/smat.h:1, src/smat.cu:103-202):
+ * C (row x n, FP32, row-major) = A * B with B = 1 (col x n) as in the
+ * reference, on the FP16 block-SpMM kernel; C is written to outvalues (the
+ * reference never writes it). n: any multiple of 32. */
+void bsp_smatSpmmfp16(int* rowPtr, int* colIdx, int row, int col, int n, int nnz, float* values,
+                      float* outvalues);
+/* Replaces int cusparse_spmv_fp64(...) (include/cusp.h:4, src/cusparse.cu:104-197)
+ * with rocSPARSE: 64-bit indices, default algorithm, 10 warmups + `repeat`
+ * timed calls. The vendor comparator, not a library kernel. */
+int bsp_cusparse_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* hA_values,
+                           double* hX, double* hY, int64_t A_num_rows, int64_t A_num_cols,
+                           int64_t A_nnz, int repeat, double* avg_time);
+/* Device-level FP16 split SpMV: plan from bsp_csr_plan_create(method
+ * BSP_CSR_SPLIT, FP16 values); x FP16 (ncols), y FP32 (nrows). */
+bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_stream s);
+/* Device-level rocSPARSE SpMV (FP64, int32 indices, the given algorithm:
+ * 0 default, 2 CSR adaptive, 1 CSR stream) prepared once (buffer and
+ * analysis) and run per call: the vendor point reported next to the
+ * library's kernels. */
+typedef struct bsp_rocsparse_plan_s* bsp_rocsparse_plan;
+bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
+                                     bsp_rocsparse_plan* out);
+bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const double* x, double* y, bsp_stream s);
+bsp_status bsp_rocsparse_plan_destroy(bsp_rocsparse_plan p);
 
 /* Replaces int spmv_fp64_warpper(...) (include/csrspmv.h:2, src/csrspmv.cu:104-134):
  * thread-per-row CSR, 10 warmups, `repeat` timed launches; y = A*x. */

@@ -35,6 +35,21 @@ def test_python_binding_declares_only_real_symbols():
         assert hasattr(lib().cdll, n), n
 
 
+def test_python_binding_covers_every_declared_function():
+    from benchsp_amd._lib import exported_symbols
+    missing = sorted(set(_declared()) - set(exported_symbols()))
+    assert not missing, missing
+
+
+def test_struct_mirrors_match_header_layout():
+    # the reference structs are mirrored field for field; on LP64 the dbsrMat
+    # layout is 6 ints, 2 doubles, 4 pointers, 2 int32, 4 pointers
+    from benchsp_amd._lib import CSRFormatF64, DbsrMat
+    assert C.sizeof(CSRFormatF64) == 3 * 8 + 3 * 4 + 4
+    assert C.sizeof(DbsrMat) == 6 * 4 + 2 * 8 + 4 * 8 + 2 * 4 + 4 * 8
+    assert DbsrMat.blocknnz.offset == 6 * 4 + 2 * 8 + 4 * 8
+
+
 def test_status_and_last_error():
     from benchsp_amd._lib import BspError, CsrDesc, lib
     L = lib()
