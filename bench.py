@@ -213,8 +213,8 @@ def measured_traffic():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding plan is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library "
+              "runs one matrix shard per GPU and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
     import torch
     dev = torch.device("cuda:0")

@@ -87,6 +87,12 @@ _SIGS: dict[str, tuple] = {
     "bsp_csr_spmv_f16": (None, [vp, vp, vp, vp]),
     "bsp_rocsparse_plan_create": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(vp)]),
     "bsp_rocsparse_spmv": (None, [vp, vp, vp, vp]),
+    "bsp_cusparse_spmv": (C.c_int, [vp, vp, vp, vp, vp, i64, i64, i64]),
+    "bsp_cusparse_spmv_fp16": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32]),
+    "bsp_cusparse_spmv_fp16_warpper": (VOID, [vp, vp, vp, vp, vp, i32, i32, i32]),
+    "bsp_ref_spmv_fp64": (VOID, [vp, vp, vp, vp, vp, i64, i64, f64p]),
+    "bsp_ref_spmv_fp32": (VOID, [vp, vp, vp, vp, vp, i32, i32]),
+    "bsp_ref_spmv_fp16": (VOID, [vp, vp, vp, vp, vp, i32, i32]),
     "bsp_rocsparse_plan_destroy": (None, [vp]),
     "bsp_dbsr_from_host": (None, [C.POINTER(DbsrMat), vp, C.POINTER(vp)]),
     "bsp_csrTodbsrWithRecorder": (C.c_int, [C.POINTER(CSRFormatF64), C.POINTER(DbsrMat)]),

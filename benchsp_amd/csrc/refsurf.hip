@@ -36,6 +36,38 @@ __global__ void fill_u16(uint16_t* __restrict__ a, int64_t n, uint16_t v) {
   if (i < n) a[i] = v;
 }
 
+__global__ void f16_to_f32(const uint16_t* __restrict__ a, int64_t n, float* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)__builtin_bit_cast(_Float16, a[i]);
+}
+__global__ void f32_round_f16(const float* __restrict__ a, int64_t n, uint16_t* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = __builtin_bit_cast(uint16_t, (_Float16)a[i]);
+}
+
+// Thread per row, left to right, no contraction: the reference CPU loop's
+// result bit for bit (src/ref.cpp:3-24). HALF rounds inputs and y to FP16.
+template <bool HALF>
+__global__ __launch_bounds__(kTB) void rows_seq_f32(const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const float* __restrict__ val,
+                                                    const float* __restrict__ x,
+                                                    float* __restrict__ y, int64_t n) {
+#pragma clang fp contract(off)
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float s = 0.f;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) {
+    float a = val[j], b = x[col[j]];
+    if (HALF) {
+      a = (float)(_Float16)a;
+      b = (float)(_Float16)b;
+    }
+    s += a * b;
+  }
+  y[r] = HALF ? (float)(_Float16)s : s;
+}
+
 template <class T>
 bsp_status up(DevBuf<T>& d, const T* h, int64_t n) {
   BSP_TRY(d.alloc(n));
@@ -132,14 +164,26 @@ bsp_status dbsr_spmv_host(bsp_dbsrMat* m, double* hX, double* hY, int warmup, in
 
 }  // namespace
 
+// rocsparse_spmv with explicit stages is the API of this ROCm release.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+
 struct bsp_rocsparse_plan_s {
   rocsparse_handle handle = nullptr;
   rocsparse_spmat_descr mat = nullptr;
   rocsparse_dnvec_descr x = nullptr, y = nullptr;
   rocsparse_spmv_alg alg = rocsparse_spmv_alg_default;
+  rocsparse_datatype ty = rocsparse_datatype_f64_r;  // y and compute type
   DevBuf<char> buffer;
   size_t bytes = 0;
   bsp_csr A{};
+  rocsparse_status call(rocsparse_spmv_stage stage, void* buf) {
+    const double d1 = 1.0, d0 = 0.0;
+    const float f1 = 1.f, f0 = 0.f;
+    const bool dbl = ty == rocsparse_datatype_f64_r;
+    return rocsparse_spmv(handle, rocsparse_operation_none, dbl ? (const void*)&d1 : &f1, mat, x,
+                          dbl ? (const void*)&d0 : &f0, y, ty, alg, stage, &bytes, buf);
+  }
   ~bsp_rocsparse_plan_s() {
     if (x) rocsparse_destroy_dnvec_descr(x);
     if (y) rocsparse_destroy_dnvec_descr(y);
@@ -155,50 +199,44 @@ struct bsp_rocsparse_plan_s {
       return fail(BSP_ERR_HIP, std::string(#call) + ": rocsparse status " + std::to_string(e_)); \
   } while (0)
 
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Wdeprecated-declarations"
-
 extern "C" {
 
 bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
                                      bsp_rocsparse_plan* out) {
-  BSP_REQUIRE(A && out && A->dtype == BSP_F64, "FP64 CSR required");
+  BSP_REQUIRE(A && out, "null argument");
+  BSP_REQUIRE(A->dtype == BSP_F64 || A->dtype == BSP_F32 || A->dtype == BSP_F16,
+              "rocSPARSE comparator takes FP64, FP32 or FP16 values");
   auto p = std::make_unique<bsp_rocsparse_plan_s>();
   p->A = *A;
   p->alg = (rocsparse_spmv_alg)alg;
+  const rocsparse_datatype ta = A->dtype == BSP_F64   ? rocsparse_datatype_f64_r
+                                : A->dtype == BSP_F32 ? rocsparse_datatype_f32_r
+                                                      : rocsparse_datatype_f16_r;
+  p->ty = A->dtype == BSP_F64 ? rocsparse_datatype_f64_r : rocsparse_datatype_f32_r;
   BSP_RS(rocsparse_create_handle(&p->handle));
   BSP_RS(rocsparse_set_stream(p->handle, (hipStream_t)s));
   BSP_RS(rocsparse_create_csr_descr(&p->mat, A->nrows, A->ncols, A->nnz, (void*)A->rowptr,
                                     (void*)A->colidx, (void*)A->val, rocsparse_indextype_i32,
-                                    rocsparse_indextype_i32, rocsparse_index_base_zero,
-                                    rocsparse_datatype_f64_r));
-  DevBuf<double> xd, yd;
+                                    rocsparse_indextype_i32, rocsparse_index_base_zero, ta));
+  DevBuf<double> xd, yd;  // placeholders until the first run sets x and y
   BSP_TRY(xd.alloc(A->ncols));
   BSP_TRY(yd.alloc(A->nrows));
-  BSP_RS(rocsparse_create_dnvec_descr(&p->x, A->ncols, xd.get(), rocsparse_datatype_f64_r));
-  BSP_RS(rocsparse_create_dnvec_descr(&p->y, A->nrows, yd.get(), rocsparse_datatype_f64_r));
-  const double one = 1.0, zero = 0.0;
-  BSP_RS(rocsparse_spmv(p->handle, rocsparse_operation_none, &one, p->mat, p->x, &zero, p->y,
-                        rocsparse_datatype_f64_r, p->alg, rocsparse_spmv_stage_buffer_size,
-                        &p->bytes, nullptr));
+  BSP_RS(rocsparse_create_dnvec_descr(&p->x, A->ncols, xd.get(), ta));
+  BSP_RS(rocsparse_create_dnvec_descr(&p->y, A->nrows, yd.get(), p->ty));
+  BSP_RS(p->call(rocsparse_spmv_stage_buffer_size, nullptr));
   BSP_TRY(p->buffer.alloc(p->bytes));
-  BSP_RS(rocsparse_spmv(p->handle, rocsparse_operation_none, &one, p->mat, p->x, &zero, p->y,
-                        rocsparse_datatype_f64_r, p->alg, rocsparse_spmv_stage_preprocess,
-                        &p->bytes, p->buffer.get()));
+  BSP_RS(p->call(rocsparse_spmv_stage_preprocess, p->buffer.get()));
   BSP_HIP(hipStreamSynchronize((hipStream_t)s));
   *out = p.release();
   return BSP_OK;
 }
 
-bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const double* x, double* y, bsp_stream s) {
+bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const void* x, void* y, bsp_stream s) {
   BSP_REQUIRE(p && x && y, "null argument");
   BSP_RS(rocsparse_set_stream(p->handle, (hipStream_t)s));
   BSP_RS(rocsparse_dnvec_set_values(p->x, (void*)x));
-  BSP_RS(rocsparse_dnvec_set_values(p->y, (void*)y));
-  const double one = 1.0, zero = 0.0;
-  BSP_RS(rocsparse_spmv(p->handle, rocsparse_operation_none, &one, p->mat, p->x, &zero, p->y,
-                        rocsparse_datatype_f64_r, p->alg, rocsparse_spmv_stage_compute, &p->bytes,
-                        p->buffer.get()));
+  BSP_RS(rocsparse_dnvec_set_values(p->y, y));
+  BSP_RS(p->call(rocsparse_spmv_stage_compute, p->buffer.get()));
   return BSP_OK;
 }
 
@@ -422,6 +460,125 @@ void bsp_smatSpmmfp16(int* rowPtr, int* colIdx, int row, int col, int n, int nnz
   }
   if (hipDeviceSynchronize() != hipSuccess) return;
   down(outvalues, Cfull.get(), (int64_t)row * n);
+}
+
+int bsp_cusparse_spmv(int* hA_csrOffsets, int64_t* hA_columns, float* hA_values, float* hX,
+                      float* hY, int64_t A_num_rows, int64_t A_num_cols, int64_t A_nnz) {
+  if (!hA_csrOffsets || A_num_rows < 0 || A_num_cols < 0 || A_nnz < 0 ||
+      A_nnz >= (int64_t(1) << 31) || A_num_cols >= (int64_t(1) << 31) ||
+      (A_nnz && (!hA_columns || !hA_values)) || !hX || !hY)
+    return 1;
+  DevBuf<int32_t> rp, ci;
+  DevBuf<int64_t> c64;
+  DevBuf<float> v, x, y;
+  if (up(rp, hA_csrOffsets, A_num_rows + 1) != BSP_OK || up(c64, hA_columns, A_nnz) != BSP_OK ||
+      ci.alloc(A_nnz) != BSP_OK || up(v, hA_values, A_nnz) != BSP_OK ||
+      up(x, hX, A_num_cols) != BSP_OK || y.alloc(A_num_rows) != BSP_OK)
+    return 1;
+  if (A_nnz) i64_to_i32<<<cdiv(A_nnz, kTB), kTB>>>(c64.get(), A_nnz, ci.get());
+  bsp_csr A{A_num_rows, A_num_cols, A_nnz, rp.get(), ci.get(), v.get(), BSP_F32};
+  bsp_rocsparse_plan p = nullptr;
+  if (bsp_rocsparse_plan_create(&A, 0, nullptr, &p) != BSP_OK) return 1;
+  std::unique_ptr<bsp_rocsparse_plan_s> own(p);
+  if (bsp_rocsparse_spmv(p, x.get(), y.get(), nullptr) != BSP_OK) return 1;
+  if (hipDeviceSynchronize() != hipSuccess) return 1;
+  return down(hY, y.get(), A_num_rows) == BSP_OK ? 0 : 1;
+}
+
+int bsp_cusparse_spmv_fp16(int* hA_csrOffsets, int32_t* hA_columns, const uint16_t* hA_values,
+                           const uint16_t* hX, uint16_t* hY, int32_t A_num_rows,
+                           int32_t A_num_cols, int32_t A_nnz) {
+  if (!hA_csrOffsets || A_num_rows < 0 || A_num_cols < 0 || A_nnz < 0 ||
+      (A_nnz && (!hA_columns || !hA_values)) || !hX || !hY)
+    return 1;
+  DevBuf<int32_t> rp, ci;
+  DevBuf<uint16_t> v, x, yh;
+  DevBuf<float> y;
+  if (up(rp, hA_csrOffsets, (int64_t)A_num_rows + 1) != BSP_OK || up(ci, hA_columns, A_nnz) != BSP_OK ||
+      up(v, hA_values, A_nnz) != BSP_OK || up(x, hX, A_num_cols) != BSP_OK ||
+      y.alloc(A_num_rows) != BSP_OK || yh.alloc(A_num_rows) != BSP_OK)
+    return 1;
+  bsp_csr A{A_num_rows, A_num_cols, A_nnz, rp.get(), ci.get(), v.get(), BSP_F16};
+  bsp_rocsparse_plan p = nullptr;
+  if (bsp_rocsparse_plan_create(&A, 0, nullptr, &p) != BSP_OK) return 1;
+  std::unique_ptr<bsp_rocsparse_plan_s> own(p);
+  if (bsp_rocsparse_spmv(p, x.get(), y.get(), nullptr) != BSP_OK) return 1;
+  if (A_num_rows) f32_round_f16<<<cdiv(A_num_rows, kTB), kTB>>>(y.get(), A_num_rows, yh.get());
+  if (hipDeviceSynchronize() != hipSuccess) return 1;
+  return down(hY, yh.get(), A_num_rows) == BSP_OK ? 0 : 1;
+}
+
+void bsp_cusparse_spmv_fp16_warpper(int* hA_csrOffsets, int32_t* hA_columns, float* hA_values32,
+                                    float* hX32, float* hY32, int32_t A_num_rows,
+                                    int32_t A_num_cols, int32_t A_nnz) {
+  if (A_nnz < 0 || A_num_cols < 0 || A_num_rows < 0 || !hX32 || !hY32 || (A_nnz && !hA_values32))
+    return;
+  std::vector<uint16_t> hv(A_nnz), hx(A_num_cols), hy(A_num_rows);
+  DevBuf<float> t32;
+  DevBuf<uint16_t> t16;
+  // round to FP16 on the GPU (round to nearest even, as the (half) cast does)
+  auto to_half = [&](const float* src, int64_t n, uint16_t* dst) {
+    if (n == 0) return true;
+    if (up(t32, src, n) != BSP_OK || t16.alloc(n) != BSP_OK) return false;
+    f32_to_f16<<<cdiv(n, kTB), kTB>>>(t32.get(), n, t16.get());
+    return down(dst, t16.get(), n) == BSP_OK;
+  };
+  if (!to_half(hA_values32, A_nnz, hv.data()) || !to_half(hX32, A_num_cols, hx.data())) return;
+  if (bsp_cusparse_spmv_fp16(hA_csrOffsets, hA_columns, hv.data(), hx.data(), hy.data(), A_num_rows,
+                             A_num_cols, A_nnz) != 0)
+    return;
+  DevBuf<float> yf;
+  if (A_num_rows == 0 || up(t16, hy.data(), A_num_rows) != BSP_OK || yf.alloc(A_num_rows) != BSP_OK)
+    return;
+  f16_to_f32<<<cdiv(A_num_rows, kTB), kTB>>>(t16.get(), A_num_rows, yf.get());
+  down(hY32, yf.get(), A_num_rows);
+}
+
+void bsp_ref_spmv_fp64(int64_t* A_csrOffsets, int64_t* A_columns, double* A_values, double* x,
+                       double* y, int64_t row, int64_t col, double* avg_time) {
+  if (!A_csrOffsets || row < 0 || !x || !y) return;
+  const int64_t nnz = A_csrOffsets[row];
+  DevCsr c;
+  if (upload_i64(c, A_csrOffsets, A_columns, A_values, row, col, nnz) != BSP_OK) return;
+  bsp_spmv_plan p = nullptr;
+  if (bsp_csr_plan_create(&c.A, BSP_CSR_SCALAR, nullptr, &p) != BSP_OK) return;
+  std::unique_ptr<bsp_spmv_plan_s, bsp_status (*)(bsp_spmv_plan)> own(p, bsp_plan_destroy);
+  DevBuf<double> xd, yd;
+  if (up(xd, x, col) != BSP_OK || yd.alloc(row) != BSP_OK) return;
+  double t = 0;
+  if (timed(0, 1, &t, [&] { return bsp_csr_spmv(p, xd.get(), yd.get(), nullptr); }) != BSP_OK)
+    return;
+  if (avg_time) *avg_time = t;
+  down(y, yd.get(), row);
+}
+
+static void ref_rows_f32(bool half, int* rowptr, int32_t* cols, float* vals, float* x, float* y,
+                         int32_t row, int32_t col) {
+  if (!rowptr || row < 0 || col < 0 || !x || !y) return;
+  const int64_t nnz = rowptr[row];
+  if (nnz < 0 || (nnz && (!cols || !vals))) return;
+  DevBuf<int32_t> rp, ci;
+  DevBuf<float> v, xd, yd;
+  if (up(rp, rowptr, (int64_t)row + 1) != BSP_OK || up(ci, cols, nnz) != BSP_OK ||
+      up(v, vals, nnz) != BSP_OK || up(xd, x, col) != BSP_OK || yd.alloc(row) != BSP_OK)
+    return;
+  if (row == 0) return;
+  if (half)
+    rows_seq_f32<true><<<cdiv(row, kTB), kTB>>>(rp.get(), ci.get(), v.get(), xd.get(), yd.get(), row);
+  else
+    rows_seq_f32<false><<<cdiv(row, kTB), kTB>>>(rp.get(), ci.get(), v.get(), xd.get(), yd.get(), row);
+  if (hipDeviceSynchronize() != hipSuccess) return;
+  down(y, yd.get(), row);
+}
+
+void bsp_ref_spmv_fp32(int* A_csrOffsets, int32_t* A_columns, float* A_values, float* x,
+                       float* y, int32_t row, int32_t col) {
+  ref_rows_f32(false, A_csrOffsets, A_columns, A_values, x, y, row, col);
+}
+
+void bsp_ref_spmv_fp16(int* A_csrOffsets, int32_t* A_columns, float* A_values, float* x,
+                       float* y, int32_t row, int32_t col) {
+  ref_rows_f32(true, A_csrOffsets, A_columns, A_values, x, y, row, col);
 }
 
 }  // extern "C"

@@ -390,14 +390,44 @@ int bsp_cusparse_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* 
 /* Device-level FP16 split SpMV: plan from bsp_csr_plan_create(method
  * BSP_CSR_SPLIT, FP16 values); x FP16 (ncols), y FP32 (nrows). */
 bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_stream s);
-/* Device-level rocSPARSE SpMV (FP64, int32 indices, the given algorithm:
- * 0 default, 2 CSR adaptive, 1 CSR stream) prepared once (buffer and
+/* Replaces int cusparse_spmv(...) (include/cusp.h:3, src/cusparse.cu:35-102)
+ * with rocSPARSE: FP32, one call, y = A*x. */
+int bsp_cusparse_spmv(int* hA_csrOffsets, int64_t* hA_columns, float* hA_values, float* hX,
+                      float* hY, int64_t A_num_rows, int64_t A_num_cols, int64_t A_nnz);
+/* Replaces int cusparse_spmv_fp16(...) (include/cusp.h:5, src/cusparse.cu:200-270)
+ * with rocSPARSE: A and x FP16 (binary16 bit patterns), FP32 compute and
+ * accumulation, y rounded to FP16 (round to nearest even) into hY. */
+int bsp_cusparse_spmv_fp16(int* hA_csrOffsets, int32_t* hA_columns, const uint16_t* hA_values,
+                           const uint16_t* hX, uint16_t* hY, int32_t A_num_rows,
+                           int32_t A_num_cols, int32_t A_nnz);
+/* Replaces void cusparse_spmv_fp16_warpper(...) (include/cusp.h:6,
+ * src/cusparse.cu:272-286): float inputs rounded to FP16, then as above; the
+ * FP16 result is widened back into hY32 (the reference drops it). */
+void bsp_cusparse_spmv_fp16_warpper(int* hA_csrOffsets, int32_t* hA_columns, float* hA_values32,
+                                    float* hX32, float* hY32, int32_t A_num_rows,
+                                    int32_t A_num_cols, int32_t A_nnz);
+
+/* Replace the CPU reference loops of include/ref.h:2-4 (src/ref.cpp:3-37) with
+ * GPU thread-per-row kernels that add each row left to right with no fused
+ * multiply-add, so y is bit-identical to the CPU loop: FP64; FP32; FP16
+ * (inputs rounded to FP16, FP32 sum, y rounded to FP16, returned as float).
+ * avg_time (FP64 only) receives the kernel's host-timed microseconds. */
+void bsp_ref_spmv_fp64(int64_t* A_csrOffsets, int64_t* A_columns, double* A_values, double* x,
+                       double* y, int64_t row, int64_t col, double* avg_time);
+void bsp_ref_spmv_fp32(int* A_csrOffsets, int32_t* A_columns, float* A_values, float* x,
+                       float* y, int32_t row, int32_t col);
+void bsp_ref_spmv_fp16(int* A_csrOffsets, int32_t* A_columns, float* A_values, float* x,
+                       float* y, int32_t row, int32_t col);
+
+/* Device-level rocSPARSE SpMV with int32 indices and the given algorithm
+ * (0 default, 1 CSR stream, 2 CSR adaptive), prepared once (buffer and
  * analysis) and run per call: the vendor point reported next to the
- * library's kernels. */
+ * library's kernels. Value types by A->dtype: F64 (x, y FP64), F32 (x, y
+ * FP32), F16 (x FP16, y FP32, FP32 compute). */
 typedef struct bsp_rocsparse_plan_s* bsp_rocsparse_plan;
 bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
                                      bsp_rocsparse_plan* out);
-bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const double* x, double* y, bsp_stream s);
+bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const void* x, void* y, bsp_stream s);
 bsp_status bsp_rocsparse_plan_destroy(bsp_rocsparse_plan p);
 
 /* Replaces int spmv_fp64_warpper(...) (include/csrspmv.h:2, src/csrspmv.cu:104-134):

@@ -12,7 +12,7 @@
  * Parity pins (tests/test_oracle.py): golden fixtures from the compiled
  * reference dbsr.h (oracle/_ref), closed forms for stencils with x == 1,
  * dense reconstruction of every blocked format, brute-force dense SpMV on
- * small inputs, invariants of the DASP layout. See the table in DESIGN.md.
+ * small inputs, invariants of the DASP layout.
  */
 #include <math.h>
 #include <pthread.h>

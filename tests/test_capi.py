@@ -129,3 +129,16 @@ def test_graph_generator(bsp_host):
     assert not np.any(np.diag(D))
     deg = (D != 0).sum(1)
     np.testing.assert_allclose(D[D != 0], (1 / np.sqrt(np.outer(deg, deg)))[D != 0], rtol=1e-15)
+
+
+def test_compat_header_driver_builds(tmp_path):
+    # include/benchsp_compat.hpp: the reference's C++ names and signatures
+    # (double& avg_time, CSRFormat<double,int32_t>, dbsrMat) compile against
+    # libbsp.so, and its static_asserts pin the struct layouts
+    import subprocess
+    from benchsp_amd import build as b
+    out = tmp_path / "driver"
+    r = subprocess.run([b.HIPCC, "-O0", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                        b.COMPAT_SRC, "-L", b.HERE, "-lbsp", "-o", str(out)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

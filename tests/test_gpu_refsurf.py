@@ -235,3 +235,61 @@ def test_rocsparse_plan_on_device_arrays(oracle, bsp_host):
     finally:
         _L().bsp_rocsparse_plan_destroy(h)
     assert isinstance(desc, CsrDesc)
+
+
+def test_ref_spmv_gpu_bit_identical_to_cpu_loop(oracle, bsp_host):
+    # include/ref.h on the GPU: thread per row, left to right, no FMA
+    for name, A in _inputs(bsp_host):
+        rp64, col64 = A.rowptr.astype(np.int64), A.col.astype(np.int64)
+        x = bsp_host.uniform(A.ncols, 17)
+        y = np.full(A.nrows, np.nan)
+        t = C.c_double(-1)
+        _L().bsp_ref_spmv_fp64(_p(rp64), _p(col64), _p(A.val), _p(x), _p(y), A.nrows, A.ncols,
+                               C.byref(t))
+        assert np.array_equal(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x)), name
+        assert t.value >= 0
+        rp = A.rowptr.astype(np.int32)
+        v32, x32 = A.val.astype(np.float32), x.astype(np.float32)
+        for fn, ref in (("bsp_ref_spmv_fp32", oracle.spmv_f32), ("bsp_ref_spmv_fp16", oracle.spmv_f16)):
+            y32 = np.full(A.nrows, np.nan, np.float32)
+            getattr(_L(), fn)(_p(rp), _p(A.col), _p(v32), _p(x32), _p(y32), A.nrows, A.ncols)
+            assert np.array_equal(y32, ref(A.rowptr, A.col, v32, x32)), (name, fn)
+
+
+def test_cusparse_fp32_fp16_surfaces_are_rocsparse(oracle, bsp_host):
+    for name, A in _inputs(bsp_host):
+        rp = A.rowptr.astype(np.int32)
+        v32 = A.val.astype(np.float32)
+        x32 = bsp_host.uniform(A.ncols, 18).astype(np.float32)
+        absb = oracle.spmv_f64(A.rowptr, A.col, np.abs(v32.astype(np.float64)),
+                               np.abs(x32.astype(np.float64)))
+        # FP32: any summation order, FP32 rounding per step
+        y = np.full(A.nrows, np.nan, np.float32)
+        assert _L().bsp_cusparse_spmv(_p(rp), _p(A.col.astype(np.int64)), _p(v32), _p(x32), _p(y),
+                                      A.nrows, A.ncols, A.nnz) == 0
+        y64 = oracle.spmv_f64(A.rowptr, A.col, v32.astype(np.float64), x32.astype(np.float64))
+        assert np.all(np.abs(y - y64) <= 2.0 ** -20 * absb + 1e-30), name
+        # FP16 in, FP32 compute, FP16 out: within one half ulp of the oracle's
+        # FP16 result plus the FP32 reordering error
+        ref16 = oracle.spmv_f16(A.rowptr, A.col, v32, x32)
+        vh = v32.astype(np.float16)
+        xh = x32.astype(np.float16)
+        yh = np.zeros(A.nrows, np.float16)
+        assert _L().bsp_cusparse_spmv_fp16(_p(rp), _p(A.col), _p(vh), _p(xh), _p(yh), A.nrows,
+                                           A.ncols, A.nnz) == 0
+        tol = 2.0 ** -10 * np.abs(ref16) + 2.0 ** -18 * absb + 2.0 ** -24
+        assert np.all(np.abs(yh.astype(np.float32) - ref16) <= tol), name
+        yw = np.full(A.nrows, np.nan, np.float32)
+        _L().bsp_cusparse_spmv_fp16_warpper(_p(rp), _p(A.col), _p(v32), _p(x32), _p(yw), A.nrows,
+                                            A.ncols, A.nnz)
+        assert np.array_equal(yw, yh.astype(np.float32)), name
+
+
+def test_compat_driver_runs():
+    # the C++ driver over include/benchsp_compat.hpp, built by build()
+    import subprocess
+    from benchsp_amd import build as b
+    assert os.path.exists(b.COMPAT_BIN), "run __graft_entry__.build() first"
+    r = subprocess.run([b.COMPAT_BIN, "96"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr

@@ -150,10 +150,12 @@ def test_bsr4_map_and_balance(oracle, bsp_host):
 def test_bsr4_stencil27_block_count(oracle, bsp_host):
     A = bsp_host.stencil27_64()
     b = oracle.bcsr(A.nrows, A.ncols, A.rowptr, A.col, A.val, 4, 4)
-    # 64 block rows per z-line... closed form: along each axis of a 64^3 grid
-    # a 4-row block touches 6 block columns (the 4 cells +-1 span 6 cells,
-    # hitting 2 or 3 aligned 4-blocks) -- counted directly instead:
-    assert b["nblocks"] == 1_660_600
+    # closed form: a block row is 4 consecutive x cells of one (z, y) line.
+    # For each neighbour line (dz, dy) its columns span x cells 4b-1..4b+4,
+    # i.e. 3 aligned 4-blocks for the 14 inner b and 2 for b = 0 and 15:
+    # 46 blocks per line pair. Summed over z, the number of valid dz is
+    # 62*3 + 2*2 = 190, the same for y: 46 * 190 * 190.
+    assert b["nblocks"] == 46 * 190 * 190 == 1_660_600
 
 
 # ---------------------------------------------------------------- DBSR -----
