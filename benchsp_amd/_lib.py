@@ -85,6 +85,14 @@ VOID = "void"  # restype marker for the reference's void functions
 # name -> (restype, argtypes); restype None means bsp_status (checked).
 _SIGS: dict[str, tuple] = {
     "bsp_csr_spmv_f16": (None, [vp, vp, vp, vp]),
+    "bsp_batch_create": (None, [vp, i64, vp, vp, C.POINTER(vp)]),
+    "bsp_batch_layout": (None, [vp, vp, vp]),
+    "bsp_batch_choice": (None, [vp, vp, vp, vp]),
+    "bsp_batch_info": (None, [vp, i64p, i64p, i64p]),
+    "bsp_batch_run": (None, [vp, C.c_int, vp, vp, vp, vp, vp]),
+    "bsp_batch_graph_capture": (None, [vp, C.c_int, vp, vp, vp, vp]),
+    "bsp_batch_graph_launch": (None, [vp, vp]),
+    "bsp_batch_destroy": (None, [vp]),
     "bsp_rocsparse_plan_create": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(vp)]),
     "bsp_rocsparse_spmv": (None, [vp, vp, vp, vp]),
     "bsp_cusparse_spmv": (C.c_int, [vp, vp, vp, vp, vp, i64, i64, i64]),

@@ -1,0 +1,638 @@
+// Many CSR matrices of uneven size per step: FP64 SpMV and an 8-column FP16
+// SpMM for each, every matrix's work cut into tiles of one persistent launch
+// per operation. A per-matrix autotuner picks the format (CSR or 4x4 BSR) and
+// the tile window (rows per workgroup, hence lanes per row) by timing the
+// candidates; a hipGraph holds a whole step. A plain schedule (one
+// thread-per-row launch per matrix and operation) is kept as the comparison.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "csr_adaptive.hpp"
+
+namespace bsp {
+namespace {
+
+constexpr int kCap = 1024;  // window capacity of adaptive_rows<kCap>
+constexpr int kCols = 8;    // SpMM columns
+constexpr int kKindCsr = 0, kKindBsr = 1;
+constexpr int kMaxGrid = 2048;  // bound on persistent workgroups (8 per CU)
+constexpr int kTileOverhead = 64;  // a tile's fixed cost in units of rows+entries
+const int kWindows[] = {128, 256, 512, 1024};
+
+struct BTile {
+  int32_t m, kind, r0, r1, p0, p1, pad0, pad1;
+};
+
+struct BMat {
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* val;
+  const uint16_t* hval;
+  const int32_t* brp;  // 4x4 BSR (when chosen)
+  const int32_t* bci;
+  const double* bval;
+  int64_t xoff, yoff;
+  int32_t nrows, ncols;
+};
+
+__device__ __forceinline__ float half_to_f32(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+
+// Block rows [br0, br1) (at most four) of a 4x4 BSR, a wave per block row;
+// lane = slot*4 + r, 16 blocks per step, partial sums of equal r added across
+// slots with lane shuffles.
+__device__ __forceinline__ void bsr4_rows(const int32_t* __restrict__ brp,
+                                          const int32_t* __restrict__ bci,
+                                          const double* __restrict__ bval,
+                                          const double* __restrict__ x, double* __restrict__ y,
+                                          int br0, int br1, int nrows, int ncols) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int br = br0 + wave;
+  if (br >= br1) return;
+  const int slot = lane >> 2, r = lane & 3;
+  double acc = 0.0;
+  for (int k = brp[br] + slot, e = brp[br + 1]; k < e; k += 16) {
+    const int c0 = bci[k] * 4;
+    const double* p = bval + (int64_t)k * 16 + r * 4;
+    if (c0 + 4 <= ncols) {
+      acc += ld_stream(&p[0]) * x[c0] + ld_stream(&p[1]) * x[c0 + 1] +
+             ld_stream(&p[2]) * x[c0 + 2] + ld_stream(&p[3]) * x[c0 + 3];
+    } else {
+      for (int c = 0; c < ncols - c0; ++c) acc += ld_stream(&p[c]) * x[c0 + c];
+    }
+  }
+#pragma unroll
+  for (int o = 4; o < kWave; o <<= 1) acc += __shfl_xor(acc, o, 64);
+  const int row = br * 4 + r;
+  if (slot == 0 && row < nrows) y[row] = acc;
+}
+
+// Rows [r0, r1) with entries [p0, p1) of Y = A X, X rows of 8 halves (one
+// 16-byte load), Y rows of 8 floats. Groups of lpr lanes per row as in
+// adaptive_rows; one long row takes the whole workgroup.
+__device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
+                                           const int32_t* __restrict__ col,
+                                           const uint16_t* __restrict__ hval,
+                                           const uint4* __restrict__ X, float* __restrict__ Y,
+                                           int r0, int p0, int r1, int p1) {
+  __shared__ float red[kWG / kWave][kCols];
+  const int tid = threadIdx.x;
+  const int nrows = r1 - r0, nnz = p1 - p0;
+  float acc[kCols];
+  auto fma_row = [&](int i) {
+    const float v = half_to_f32(hval[i]);
+    const uint4 q = X[col[i]];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[2 * k] += v * half_to_f32((uint16_t)(w[k] & 0xffff));
+      acc[2 * k + 1] += v * half_to_f32((uint16_t)(w[k] >> 16));
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < kCols; ++k) acc[k] = 0.f;
+  if (nrows == 1 && nnz > Ad<kCap>::LONG) {
+    for (int i = p0 + tid; i < p1; i += kWG) fma_row(i);
+#pragma unroll
+    for (int k = 0; k < kCols; ++k) {
+      float v = acc[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      acc[k] = v;
+    }
+    if ((tid & (kWave - 1)) == 0)
+#pragma unroll
+      for (int k = 0; k < kCols; ++k) red[tid / kWave][k] = acc[k];
+    __syncthreads();
+    if (tid < kCols)
+      Y[(int64_t)r0 * kCols + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+    return;
+  }
+  int lpr = 1;
+  while (lpr < kWave && nrows * lpr * 2 <= kWG) lpr <<= 1;
+  const int g = tid / lpr, l = tid & (lpr - 1), groups = kWG / lpr;
+  for (int rr = g; rr < nrows; rr += groups) {
+#pragma unroll
+    for (int k = 0; k < kCols; ++k) acc[k] = 0.f;
+    for (int i = rowptr[r0 + rr] + l, e = rowptr[r0 + rr + 1]; i < e; i += lpr) fma_row(i);
+#pragma unroll
+    for (int k = 0; k < kCols; ++k)
+      for (int o = lpr >> 1; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (l == 0) {
+      float4* out = reinterpret_cast<float4*>(Y + (int64_t)(r0 + rr) * kCols);
+      out[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+      out[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+    }
+  }
+}
+
+// Persistent launches: workgroup w (XCD-remapped, so an XCD owns a contiguous
+// run of tiles) runs tiles [wg_start[w], wg_start[w+1]) in order; the cuts
+// balance rows+entries plus a fixed cost per tile. No counters, no atomics:
+// every row of every matrix is written by exactly one tile.
+__global__ __launch_bounds__(kWG) void batch_spmv_f64(const BTile* __restrict__ tiles,
+                                                      const int32_t* __restrict__ wg_start,
+                                                      const BMat* __restrict__ mats,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y) {
+  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  const int t1 = wg_start[w + 1];
+  for (int t = wg_start[w]; t < t1; ++t) {
+    const BTile T = tiles[t];
+    const BMat d = mats[T.m];
+    if (T.kind == kKindCsr)
+      adaptive_rows<kCap, false>(d.rowptr, d.col, d.val, x + d.xoff, y + d.yoff, T.r0, T.p0, T.r1,
+                                 T.p1);
+    else
+      bsr4_rows(d.brp, d.bci, d.bval, x + d.xoff, y + d.yoff, T.r0, T.r1, d.nrows, d.ncols);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kWG) void batch_spmm8_f16(const BTile* __restrict__ tiles,
+                                                       const int32_t* __restrict__ wg_start,
+                                                       const BMat* __restrict__ mats,
+                                                       const uint4* __restrict__ X,
+                                                       float* __restrict__ Y) {
+  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  const int t1 = wg_start[w + 1];
+  for (int t = wg_start[w]; t < t1; ++t) {
+    const BTile T = tiles[t];
+    const BMat d = mats[T.m];
+    spmm8_rows(d.rowptr, d.col, d.hval, X + d.xoff, Y + d.yoff * kCols, T.r0, T.p0, T.r1, T.p1);
+    __syncthreads();
+  }
+}
+
+// The plain schedule: thread per row, one launch per matrix.
+__global__ __launch_bounds__(kWG) void plain_spmv_f64(const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y, int n) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) s += val[j] * x[col[j]];
+  y[r] = s;
+}
+
+__global__ __launch_bounds__(kWG) void plain_spmm8_f16(const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const uint16_t* __restrict__ hval,
+                                                       const uint4* __restrict__ X,
+                                                       float* __restrict__ Y, int n) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float acc[kCols] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) {
+    const float v = half_to_f32(hval[j]);
+    const uint4 q = X[col[j]];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[2 * k] += v * half_to_f32((uint16_t)(w[k] & 0xffff));
+      acc[2 * k + 1] += v * half_to_f32((uint16_t)(w[k] >> 16));
+    }
+  }
+  float4* out = reinterpret_cast<float4*>(Y + (int64_t)r * kCols);
+  out[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  out[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+}
+
+__global__ void f64_to_f16(const double* __restrict__ a, int64_t n, uint16_t* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = __builtin_bit_cast(uint16_t, (_Float16)a[i]);
+}
+
+// ---- host-side schedule ------------------------------------------------------
+
+// Tile starts of one matrix for window c: the rule of csr_block_flags (a new
+// tile where the merge-path window of u(r) = rowptr[r] + r changes, and around
+// every row longer than c/2).
+void csr_tiles(const std::vector<int32_t>& rp, int m, int c, std::vector<BTile>& out) {
+  const int n = (int)rp.size() - 1;
+  const int longrow = c / 2;
+  int start = 0;
+  for (int r = 1; r <= n; ++r) {
+    bool cut = r == n;
+    if (!cut) {
+      const int64_t key = ((int64_t)rp[r] + r) / c, keyp = ((int64_t)rp[r - 1] + r - 1) / c;
+      cut = key != keyp || rp[r + 1] - rp[r] > longrow || rp[r] - rp[r - 1] > longrow;
+    }
+    if (cut) {
+      out.push_back(BTile{m, kKindCsr, start, r, rp[start], rp[r], 0, 0});
+      start = r;
+    }
+  }
+}
+
+void bsr_tiles(const std::vector<int32_t>& brp, int m, std::vector<BTile>& out) {
+  const int brows = (int)brp.size() - 1;
+  for (int b = 0; b < brows; b += kWG / kWave)
+    out.push_back(BTile{m, kKindBsr, b, std::min(brows, b + kWG / kWave), brp[b],
+                        brp[std::min(brows, b + kWG / kWave)], 0, 0});
+}
+
+int64_t tile_weight(const BTile& t) {
+  return t.kind == kKindCsr ? (int64_t)(t.r1 - t.r0) + (t.p1 - t.p0) + kTileOverhead
+                            : (int64_t)(t.p1 - t.p0) * 16 + kTileOverhead;
+}
+
+// Contiguous per-workgroup tile ranges with balanced weight.
+std::vector<int32_t> wg_cuts(const std::vector<BTile>& tiles, int grid) {
+  std::vector<int64_t> pre(tiles.size() + 1, 0);
+  for (size_t i = 0; i < tiles.size(); ++i) pre[i + 1] = pre[i] + tile_weight(tiles[i]);
+  std::vector<int32_t> cut(grid + 1);
+  const int64_t total = pre.back();
+  for (int w = 0; w <= grid; ++w) {
+    const int64_t target = total * w / grid;
+    cut[w] = (int32_t)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+  }
+  cut[0] = 0;
+  cut[grid] = (int32_t)tiles.size();
+  return cut;
+}
+
+struct Schedule {
+  DevBuf<BTile> tiles;
+  DevBuf<int32_t> cuts;
+  int64_t ntiles = 0;
+  int grid = 0;
+  int resident = kMaxGrid;  // workgroups the device holds at once for the kernel
+  bsp_status upload(const std::vector<BTile>& t, hipStream_t s) {
+    ntiles = (int64_t)t.size();
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, ntiles));
+    std::vector<int32_t> c = wg_cuts(t, grid);
+    BSP_TRY(tiles.alloc(t.size()));
+    BSP_TRY(cuts.alloc(c.size()));
+    if (!t.empty())
+      BSP_HIP(hipMemcpyAsync(tiles.get(), t.data(), sizeof(BTile) * t.size(), hipMemcpyHostToDevice, s));
+    BSP_HIP(hipMemcpyAsync(cuts.get(), c.data(), sizeof(int32_t) * c.size(), hipMemcpyHostToDevice, s));
+    BSP_HIP(hipStreamSynchronize(s));
+    return BSP_OK;
+  }
+};
+
+}  // namespace
+}  // namespace bsp
+
+using namespace bsp;
+
+struct bsp_batch_s {
+  std::vector<bsp_csr> A;
+  std::vector<int64_t> row_off, col_off;
+  std::vector<int32_t> fmt, win_spmv, win_spmm;
+  std::vector<bsp_bsr> bsr;  // owned, per matrix (null unless chosen)
+  DevBuf<uint16_t> hval;     // FP16 copies of every matrix's values
+  DevBuf<BMat> mats;
+  Schedule spmv, spmm;
+  hipStream_t cap = nullptr;  // capture stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~bsp_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    for (bsp_bsr h : bsr)
+      if (h) bsp_bsr_destroy(h);
+  }
+};
+
+namespace {
+
+bsp_status upload_mats(bsp_batch_s* b, hipStream_t s) {
+  const size_t M = b->A.size();
+  std::vector<BMat> h(M);
+  int64_t hoff = 0;
+  for (size_t m = 0; m < M; ++m) {
+    const bsp_csr& A = b->A[m];
+    BMat d{};
+    d.rowptr = A.rowptr;
+    d.col = A.colidx;
+    d.val = static_cast<const double*>(A.val);
+    d.hval = b->hval.get() + hoff;
+    d.xoff = b->col_off[m];
+    d.yoff = b->row_off[m];
+    d.nrows = (int32_t)A.nrows;
+    d.ncols = (int32_t)A.ncols;
+    if (b->bsr[m]) {
+      bsp_bsr_view v;
+      bsp_bsr_view_get(b->bsr[m], &v);
+      d.brp = v.rowptr;
+      d.bci = v.colidx;
+      d.bval = v.val;
+    }
+    h[m] = d;
+    hoff += A.nnz;
+  }
+  BSP_TRY(b->mats.alloc(M));
+  BSP_HIP(hipMemcpyAsync(b->mats.get(), h.data(), sizeof(BMat) * M, hipMemcpyHostToDevice, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
+void launch_spmv(const Schedule& S, const BMat* mats, const double* x, double* y, hipStream_t s) {
+  if (S.ntiles)
+    batch_spmv_f64<<<S.grid, kWG, 0, s>>>(S.tiles.get(), S.cuts.get(), mats, x, y);
+}
+
+void launch_spmm(const Schedule& S, const BMat* mats, const uint16_t* X, float* Y, hipStream_t s) {
+  if (S.ntiles)
+    batch_spmm8_f16<<<S.grid, kWG, 0, s>>>(S.tiles.get(), S.cuts.get(), mats,
+                                           reinterpret_cast<const uint4*>(X), Y);
+}
+
+// Workgroups of `kernel` resident on the whole device at once (one full
+// wave of the persistent grid), capped at kMaxGrid.
+template <class K>
+bsp_status resident_grid(K kernel, int* out) {
+  int dev = 0, cus = 0, per = 0;
+  BSP_HIP(hipGetDevice(&dev));
+  BSP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  BSP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kWG, 0));
+  *out = std::max(1, std::min(kMaxGrid, cus * std::max(per, 1)));
+  return BSP_OK;
+}
+
+// Time of one persistent launch over `reps` copies of one matrix's tiles
+// (enough copies to give every workgroup of the grid work): the matrix's
+// throughput cost per copy inside a full launch.
+bsp_status time_candidate(const std::vector<BTile>& one, bool spmm, int resident,
+                          const BMat* mats, const double* x, double* y, const uint16_t* X, float* Y,
+                          hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* per_copy) {
+  const int64_t reps = std::max<int64_t>(1, cdiv(2 * resident, (int64_t)one.size()));
+  std::vector<BTile> t;
+  t.reserve(one.size() * reps);
+  for (int64_t r = 0; r < reps; ++r) t.insert(t.end(), one.begin(), one.end());
+  Schedule S;
+  S.resident = resident;
+  BSP_TRY(S.upload(t, s));
+  float best = 1e30f;
+  for (int it = 0; it < 4; ++it) {
+    BSP_HIP(hipEventRecord(e0, s));
+    if (spmm)
+      launch_spmm(S, mats, X, Y, s);
+    else
+      launch_spmv(S, mats, x, y, s);
+    BSP_HIP(hipEventRecord(e1, s));
+    BSP_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    BSP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (it > 0) best = std::min(best, ms);
+  }
+  BSP_HIP(hipGetLastError());
+  *per_copy = best / (float)reps;
+  return BSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_options* opt,
+                            bsp_stream stream, bsp_batch* out) {
+  BSP_REQUIRE(mats && out && count > 0 && count < (int64_t(1) << 31), "bad batch arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int tune = opt ? opt->tune : 0;
+  BSP_REQUIRE(tune >= 0 && tune <= 2, "tune must be 0, 1 or 2");
+  auto b = std::make_unique<bsp_batch_s>();
+  const size_t M = (size_t)count;
+  b->A.assign(mats, mats + M);
+  b->row_off.assign(M + 1, 0);
+  b->col_off.assign(M + 1, 0);
+  int64_t total_nnz = 0, max_rows = 0, max_cols = 0;
+  for (size_t m = 0; m < M; ++m) {
+    const bsp_csr& A = b->A[m];
+    BSP_REQUIRE(A.dtype == BSP_F64, "batch matrices need FP64 values");
+    BSP_REQUIRE(A.nrows >= 0 && A.ncols >= 0 && A.nnz >= 0 && A.nnz < (int64_t(1) << 31) &&
+                    A.nrows < (int64_t(1) << 31) && A.ncols < (int64_t(1) << 31),
+                "batch matrix sizes must fit int32");
+    BSP_REQUIRE(A.nrows == 0 || A.rowptr, "null rowptr");
+    BSP_REQUIRE(A.nnz == 0 || (A.colidx && A.val), "null colidx/val");
+    b->row_off[m + 1] = b->row_off[m] + A.nrows;
+    b->col_off[m + 1] = b->col_off[m] + A.ncols;
+    total_nnz += A.nnz;
+    max_rows = std::max(max_rows, A.nrows);
+    max_cols = std::max(max_cols, A.ncols);
+  }
+  BSP_REQUIRE(b->row_off[M] < (int64_t(1) << 31) && b->col_off[M] < (int64_t(1) << 31),
+              "batch total rows/columns must fit int32");
+  b->fmt.assign(M, BSP_BATCH_CSR);
+  b->win_spmv.assign(M, kCap);
+  b->win_spmm.assign(M, kCap);
+  b->bsr.assign(M, nullptr);
+  BSP_TRY(resident_grid(batch_spmv_f64, &b->spmv.resident));
+  BSP_TRY(resident_grid(batch_spmm8_f16, &b->spmm.resident));
+
+  // FP16 values (round to nearest even) for the SpMM
+  BSP_TRY(b->hval.alloc(total_nnz));
+  {
+    int64_t off = 0;
+    for (size_t m = 0; m < M; ++m) {
+      const bsp_csr& A = b->A[m];
+      if (A.nnz)
+        f64_to_f16<<<cdiv(A.nnz, kWG), kWG, 0, s>>>(static_cast<const double*>(A.val), A.nnz,
+                                                     b->hval.get() + off);
+      off += A.nnz;
+    }
+    BSP_HIP(hipGetLastError());
+  }
+  // host copies of the row pointers (the schedule is cut on the host)
+  std::vector<std::vector<int32_t>> rp(M);
+  for (size_t m = 0; m < M; ++m) {
+    rp[m].resize(b->A[m].nrows + 1);
+    if (b->A[m].nrows)
+      BSP_HIP(hipMemcpyAsync(rp[m].data(), b->A[m].rowptr, sizeof(int32_t) * rp[m].size(),
+                             hipMemcpyDeviceToHost, s));
+    else
+      rp[m][0] = 0;
+  }
+  BSP_HIP(hipStreamSynchronize(s));
+
+  if (tune == 1 || tune == 2) {
+    // 4x4 BSR is a candidate only where it moves no more bytes than CSR
+    // (>= 11 of 16 block entries filled); the others are freed at once.
+    for (size_t m = 0; m < M; ++m) {
+      const bsp_csr& A = b->A[m];
+      if (A.nnz < 16 * 64) continue;
+      bsp_bsr h = nullptr;
+      BSP_TRY(bsp_bsr_build(&A, 4, 4, stream, &h));
+      bsp_bsr_view v;
+      bsp_bsr_view_get(h, &v);
+      if (v.avg_nnz >= 11.0)
+        b->bsr[m] = h;
+      else
+        bsp_bsr_destroy(h);
+    }
+  }
+  if (tune == 2)  // every eligible matrix in 4x4 BSR, no timing
+    for (size_t m = 0; m < M; ++m)
+      if (b->bsr[m]) b->fmt[m] = BSP_BATCH_BSR4;
+  if (tune == 1) {
+    BSP_TRY(upload_mats(b.get(), s));
+    DevBuf<double> x, y;
+    DevBuf<uint16_t> X;
+    DevBuf<float> Y;
+    BSP_TRY(x.alloc(b->col_off[M]));
+    BSP_TRY(y.alloc(b->row_off[M]));
+    BSP_TRY(X.alloc(b->col_off[M] * kCols));
+    BSP_TRY(Y.alloc(b->row_off[M] * kCols));
+    BSP_HIP(hipMemsetAsync(x.get(), 0, sizeof(double) * std::max<int64_t>(1, b->col_off[M]), s));
+    BSP_HIP(hipMemsetAsync(X.get(), 0, sizeof(uint16_t) * std::max<int64_t>(1, b->col_off[M] * kCols), s));
+    hipEvent_t e0, e1;
+    BSP_HIP(hipEventCreate(&e0));
+    BSP_HIP(hipEventCreate(&e1));
+    bsp_status st = BSP_OK;
+    for (size_t m = 0; m < M && st == BSP_OK; ++m) {
+      if (b->A[m].nrows == 0) continue;
+      float best_v = 1e30f, best_m = 1e30f;
+      for (int c : kWindows) {
+        std::vector<BTile> one;
+        csr_tiles(rp[m], (int)m, c, one);
+        float t = 0;
+        st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
+        if (st != BSP_OK) break;
+        if (t < best_v) best_v = t, b->win_spmv[m] = c, b->fmt[m] = BSP_BATCH_CSR;
+        st = time_candidate(one, true, b->spmm.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
+        if (st != BSP_OK) break;
+        if (t < best_m) best_m = t, b->win_spmm[m] = c;
+      }
+      if (st == BSP_OK && b->bsr[m]) {
+        bsp_bsr_view v;
+        bsp_bsr_view_get(b->bsr[m], &v);
+        std::vector<int32_t> brp(v.brows + 1);
+        BSP_HIP(hipMemcpy(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost));
+        std::vector<BTile> one;
+        bsr_tiles(brp, (int)m, one);
+        float t = 0;
+        st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
+        if (st == BSP_OK && t < best_v) best_v = t, b->fmt[m] = BSP_BATCH_BSR4;
+      }
+      if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR4 && b->bsr[m]) {
+        bsp_bsr_destroy(b->bsr[m]);
+        b->bsr[m] = nullptr;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    BSP_TRY(st);
+  }
+  BSP_TRY(upload_mats(b.get(), s));
+
+  std::vector<BTile> tv, tm;
+  for (size_t m = 0; m < M; ++m) {
+    if (b->A[m].nrows == 0) continue;
+    if (b->fmt[m] == BSP_BATCH_BSR4) {
+      bsp_bsr_view v;
+      bsp_bsr_view_get(b->bsr[m], &v);
+      std::vector<int32_t> brp(v.brows + 1);
+      BSP_HIP(hipMemcpy(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost));
+      bsr_tiles(brp, (int)m, tv);
+    } else {
+      csr_tiles(rp[m], (int)m, b->win_spmv[m], tv);
+    }
+    csr_tiles(rp[m], (int)m, b->win_spmm[m], tm);
+  }
+  BSP_TRY(b->spmv.upload(tv, s));
+  BSP_TRY(b->spmm.upload(tm, s));
+  *out = b.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_layout(bsp_batch b, int64_t* row_off, int64_t* col_off) {
+  BSP_REQUIRE(b, "null batch");
+  if (row_off) std::copy(b->row_off.begin(), b->row_off.end(), row_off);
+  if (col_off) std::copy(b->col_off.begin(), b->col_off.end(), col_off);
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_choice(bsp_batch b, int32_t* format, int32_t* spmv_window,
+                            int32_t* spmm_window) {
+  BSP_REQUIRE(b, "null batch");
+  if (format) std::copy(b->fmt.begin(), b->fmt.end(), format);
+  if (spmv_window) std::copy(b->win_spmv.begin(), b->win_spmv.end(), spmv_window);
+  if (spmm_window) std::copy(b->win_spmm.begin(), b->win_spmm.end(), spmm_window);
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_info(bsp_batch b, int64_t* spmv_tiles, int64_t* spmm_tiles, int64_t* grid) {
+  BSP_REQUIRE(b, "null batch");
+  if (spmv_tiles) *spmv_tiles = b->spmv.ntiles;
+  if (spmm_tiles) *spmm_tiles = b->spmm.ntiles;
+  if (grid) *grid = std::max(b->spmv.grid, b->spmm.grid);
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_run(bsp_batch b, int schedule, const double* x, double* y, const uint16_t* X,
+                         float* Y, bsp_stream stream) {
+  BSP_REQUIRE(b, "null batch");
+  BSP_REQUIRE((x == nullptr) == (y == nullptr) && (X == nullptr) == (Y == nullptr),
+              "x/y and X/Y come in pairs");
+  hipStream_t s = (hipStream_t)stream;
+  if (schedule == BSP_BATCH_PERSISTENT) {
+    if (x) launch_spmv(b->spmv, b->mats.get(), x, y, s);
+    if (X) launch_spmm(b->spmm, b->mats.get(), X, Y, s);
+  } else if (schedule == BSP_BATCH_PLAIN) {
+    int64_t hoff = 0;
+    for (size_t m = 0; m < b->A.size(); ++m) {
+      const bsp_csr& A = b->A[m];
+      const int n = (int)A.nrows;
+      if (n > 0) {
+        if (x)
+          plain_spmv_f64<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx,
+                                                      static_cast<const double*>(A.val),
+                                                      x + b->col_off[m], y + b->row_off[m], n);
+        if (X)
+          plain_spmm8_f16<<<cdiv(n, kWG), kWG, 0, s>>>(
+              A.rowptr, A.colidx, b->hval.get() + hoff,
+              reinterpret_cast<const uint4*>(X) + b->col_off[m], Y + b->row_off[m] * kCols, n);
+      }
+      hoff += A.nnz;
+    }
+  } else {
+    return fail(BSP_ERR_INVALID, "unknown batch schedule");
+  }
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_graph_capture(bsp_batch b, int schedule, const double* x, double* y,
+                                   const uint16_t* X, float* Y) {
+  BSP_REQUIRE(b, "null batch");
+  if (b->exec) {
+    BSP_HIP(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    BSP_HIP(hipGraphDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  if (!b->cap) BSP_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+  BSP_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+  bsp_status st = bsp_batch_run(b, schedule, x, y, X, Y, b->cap);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->cap, &g);
+  BSP_TRY(st);
+  if (e != hipSuccess) return fail(BSP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  BSP_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_graph_launch(bsp_batch b, bsp_stream stream) {
+  BSP_REQUIRE(b && b->exec, "no captured graph");
+  BSP_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_destroy(bsp_batch b) {
+  delete b;
+  return BSP_OK;
+}
+
+}  // extern "C"

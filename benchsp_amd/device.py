@@ -282,3 +282,72 @@ class BlockSpmm:
                 self._h = None
         except Exception:
             pass
+
+
+class BatchOptions(C.Structure):
+    """struct bsp_batch_options."""
+    _fields_ = [("tune", C.c_int32), ("reserved0", C.c_int32), ("reserved", i64 * 7)]
+
+
+PERSISTENT, PLAIN = 0, 1
+
+
+class Batch:
+    """Many device CSRs run together (bsp_batch_*): per step an FP64 SpMV and
+    an 8-column FP16 SpMM (FP32 accumulation) for every matrix, over
+    concatenated vectors laid out by `row_off` / `col_off`."""
+
+    COLS = 8
+
+    def __init__(self, mats: list[DeviceCsr], tune: bool = True):
+        self.mats = mats  # keeps the borrowed device arrays alive
+        descs = (CsrDesc * len(mats))(*[m.desc() for m in mats])
+        opt = BatchOptions(int(tune), 0)
+        h = C.c_void_p()
+        lib().bsp_batch_create(descs, len(mats), C.byref(opt), stream_handle(), C.byref(h))
+        self._h = h
+        M = len(mats)
+        ro, co = np.zeros(M + 1, np.int64), np.zeros(M + 1, np.int64)
+        lib().bsp_batch_layout(h, ro.ctypes.data, co.ctypes.data)
+        self.row_off, self.col_off = ro, co
+        self.nrows, self.ncols = int(ro[-1]), int(co[-1])
+
+    def choice(self) -> dict:
+        M = len(self.mats)
+        f, v, m = (np.zeros(M, np.int32) for _ in range(3))
+        lib().bsp_batch_choice(self._h, f.ctypes.data, v.ctypes.data, m.ctypes.data)
+        return dict(format=f, spmv_window=v, spmm_window=m)
+
+    def info(self) -> dict:
+        a, b, g = i64(), i64(), i64()
+        lib().bsp_batch_info(self._h, C.byref(a), C.byref(b), C.byref(g))
+        return dict(spmv_tiles=a.value, spmm_tiles=b.value, grid=g.value)
+
+    def _check(self, x, y, X, Y):
+        if x is not None:
+            assert x.dtype == torch.float64 and x.is_contiguous() and x.numel() == self.ncols
+            assert y.dtype == torch.float64 and y.is_contiguous() and y.numel() == self.nrows
+        if X is not None:
+            assert X.dtype == torch.float16 and X.is_contiguous()
+            assert tuple(X.shape) == (self.ncols, self.COLS) and X.data_ptr() % 16 == 0
+            assert Y.dtype == torch.float32 and Y.is_contiguous()
+            assert tuple(Y.shape) == (self.nrows, self.COLS) and Y.data_ptr() % 16 == 0
+
+    def run(self, x=None, y=None, X=None, Y=None, schedule: int = PERSISTENT):
+        self._check(x, y, X, Y)
+        lib().bsp_batch_run(self._h, schedule, ptr(x), ptr(y), ptr(X), ptr(Y), stream_handle())
+
+    def capture(self, x=None, y=None, X=None, Y=None, schedule: int = PERSISTENT):
+        self._check(x, y, X, Y)
+        lib().bsp_batch_graph_capture(self._h, schedule, ptr(x), ptr(y), ptr(X), ptr(Y))
+
+    def replay(self):
+        lib().bsp_batch_graph_launch(self._h, stream_handle())
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

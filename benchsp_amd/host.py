@@ -128,3 +128,22 @@ def batch_sizes(count: int = 512, lo: int = 1000, hi: int = 65536, seed: int = 5
     """n = round(exp(U[ln lo, ln hi))) per matrix, from the splitmix64 stream."""
     u = (uniform(count, seed) + 1.0) * 0.5
     return [int(round(math.exp(math.log(lo) + ui * (math.log(hi) - math.log(lo))))) for ui in u]
+
+
+def batch_mix(count: int = 512, lo: int = 1000, hi: int = 65536, seed: int = 5) -> list[Csr]:
+    """The batched workload: sizes from batch_sizes, kinds by index mod 3 --
+    2-D 5-point stencil of side round(sqrt(n)), 3-D 7-point stencil of side
+    round(cbrt(n)), random with lognormal row lengths (mean 12, sigma 1.2, cap
+    min(n, 5000)) -- each matrix seeded by its index."""
+    out = []
+    for i, n in enumerate(batch_sizes(count, lo, hi, seed)):
+        k = i % 3
+        if k == 0:
+            s = max(2, int(round(math.sqrt(n))))
+            out.append(stencil5(s, s))
+        elif k == 1:
+            s = max(2, int(round(n ** (1.0 / 3.0))))
+            out.append(stencil3d(s, s, s, 7, seed + i))
+        else:
+            out.append(powerlaw(n, 12.0, 1.2, min(n, 5000), seed + i))
+    return out

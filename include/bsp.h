@@ -390,6 +390,47 @@ int bsp_cusparse_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* 
 /* Device-level FP16 split SpMV: plan from bsp_csr_plan_create(method
  * BSP_CSR_SPLIT, FP16 values); x FP16 (ncols), y FP32 (nrows). */
 bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_stream s);
+/* ---- Batched SpMV + SpMM over many matrices --------------------------------
+ * No counterpart in the reference (its drivers run one matrix per process).
+ * A batch borrows `count` device CSRs (FP64 values, int32 indices). One step
+ * is y = A_m x_m (FP64) and Y_m = A_m X_m (A in FP16, X 8 columns of FP16,
+ * FP32 accumulation and output) for every m. Vectors are concatenated: x_m
+ * starts at col_off[m] of x, y_m at row_off[m] of y; X and Y are row-major
+ * with 8 entries per row (X: 16-byte aligned). The persistent schedule runs
+ * each operation as one launch over tiles of all matrices; the plain
+ * schedule launches a thread-per-row kernel per matrix and operation. With
+ * tune = 1 the builder times, per matrix, the CSR tile windows {128, 256,
+ * 512, 1024} rows+entries (which set rows per workgroup and lanes per row)
+ * and, where 4x4 blocks are at least 11/16 full, the 4x4 BSR format, each in
+ * a full persistent launch, and keeps the fastest; tune = 0 uses CSR windows
+ * of 1024; tune = 2 puts every eligible matrix in 4x4 BSR without timing.
+ * A captured graph holds one step; replaying it needs no host work. */
+typedef struct bsp_batch_s* bsp_batch;
+typedef struct bsp_batch_options {
+  int32_t tune;
+  int32_t reserved0;
+  int64_t reserved[7];
+} bsp_batch_options;
+enum { BSP_BATCH_CSR = 0, BSP_BATCH_BSR4 = 1 };
+enum { BSP_BATCH_PERSISTENT = 0, BSP_BATCH_PLAIN = 1 };
+bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_options* opt,
+                            bsp_stream s, bsp_batch* out);
+/* row_off, col_off: count+1 entries each. */
+bsp_status bsp_batch_layout(bsp_batch b, int64_t* row_off, int64_t* col_off);
+/* Per matrix: BSP_BATCH_CSR/BSR4 and the CSR windows chosen (count each). */
+bsp_status bsp_batch_choice(bsp_batch b, int32_t* format, int32_t* spmv_window,
+                            int32_t* spmm_window);
+bsp_status bsp_batch_info(bsp_batch b, int64_t* spmv_tiles, int64_t* spmm_tiles, int64_t* grid);
+/* One step on stream s; x/y or X/Y may both be NULL to skip that operation. */
+bsp_status bsp_batch_run(bsp_batch b, int schedule, const double* x, double* y, const uint16_t* X,
+                         float* Y, bsp_stream s);
+/* Captures one step (same arguments) into a graph owned by the batch,
+ * replacing any earlier one; bsp_batch_graph_launch replays it on s. */
+bsp_status bsp_batch_graph_capture(bsp_batch b, int schedule, const double* x, double* y,
+                                   const uint16_t* X, float* Y);
+bsp_status bsp_batch_graph_launch(bsp_batch b, bsp_stream s);
+bsp_status bsp_batch_destroy(bsp_batch b);
+
 /* Replaces int cusparse_spmv(...) (include/cusp.h:3, src/cusparse.cu:35-102)
  * with rocSPARSE: FP32, one call, y = A*x. */
 int bsp_cusparse_spmv(int* hA_csrOffsets, int64_t* hA_columns, float* hA_values, float* hX,

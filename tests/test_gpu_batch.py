@@ -1,0 +1,138 @@
+"""Batched FP64 SpMV + 8-column FP16 SpMM over many matrices (bsp_batch_*):
+persistent schedule (fixed, autotuned, forced 4x4 BSR), plain schedule and
+graph replays, each matrix against the oracle."""
+import numpy as np
+import pytest
+
+from benchsp_amd.host import Csr
+from tests._parity import abs_spmv, assert_spmv_close
+
+pytestmark = pytest.mark.gpu
+COLS = 8
+
+
+def _mix(bsp_host):
+    rng = np.random.default_rng(8)
+    mats = bsp_host.batch_mix(count=30, lo=200, hi=6000, seed=11)
+    # a matrix with one very long row and empty rows
+    lens = np.concatenate([[0, 3, 2000, 0], rng.integers(0, 6, 300), [700, 1]])
+    rp = np.zeros(len(lens) + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(2500, L, replace=False)) for L in lens]).astype(np.int32)
+    mats.append(Csr(len(lens), 2500, rp, col, rng.uniform(-1, 1, int(rp[-1]))))
+    # dense 4x4 blocks (eligible for the BSR candidate), ragged last block row/col
+    mats.append(bsp_host.blocksparse(40, 37, 4, 600, seed=3))
+    B = mats[-1]
+    keep = B.nrows - 2, B.ncols - 3
+    d = B.to_dense()[: keep[0], : keep[1]]
+    r, c = np.nonzero(d)
+    rp = np.zeros(keep[0] + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    mats[-1] = Csr(keep[0], keep[1], np.cumsum(rp), c.astype(np.int32), d[r, c])
+    mats.append(Csr(0, 5, np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0)))
+    return mats
+
+
+def _vectors(bsp_host, mats, seed):
+    xs = [bsp_host.uniform(A.ncols, seed + i) for i, A in enumerate(mats)]
+    Xs = [bsp_host.normal(A.ncols * COLS, seed + 1000 + i).reshape(A.ncols, COLS).astype(np.float16)
+          for i, A in enumerate(mats)]
+    return xs, Xs
+
+
+def _check(oracle, mats, xs, Xs, y, Y, B):
+    ro = B.row_off
+    for i, A in enumerate(mats):
+        yi = y[ro[i]:ro[i + 1]]
+        assert_spmv_close(yi, oracle.spmv_f64(A.rowptr, A.col, A.val, xs[i]),
+                          abs_spmv(oracle, A, xs[i]), what=f"spmv matrix {i}")
+        # FP16 A and X, FP32 accumulation: vs FP64 on the rounded operands
+        a16 = A.val.astype(np.float16).astype(np.float64)
+        X64 = Xs[i].astype(np.float64)
+        ref = oracle.csr_spmm_f64(A.rowptr, A.col, a16, X64)
+        bound = oracle.csr_spmm_f64(A.rowptr, A.col, np.abs(a16), np.abs(X64))
+        Yi = Y[ro[i]:ro[i + 1]]
+        assert np.all(np.abs(Yi - ref) <= 1e-5 * bound + 1e-30), f"spmm matrix {i}"
+
+
+def _device(mats, xs, Xs):
+    import torch
+    from benchsp_amd.device import DeviceCsr
+    dm = [DeviceCsr.from_host(A) for A in mats]
+    x = torch.from_numpy(np.concatenate(xs)).cuda()
+    X = torch.from_numpy(np.concatenate(Xs).reshape(-1, COLS)).cuda()
+    return dm, x, X
+
+
+@pytest.mark.parametrize("tune", [0, 1, 2])
+def test_batch_persistent(oracle, bsp_host, tune):
+    import torch
+    from benchsp_amd.device import Batch
+    mats = _mix(bsp_host)
+    xs, Xs = _vectors(bsp_host, mats, 21)
+    dm, x, X = _device(mats, xs, Xs)
+    B = Batch(dm, tune=tune)
+    ch = B.choice()
+    assert set(np.unique(ch["spmv_window"])) <= {128, 256, 512, 1024}
+    if tune == 2:
+        assert ch["format"][-2] == 1  # the dense-block matrix runs as 4x4 BSR
+    if tune == 0:
+        assert np.all(ch["format"] == 0) and np.all(ch["spmv_window"] == 1024)
+    y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    Y = torch.full((B.nrows, COLS), float("nan"), dtype=torch.float32, device="cuda")
+    B.run(x, y, X, Y)
+    torch.cuda.synchronize()
+    _check(oracle, mats, xs, Xs, y.cpu().numpy(), Y.cpu().numpy(), B)
+
+
+def test_batch_plain_schedule_matches(oracle, bsp_host):
+    import torch
+    from benchsp_amd.device import PLAIN, Batch
+    mats = _mix(bsp_host)
+    xs, Xs = _vectors(bsp_host, mats, 31)
+    dm, x, X = _device(mats, xs, Xs)
+    B = Batch(dm, tune=0)
+    y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    Y = torch.full((B.nrows, COLS), float("nan"), dtype=torch.float32, device="cuda")
+    B.run(x, y, X, Y, schedule=PLAIN)
+    torch.cuda.synchronize()
+    yh = y.cpu().numpy()
+    _check(oracle, mats, xs, Xs, yh, Y.cpu().numpy(), B)
+    for i, A in enumerate(mats):  # thread per row, left to right: the CPU loop
+        assert np.array_equal(yh[B.row_off[i]:B.row_off[i + 1]],
+                              oracle.spmv_f64(A.rowptr, A.col, A.val, xs[i]))
+
+
+def test_batch_graph_replays(oracle, bsp_host):
+    # the graph reads x and X at replay time: change them between replays,
+    # with eager work and synchronisation in between, and check every replay
+    import torch
+    from benchsp_amd.device import Batch
+    mats = _mix(bsp_host)
+    xs, Xs = _vectors(bsp_host, mats, 41)
+    dm, x, X = _device(mats, xs, Xs)
+    B = Batch(dm, tune=1)
+    y = torch.zeros(B.nrows, dtype=torch.float64, device="cuda")
+    Y = torch.zeros(B.nrows, COLS, dtype=torch.float32, device="cuda")
+    B.capture(x, y, X, Y)
+    for rep in range(3):
+        xs, Xs = _vectors(bsp_host, mats, 50 + 7 * rep)
+        x.copy_(torch.from_numpy(np.concatenate(xs)))
+        X.copy_(torch.from_numpy(np.concatenate(Xs).reshape(-1, COLS)))
+        y.fill_(float("nan"))
+        Y.fill_(float("nan"))
+        torch.cuda.synchronize()
+        B.replay()
+        torch.cuda.synchronize()
+        _check(oracle, mats, xs, Xs, y.cpu().numpy(), Y.cpu().numpy(), B)
+        _ = (x * 2).sum().item()  # eager work between replays
+
+
+def test_batch_rejects_bad_input(bsp_host):
+    import torch
+    from benchsp_amd._lib import BspError
+    from benchsp_amd.device import Batch, DeviceCsr
+    A = bsp_host.stencil5(8, 8)
+    d = DeviceCsr.from_host(A, dtype=torch.float32)
+    with pytest.raises(BspError):
+        Batch([d])
