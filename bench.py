@@ -19,6 +19,10 @@ dvals = dC*B^T on A's blocks) = 3 * 2*nnz*N FLOP, on FP16 (and BF16) MFMA
 with FP32 accumulation, against the same three products with the library's
 CSR SpMM in FP32.
 
+Batched leg: 512 matrices of uneven size, an FP64 SpMV and an 8-column FP16
+SpMM each per step, plain per-matrix launches against the persistent
+autotuned schedule captured in a graph.
+
 Prints one JSON line. `--gpus N` with N > 1 is refused: this build measures
 one GPU.
 """
@@ -47,6 +51,7 @@ def parse():
     ap.add_argument("--method", default="adaptive")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmm", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--warm", action="store_true", help="single copy (cache-warm) timing")
@@ -198,6 +203,85 @@ def spmm_leg(args, dev):
     return out
 
 
+def batch_leg(args, dev):
+    """512 matrices of uneven size (n log-uniform in [1000, 65536]; 2-D 5-pt,
+    3-D 7-pt and lognormal-random by index mod 3), each with an FP64 SpMV and
+    an 8-column FP16 SpMM per step. Schedules: plain (a thread-per-row launch
+    per matrix and operation, on the stream and captured in a graph) and the
+    persistent tile launches (fixed 1024 windows, and autotuned), captured."""
+    import numpy as np
+    import torch
+    from benchsp_amd import host
+    from benchsp_amd.device import PERSISTENT, PLAIN, Batch, DeviceCsr
+
+    t0 = time.perf_counter()
+    mats = host.batch_mix(512)
+    dm = [DeviceCsr.from_host(A, dev) for A in mats]
+    nnz = sum(A.nnz for A in mats)
+    nrows = sum(A.nrows for A in mats)
+    ncols = sum(A.ncols for A in mats)
+    spmv_bytes = sum(A.csr_bytes() for A in mats)
+    # FP16 values + int32 columns + row pointers + X (8 halves) + Y (8 floats)
+    spmm_bytes = 6 * nnz + 4 * (nrows + len(mats)) + 16 * ncols + 32 * nrows
+    x = torch.from_numpy(host.uniform(ncols, 300)).to(dev)
+    X = torch.from_numpy(host.normal(ncols * 8, 301).astype(np.float16).reshape(ncols, 8)).to(dev)
+    y = torch.empty(nrows, dtype=torch.float64, device=dev)
+    Y = torch.empty(nrows, 8, dtype=torch.float32, device=dev)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fixed = Batch(dm, tune=0)
+    t_fixed = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tuned = Batch(dm, tune=1)
+    t_tuned = time.perf_counter() - t0
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    res = {}
+    res["plain_stream_ms"] = timed(lambda: fixed.run(x, y, X, Y, schedule=PLAIN))
+    fixed.capture(x, y, X, Y, schedule=PLAIN)
+    res["plain_graph_ms"] = timed(fixed.replay)
+    fixed.capture(x, y, X, Y, schedule=PERSISTENT)
+    res["persistent_fixed_graph_ms"] = timed(fixed.replay)
+    tuned.capture(x, y, X, Y, schedule=PERSISTENT)
+    res["persistent_tuned_graph_ms"] = timed(tuned.replay)
+    res["persistent_tuned_stream_ms"] = timed(lambda: tuned.run(x, y, X, Y))
+    # the two operations on their own (tuned, on the stream)
+    res["tuned_spmv_only_ms"] = timed(lambda: tuned.run(x, y))
+    res["tuned_spmm_only_ms"] = timed(lambda: tuned.run(None, None, X, Y))
+    ch = tuned.choice()
+    best = res["persistent_tuned_graph_ms"]
+    out = {"workload": "512 matrices, n log-uniform [1000, 65536], 5-pt / 7-pt / lognormal "
+                       "mean 12 by index mod 3; step = FP64 SpMV + 8-col FP16 SpMM each",
+           "matrices": len(mats), "nnz": nnz, "rows": nrows,
+           "bytes_per_step": spmv_bytes + spmm_bytes,
+           **{k: round(v, 4) for k, v in res.items()},
+           "eff_GBs_tuned": round((spmv_bytes + spmm_bytes) / (best * 1e-3) / 1e9, 1),
+           "spmv_eff_GBs_tuned": round(spmv_bytes / (res["tuned_spmv_only_ms"] * 1e-3) / 1e9, 1),
+           "speedup_vs_plain_stream": round(res["plain_stream_ms"] / best, 2),
+           "speedup_vs_plain_graph": round(res["plain_graph_ms"] / best, 2),
+           "tuned_over_fixed": round(res["persistent_fixed_graph_ms"] / best, 3),
+           "tuner_choice": {
+               "bsr4": int((ch["format"] == 1).sum()),
+               "spmv_windows": {int(k): int(v) for k, v in
+                                zip(*np.unique(ch["spmv_window"], return_counts=True))},
+               "spmm_windows": {int(k): int(v) for k, v in
+                                zip(*np.unique(ch["spmm_window"], return_counts=True))}},
+           "setup_s": {"generate": round(gen_s, 1), "build_fixed": round(t_fixed, 2),
+                       "build_tuned": round(t_tuned, 2)}}
+    return out
+
+
 def measured_traffic():
     """HBM bytes the adaptive kernel moves per step (27-pt launch + both
     power-law segment launches), from the committed rocprofv3 FETCH_SIZE /
@@ -208,6 +292,10 @@ def measured_traffic():
     with open(p) as f:
         d = json.load(f)
     return int(sum(v["hbm_bytes"] for v in d["launches"].values()))
+
+
+def _progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -221,6 +309,7 @@ def main():
     torch.cuda.set_device(dev)
 
     sp = spmv_leg(args, dev)
+    _progress(f"spmv leg done: {sp['value']:.1f} GB/s")
     out = {
         "metric": METRIC,
         "value": round(sp["value"], 1),
@@ -249,6 +338,10 @@ def main():
     }
     if not args.no_spmm:
         out["spmm_fwd_bwd"] = spmm_leg(args, dev)
+        _progress("block-spmm leg done")
+    if not args.no_batch:
+        out["batched"] = batch_leg(args, dev)
+        _progress("batched leg done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sp["mats"], sp["xs"], args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -80,11 +80,20 @@ class DaspView(C.Structure):
                                    "reg_cid", "irreg_val", "irreg_cid")])
 
 
+class BlockInfo(C.Structure):
+    """struct bsp_blockinfo."""
+    _fields_ = [("nnz", i64), ("blocks", i64), ("blocks_packed", i64), ("avg", f64),
+                ("avg_packed", f64)]
+
+
 VOID = "void"  # restype marker for the reference's void functions
 
 # name -> (restype, argtypes); restype None means bsp_status (checked).
 _SIGS: dict[str, tuple] = {
     "bsp_csr_spmv_f16": (None, [vp, vp, vp, vp]),
+    "bsp_mtx_read": (None, [C.c_char_p, C.POINTER(vp)]),
+    "bsp_coo_to_csr": (None, [i64, i64, i64, vp, vp, vp, C.POINTER(vp)]),
+    "bsp_block_info": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(BlockInfo)]),
     "bsp_batch_create": (None, [vp, i64, vp, vp, C.POINTER(vp)]),
     "bsp_batch_layout": (None, [vp, vp, vp]),
     "bsp_batch_choice": (None, [vp, vp, vp, vp]),

@@ -284,6 +284,16 @@ class BlockSpmm:
             pass
 
 
+def block_info(A: DeviceCsr, block_len: int = 4) -> dict:
+    """b x b block counts with and without left-packing of each row
+    (bsp_block_info)."""
+    from ._lib import BlockInfo
+    out = BlockInfo()
+    d = A.desc()
+    lib().bsp_block_info(C.byref(d), block_len, stream_handle(), C.byref(out))
+    return {k: getattr(out, k) for k, _ in BlockInfo._fields_}
+
+
 class BatchOptions(C.Structure):
     """struct bsp_batch_options."""
     _fields_ = [("tune", C.c_int32), ("reserved0", C.c_int32), ("reserved", i64 * 7)]

@@ -84,6 +84,21 @@ def graph(n: int, avg_deg: float = 50.0, gamma: float = 2.1, symmetric: bool = T
     return _gen(lib().bsp_gen_graph, n, avg_deg, gamma, int(symmetric), seed)
 
 
+def read_mtx(path: str) -> Csr:
+    """MatrixMarket coordinate file -> CSR sorted by (row, column)
+    (bsp_mtx_read)."""
+    return _gen(lib().bsp_mtx_read, path.encode())
+
+
+def coo_to_csr(nrows: int, ncols: int, rows, cols, vals=None) -> Csr:
+    """0-based COO -> CSR sorted by (row, column), duplicates kept in order."""
+    r = np.ascontiguousarray(rows, np.int32)
+    c = np.ascontiguousarray(cols, np.int32)
+    v = None if vals is None else np.ascontiguousarray(vals, np.float64)
+    return _gen(lib().bsp_coo_to_csr, nrows, ncols, r.size, r.ctypes.data, c.ctypes.data,
+                None if v is None else v.ctypes.data)
+
+
 def uniform(n: int, seed: int = 0) -> np.ndarray:
     out = np.empty(max(n, 1), np.float64)
     lib().bsp_gen_uniform(n, seed, out.ctypes.data)

@@ -291,6 +291,26 @@ bsp_status bsp_host_csr_info(bsp_host_csr h, int64_t* nrows, int64_t* ncols, int
 bsp_status bsp_host_csr_copy(bsp_host_csr h, int64_t* rowptr, int32_t* col, double* val);
 void bsp_host_csr_free(bsp_host_csr h);
 
+/* ---- Matrix files and block statistics ------------------------------------ */
+/* Replaces the drivers' reader and assembly (bench/spmvfp64.cpp:36-93, which
+ * use fast_matrix_market): a MatrixMarket coordinate file (real, integer or
+ * pattern = 1.0; general, symmetric or skew-symmetric, the mirrored
+ * off-diagonal entries added) into a host CSR sorted by (row, column).
+ * Duplicate entries are all kept, in file order. */
+bsp_status bsp_mtx_read(const char* path, bsp_host_csr* out);
+/* COO (0-based) to host CSR, the same assembly; vals may be NULL (1.0). */
+bsp_status bsp_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rows,
+                          const int32_t* cols, const double* vals, bsp_host_csr* out);
+/* Replaces bench/blockinfo.cpp:96-119: b x b blocks holding an entry
+ * (`blocks`, avg = nnz / blocks) and blocks after packing each row's entries
+ * to the left (`blocks_packed` = sum over groups of b rows of the longest
+ * row's ceil(len / b)). Computed on the GPU from a device CSR. */
+typedef struct bsp_blockinfo {
+  int64_t nnz, blocks, blocks_packed;
+  double avg, avg_packed;
+} bsp_blockinfo;
+bsp_status bsp_block_info(const bsp_csr* A, int block_len, bsp_stream s, bsp_blockinfo* out);
+
 /* ------------------------------------------------------------------------ */
 /* Reference call surface (host pointers).                                   */
 /* Every function uploads its inputs, builds what the reference builds (on the

@@ -832,3 +832,37 @@ void orc_bsr_spmm_f64(int64_t brows, int32_t bs, const int32_t* browptr, const i
       }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Block statistics: bench/blockinfo.cpp:96-119. `blocks` counts the distinct
+ * (row / b, col / b) keys; `packed` counts the distinct (row / b, k / b)
+ * keys, k being an entry's position within its row (the "recorder" count).
+ * Keys are 64-bit here (the reference's int key row/b*1000005 + col/b
+ * overflows once row/b reaches 2148). */
+static int cmp_i64(const void* a, const void* b) {
+  const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+  return (x > y) - (x < y);
+}
+
+static int64_t distinct_i64(int64_t* k, int64_t n) {
+  if (n == 0) return 0;
+  qsort(k, (size_t)n, sizeof(int64_t), cmp_i64);
+  int64_t d = 1;
+  for (int64_t i = 1; i < n; i++) d += k[i] != k[i - 1];
+  return d;
+}
+
+void orc_blockinfo(int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* col,
+                   int32_t b, int64_t* blocks, int64_t* packed) {
+  const int64_t nnz = rowptr[nrows];
+  const int64_t span = ncols / b + 2;
+  int64_t* k = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+  for (int64_t i = 0; i < nrows; i++)
+    for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++) k[j] = (i / b) * span + col[j] / b;
+  *blocks = distinct_i64(k, nnz);
+  for (int64_t i = 0; i < nrows; i++)
+    for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++)
+      k[j] = (i / b) * span + (j - rowptr[i]) / b;
+  *packed = distinct_i64(k, nnz);
+  free(k);
+}

@@ -55,6 +55,8 @@ def lib():
         L.orc_dasp_build.argtypes = [C.c_int32, _i32p, _i32p, _f64p, C.c_double, C.c_int32]
         L.orc_dasp_build.restype = C.POINTER(_Dasp)
         L.orc_dasp_free.argtypes = [C.POINTER(_Dasp)]
+        L.orc_blockinfo.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, C.c_int32,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -155,6 +157,14 @@ def f16_bits_to_f32(h: int) -> float:
 
 # --------------------------------------------------------------------------
 # Builders
+
+def blockinfo(nrows, ncols, rowptr, col, b: int = 4):
+    """(blocks, packed blocks) of bench/blockinfo.cpp:96-119."""
+    bl, pk = C.c_int64(), C.c_int64()
+    lib().orc_blockinfo(int(nrows), int(ncols), _c64(rowptr), _c32(col), int(b), C.byref(bl),
+                        C.byref(pk))
+    return bl.value, pk.value
+
 
 def split(rowptr, threshold: int = 128):
     """Short/long row maps (src/spmv.cu:131-196)."""

@@ -157,3 +157,15 @@ def test_dasp_spmv(oracle, bsp_host, threshold, longest):
         assert_spmv_close(y, y_ref, abs_spmv(oracle, A, x), what=f"{name} dasp")
         order = D.arrays()["order_rid"]
         assert np.array_equal(yp, y[order])  # Y_val in order_rid order
+
+
+@pytest.mark.parametrize("b", [3, 4, 16])
+def test_block_info(oracle, bsp_host, b):
+    # bench/blockinfo.cpp's two block counts, on the GPU, against the oracle
+    from benchsp_amd.device import DeviceCsr, block_info
+    for name, A in _mats(bsp_host):
+        got = block_info(DeviceCsr.from_host(A), b)
+        blocks, packed = oracle.blockinfo(A.nrows, A.ncols, A.rowptr, A.col, b)
+        assert (got["nnz"], got["blocks"], got["blocks_packed"]) == (A.nnz, blocks, packed), name
+        if blocks:
+            assert got["avg"] == A.nnz / blocks and got["avg_packed"] == A.nnz / packed

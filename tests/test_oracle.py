@@ -336,3 +336,30 @@ def test_dasp_slab_threshold(oracle, L, regular, irreg):
     assert D["blocknum"] == 4  # ceil(8/8) rounded up to a multiple of 4
     assert D["block_ptr"][1] == regular
     assert np.all(np.diff(D["irreg_rpt"]) == irreg)
+
+
+# ----------------------------------------------------------- blockinfo ----
+
+def test_blockinfo_closed_forms(oracle, bsp_host):
+    # 27-pt 64^3, b = 4: blocks as in test_bsr4_stencil27_block_count; packed:
+    # a group is 4 x-cells of one (z, y) line whose longest row has 3a
+    # entries, a = (valid dz)(valid dy) in {9, 6, 4} on 62^2, 4*62 and 4
+    # lines: 16 groups per line times ceil(3a/4) in {7, 5, 3}
+    A = bsp_host.stencil27_64()
+    blocks, packed = oracle.blockinfo(A.nrows, A.ncols, A.rowptr, A.col, 4)
+    assert blocks == 46 * 190 * 190
+    assert packed == 16 * (62 * 62 * 7 + 4 * 62 * 5 + 4 * 3) == 450_560
+
+
+@pytest.mark.parametrize("b", [1, 3, 4, 16])
+def test_blockinfo_brute_force(oracle, b):
+    rng = np.random.default_rng(b)
+    A = _rand_csr(rng, rng.integers(0, 30, 77), 91)
+    blocks, packed = oracle.blockinfo(A.nrows, A.ncols, A.rowptr, A.col, b)
+    rows = np.repeat(np.arange(A.nrows), A.row_lengths())
+    pos = np.arange(A.nnz) - A.rowptr[rows]
+    assert blocks == len({(r // b, c // b) for r, c in zip(rows, A.col)})
+    assert packed == len({(r // b, k // b) for r, k in zip(rows, pos)})
+    L = A.row_lengths()
+    assert packed == sum(int(max(-(-L[g:g + b] // b))) if L[g:g + b].size else 0
+                         for g in range(0, A.nrows, b))
