@@ -20,7 +20,7 @@ constexpr int kCols = 8;    // SpMM columns
 constexpr int kKindCsr = 0, kKindBsr = 1;
 constexpr int kMaxGrid = 2048;  // bound on persistent workgroups (8 per CU)
 constexpr int kTileOverhead = 64;  // a tile's fixed cost in units of rows+entries
-const int kWindows[] = {128, 256, 512, 1024};
+const int kWindows[] = {128, 256, 512, 1024};  // tile windows (rows+entries)
 
 struct BTile {
   int32_t m, kind, r0, r1, p0, p1, pad0, pad1;
@@ -71,21 +71,27 @@ __device__ __forceinline__ void bsr4_rows(const int32_t* __restrict__ brp,
   if (slot == 0 && row < nrows) y[row] = acc;
 }
 
-// Rows [r0, r1) with entries [p0, p1) of Y = A X, X rows of 8 halves (one
-// 16-byte load), Y rows of 8 floats. Groups of lpr lanes per row as in
-// adaptive_rows; one long row takes the whole workgroup.
+// Rows [r0, r1) with entries [p0, p1) of Y = A X (at most kCapM rows and
+// 1.5 kCapM entries, or one row), X rows of 8 halves, Y rows of 8 floats.
+// Entries are streamed coalesced, one lane per entry: a 16-byte load brings
+// the entry's X row, the 8 products go to LDS column-major; then q lanes per
+// (row, column) pair add a row's products (q = the largest power of two
+// <= 8 with rows*8*q <= 256). One long row takes the whole workgroup.
+constexpr int kCapM = 512;
+constexpr int kMaxM = kCapM + kCapM / 2;
+constexpr int kItM = kMaxM / kWG;
+
 __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
                                            const int32_t* __restrict__ col,
                                            const uint16_t* __restrict__ hval,
                                            const uint4* __restrict__ X, float* __restrict__ Y,
                                            int r0, int p0, int r1, int p1) {
+  __shared__ float prod[kCols][kMaxM + 1];  // odd stride: columns in distinct banks
+  __shared__ int32_t rptr[kCapM + 1];
   __shared__ float red[kWG / kWave][kCols];
   const int tid = threadIdx.x;
   const int nrows = r1 - r0, nnz = p1 - p0;
-  float acc[kCols];
-  auto fma_row = [&](int i) {
-    const float v = half_to_f32(hval[i]);
-    const uint4 q = X[col[i]];
+  auto unpack = [](uint4 q, float v, float* acc) {
     const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -93,10 +99,9 @@ __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
       acc[2 * k + 1] += v * half_to_f32((uint16_t)(w[k] >> 16));
     }
   };
-#pragma unroll
-  for (int k = 0; k < kCols; ++k) acc[k] = 0.f;
-  if (nrows == 1 && nnz > Ad<kCap>::LONG) {
-    for (int i = p0 + tid; i < p1; i += kWG) fma_row(i);
+  if (nrows == 1 && nnz > kCapM / 2) {
+    float acc[kCols] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int i = p0 + tid; i < p1; i += kWG) unpack(X[col[i]], half_to_f32(hval[i]), acc);
 #pragma unroll
     for (int k = 0; k < kCols; ++k) {
       float v = acc[k];
@@ -111,21 +116,47 @@ __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
       Y[(int64_t)r0 * kCols + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
     return;
   }
-  int lpr = 1;
-  while (lpr < kWave && nrows * lpr * 2 <= kWG) lpr <<= 1;
-  const int g = tid / lpr, l = tid & (lpr - 1), groups = kWG / lpr;
-  for (int rr = g; rr < nrows; rr += groups) {
+  const int iters = (nnz + kWG - 1) / kWG;
+  int ci[kItM];
+  uint16_t hv[kItM];
+  if (nnz > 0) {
 #pragma unroll
-    for (int k = 0; k < kCols; ++k) acc[k] = 0.f;
-    for (int i = rowptr[r0 + rr] + l, e = rowptr[r0 + rr + 1]; i < e; i += lpr) fma_row(i);
+    for (int k = 0; k < kItM; ++k)
+      if (k < iters) {
+        const int j = min(p0 + tid + k * kWG, p1 - 1);
+        ci[k] = ld_stream(&col[j]);
+        hv[k] = ld_stream(&hval[j]);
+      }
+  }
+  for (int i = tid; i <= nrows; i += kWG) rptr[i] = rowptr[r0 + i] - p0;
+  uint4 xq[kItM];
+  if (nnz > 0) {
 #pragma unroll
-    for (int k = 0; k < kCols; ++k)
-      for (int o = lpr >> 1; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-    if (l == 0) {
-      float4* out = reinterpret_cast<float4*>(Y + (int64_t)(r0 + rr) * kCols);
-      out[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
-      out[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+    for (int k = 0; k < kItM; ++k)
+      if (k < iters) xq[k] = X[ci[k]];
+  }
+  if (nnz > 0) {
+#pragma unroll
+    for (int k = 0; k < kItM; ++k) {
+      const int i = tid + k * kWG;
+      if (k < iters && i < nnz) {
+        float acc[kCols] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        unpack(xq[k], half_to_f32(hv[k]), acc);
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) prod[c][i] = acc[c];
+      }
     }
+  }
+  __syncthreads();
+  int q = 1;
+  while (q < 8 && nrows * kCols * q * 2 <= kWG) q <<= 1;
+  const int sub = tid & (q - 1);
+  for (int pr = tid / q; pr < nrows * kCols; pr += kWG / q) {
+    const int rr = pr >> 3, c = pr & 7;
+    float sum = 0.f;
+    for (int i = rptr[rr] + sub, e = rptr[rr + 1]; i < e; i += q) sum += prod[c][i];
+    for (int o = q >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (sub == 0) Y[(int64_t)(r0 + rr) * kCols + c] = sum;
   }
 }
 
@@ -142,12 +173,13 @@ __global__ __launch_bounds__(kWG) void batch_spmv_f64(const BTile* __restrict__ 
   const int t1 = wg_start[w + 1];
   for (int t = wg_start[w]; t < t1; ++t) {
     const BTile T = tiles[t];
-    const BMat d = mats[T.m];
+    const BMat* d = mats + T.m;
     if (T.kind == kKindCsr)
-      adaptive_rows<kCap, false>(d.rowptr, d.col, d.val, x + d.xoff, y + d.yoff, T.r0, T.p0, T.r1,
-                                 T.p1);
+      adaptive_rows<kCap, false>(d->rowptr, d->col, d->val, x + d->xoff, y + d->yoff, T.r0, T.p0,
+                                 T.r1, T.p1);
     else
-      bsr4_rows(d.brp, d.bci, d.bval, x + d.xoff, y + d.yoff, T.r0, T.r1, d.nrows, d.ncols);
+      bsr4_rows(d->brp, d->bci, d->bval, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
+                d->ncols);
     __syncthreads();
   }
 }
@@ -158,12 +190,14 @@ __global__ __launch_bounds__(kWG) void batch_spmm8_f16(const BTile* __restrict__
                                                        const uint4* __restrict__ X,
                                                        float* __restrict__ Y) {
   const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  int t = wg_start[w];
   const int t1 = wg_start[w + 1];
-  for (int t = wg_start[w]; t < t1; ++t) {
+  for (; t < t1; ++t) {
     const BTile T = tiles[t];
-    const BMat d = mats[T.m];
-    spmm8_rows(d.rowptr, d.col, d.hval, X + d.xoff, Y + d.yoff * kCols, T.r0, T.p0, T.r1, T.p1);
-    __syncthreads();
+    const BMat* m = mats + T.m;
+    spmm8_rows(m->rowptr, m->col, m->hval, X + m->xoff, Y + m->yoff * kCols, T.r0, T.p0, T.r1,
+               T.p1);
+    __syncthreads();  // LDS is reused by the next tile
   }
 }
 
@@ -424,7 +458,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
               "batch total rows/columns must fit int32");
   b->fmt.assign(M, BSP_BATCH_CSR);
   b->win_spmv.assign(M, kCap);
-  b->win_spmm.assign(M, kCap);
+  b->win_spmm.assign(M, kCapM);
   b->bsr.assign(M, nullptr);
   BSP_TRY(resident_grid(batch_spmv_f64, &b->spmv.resident));
   BSP_TRY(resident_grid(batch_spmm8_f16, &b->spmm.resident));
@@ -498,6 +532,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st != BSP_OK) break;
         if (t < best_v) best_v = t, b->win_spmv[m] = c, b->fmt[m] = BSP_BATCH_CSR;
+        if (c > kCapM) continue;
         st = time_candidate(one, true, b->spmm.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st != BSP_OK) break;
         if (t < best_m) best_m = t, b->win_spmm[m] = c;

@@ -420,10 +420,12 @@ bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_st
  * each operation as one launch over tiles of all matrices; the plain
  * schedule launches a thread-per-row kernel per matrix and operation. With
  * tune = 1 the builder times, per matrix, the CSR tile windows {128, 256,
- * 512, 1024} rows+entries (which set rows per workgroup and lanes per row)
- * and, where 4x4 blocks are at least 11/16 full, the 4x4 BSR format, each in
- * a full persistent launch, and keeps the fastest; tune = 0 uses CSR windows
- * of 1024; tune = 2 puts every eligible matrix in 4x4 BSR without timing.
+ * 512, 1024} rows+entries for the SpMV and {128, 256, 512} for the SpMM
+ * (which set rows per workgroup and lanes per row) and, where 4x4 blocks are
+ * at least 11/16 full, the 4x4 BSR format for the SpMV, each in a full
+ * persistent launch, and keeps the fastest; tune = 0 uses windows of 1024
+ * (SpMV) and 512 (SpMM); tune = 2 puts every eligible matrix in 4x4 BSR
+ * without timing.
  * A captured graph holds one step; replaying it needs no host work. */
 typedef struct bsp_batch_s* bsp_batch;
 typedef struct bsp_batch_options {
