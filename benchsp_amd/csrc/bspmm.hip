@@ -14,15 +14,15 @@
 //   B (Kx16): lane l holds B[E*(l/16) .. +E-1][l%16]
 //   D (16x16 FP32): lane l holds D[4*(l/16)+v][l%16] in acc[v]
 // Both operands are read k-contiguous: A from hv (row-major), B from a
-// transposed copy of the dense operand (B^T is N x K row-major), so every
-// lane issues one 8- or 16-byte load per operand.
+// per-call panel copy of the dense operand that stores every 16-row slab in
+// MFMA fragment order, so a wave's operand load is 512 contiguous bytes.
 //
 //   forward  C[16 rows of br, :] = sum_b scale_b * hv_b * B[16 rows of bc_b, :]
 //   dB       dB[16 rows of bc, :] = sum_b scale_b * hvt_b * dC[16 rows of br_b, :]
 //   dvals    dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T   (K = N)
-// One workgroup per block row (block column for dB); its four waves take
-// every fourth block and their FP32 partials are added in wave order through
-// LDS, so results are deterministic.
+// One workgroup per block row (block column for dB); each wave owns a set of
+// 16-column tiles and adds the row's blocks in index order, so results are
+// deterministic.
 #include <cmath>
 #include <memory>
 
@@ -36,7 +36,7 @@ struct bsp_bspmm_s {
   bsp::DevBuf<float> scale;
   bsp::DevBuf<uint16_t> hv, hvt;
   bsp::DevBuf<int32_t> tcolptr, tperm, tbrow;
-  bsp::DevBuf<uint16_t> work;  // transposed dense operand
+  bsp::DevBuf<uint16_t> work;  // dense operand in panel layout
   int64_t work_cap = 0;
 };
 
@@ -134,92 +134,124 @@ __global__ void gather_i32(const int32_t* __restrict__ src, const int32_t* __res
   if (k < n) dst[k] = src[perm[k]];
 }
 
-// dst[c][r] = src[r][c] for r < R, c < Cn; dst has Rp >= R columns, the
-// extra ones zero. 64x64 tiles through LDS.
-__global__ __launch_bounds__(kTB) void transpose_u16(const uint16_t* __restrict__ src, int64_t R,
-                                                     int64_t Cn, uint16_t* __restrict__ dst,
-                                                     int64_t Rp) {
-  __shared__ uint16_t t[64][65];
-  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int k = ty; k < 64; k += 4) {
-    const int64_t r = r0 + k, c = c0 + tx;
-    t[k][tx] = (r < R && c < Cn) ? src[r * Cn + c] : (uint16_t)0;
-  }
-  __syncthreads();
-  for (int k = ty; k < 64; k += 4) {
-    const int64_t c = c0 + k, r = r0 + tx;
-    if (c < Cn && r < Rp) dst[c * Rp + r] = t[tx][k];
-  }
-}
-
-// Adds the four waves' NT tiles in wave order (through LDS) and stores rows
-// [row0, row0+16) of a row-major FP32 matrix with N = 16*NT columns.
+// Panel layout of a dense R x N operand (row-major, N = 16*NT): for every
+// 16-row slab o and 16-column tile t the MFMA B-operand fragment in lane
+// order, P[((o*NT + t)*64 + l)*4 + e] = src[16o + 4(l/16) + e][16t + l%16]
+// (rows >= R read as zero), so a wave's fragment load is 512 contiguous bytes.
 template <int NT>
-__device__ __forceinline__ void reduce_store(f4 (&acc)[NT], float* red, float* __restrict__ out,
-                                             int64_t row0, int64_t rows, int64_t N) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  if (w > 0) {
+__global__ __launch_bounds__(kTB) void panelize(const uint16_t* __restrict__ src, int64_t R,
+                                                uint16_t* __restrict__ dst) {
+  constexpr int N = NT * 16;
+  __shared__ uint16_t t[16][N + 2];
+  const int64_t o = blockIdx.x;
+  for (int idx = threadIdx.x; idx < 16 * N / 8; idx += kTB) {
+    const int row = idx / (N / 8), c8 = (idx % (N / 8)) * 8;
+    const int64_t r = o * 16 + row;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (r < R) v = *reinterpret_cast<const uint4*>(src + r * N + c8);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) red[((w - 1) * NT * 4 + t * 4 + v) * kWave + lane] = acc[t][v];
+    for (int j = 0; j < 4; ++j) {
+      t[row][c8 + 2 * j] = (uint16_t)(w[j] & 0xffff);
+      t[row][c8 + 2 * j + 1] = (uint16_t)(w[j] >> 16);
+    }
   }
   __syncthreads();
-  if (w == 0) {
+  for (int idx = threadIdx.x; idx < NT * 32; idx += kTB) {
+    uint32_t w[4];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        float s = acc[t][v];
-        for (int q = 0; q < 3; ++q) s += red[(q * NT * 4 + t * 4 + v) * kWave + lane];
-        const int64_t row = row0 + 4 * (lane / 16) + v;
-        if (row < rows) out[row * N + t * 16 + (lane & 15)] = s;
-      }
+    for (int h = 0; h < 2; ++h) {
+      const int f = 2 * idx + h, tt = f / 64, l = f % 64, i = l % 16, kg = l / 16;
+      w[2 * h] = (uint32_t)t[4 * kg][tt * 16 + i] | ((uint32_t)t[4 * kg + 1][tt * 16 + i] << 16);
+      w[2 * h + 1] = (uint32_t)t[4 * kg + 2][tt * 16 + i] | ((uint32_t)t[4 * kg + 3][tt * 16 + i] << 16);
+    }
+    *reinterpret_cast<uint4*>(dst + (o * NT * 64 + 2 * idx) * 4) = make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
-// forward (TRANS = false) and dB (TRANS = true)
+// forward (TRANS = false) and dB (TRANS = true). One workgroup per block
+// row (block column for dB) with NW waves; wave w owns column tiles
+// [w*TPW, (w+1)*TPW) across all of the row's blocks, so its accumulators are
+// final (no cross-wave reduction) and blocks are added in index order. The
+// block loop runs eight blocks at a time so their operand loads overlap.
+template <int NT>
+struct RowsShape {
+  static constexpr int NW = NT < 8 ? NT : 8;  // waves
+  static constexpr int TPW = NT / NW;         // 16-column tiles per wave
+};
+
 template <int NT, bool BF, bool TRANS>
-__global__ __launch_bounds__(kTB) void bspmm_rows(const int32_t* __restrict__ ptr,
+__global__ __launch_bounds__(512) void bspmm_rows(const int32_t* __restrict__ ptr,
                                                   const int32_t* __restrict__ perm,
                                                   const int32_t* __restrict__ other,
                                                   const uint16_t* __restrict__ blocks,
                                                   const float* __restrict__ scale,
-                                                  const uint16_t* __restrict__ opT, int64_t ldT,
+                                                  const uint16_t* __restrict__ panel,
                                                   float* __restrict__ out, int64_t out_rows,
                                                   int64_t nouter) {
-  extern __shared__ float red[];
+  constexpr int TPW = RowsShape<NT>::TPW;
+  constexpr int U = 8;
   const int64_t r = blockIdx.x;
   if (r >= nouter) return;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const int i = lane & 15, kg = lane >> 4;
-  f4 acc[NT];
+  const uint16_t* pw = panel + ((int64_t)w * TPW * 64 + lane) * 4;
+  f4 acc[TPW];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k = ptr[r] + w; k < ptr[r + 1]; k += 4) {
+  for (int t = 0; t < TPW; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  const int k1 = ptr[r + 1];
+  int k = ptr[r];
+  for (; k + U <= k1; k += U) {
+    u2 a[U], bo[U][TPW];
+    float s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = TRANS ? perm[k + u] : k + u;
+      const int64_t o = other[k + u];
+      a[u] = *reinterpret_cast<const u2*>(blocks + (int64_t)b * 256 + i * 16 + 4 * kg);
+      s[u] = scale[b];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+        bo[u][t] = *reinterpret_cast<const u2*>(pw + (o * NT + t) * 256);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) acc[t] += s[u] * mma16<BF>(a[u], bo[u][t], f4{0.f, 0.f, 0.f, 0.f});
+  }
+  for (; k < k1; ++k) {
     const int b = TRANS ? perm[k] : k;
-    const int o = other[k];  // block column (forward) / block row (dB)
+    const int64_t o = other[k];
     const u2 a = *reinterpret_cast<const u2*>(blocks + (int64_t)b * 256 + i * 16 + 4 * kg);
     const float s = scale[b];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const u2 bo =
-          *reinterpret_cast<const u2*>(opT + (int64_t)(t * 16 + i) * ldT + (int64_t)o * 16 + 4 * kg);
-      const f4 p = mma16<BF>(a, bo, f4{0.f, 0.f, 0.f, 0.f});
-      acc[t] += s * p;
+    for (int t = 0; t < TPW; ++t) {
+      const u2 bo = *reinterpret_cast<const u2*>(pw + (o * NT + t) * 256);
+      acc[t] += s * mma16<BF>(a, bo, f4{0.f, 0.f, 0.f, 0.f});
     }
   }
-  reduce_store<NT>(acc, red, out, r * 16, out_rows, NT * 16);
+  constexpr int N = NT * 16;
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = r * 16 + 4 * kg + v;
+      if (row < out_rows) out[row * N + (w * TPW + t) * 16 + i] = acc[t][v];
+    }
 }
 
-// dvals: dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T (K = N)
+// dvals: dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T (K = N), one
+// workgroup of kDvWaves waves per block row, each wave every kDvWaves-th block.
+constexpr int kDvWaves = 4;
+
 template <int NT, bool BF>
-__global__ __launch_bounds__(kTB) void bspmm_dvals(const int32_t* __restrict__ browptr,
-                                                   const int32_t* __restrict__ bcol,
-                                                   const uint16_t* __restrict__ dC, int64_t M,
-                                                   const uint16_t* __restrict__ B, int64_t K,
-                                                   float* __restrict__ dvals, int64_t brows) {
+__global__ __launch_bounds__(kDvWaves * 64) void bspmm_dvals(const int32_t* __restrict__ browptr,
+                                                             const int32_t* __restrict__ bcol,
+                                                             const uint16_t* __restrict__ dC,
+                                                             int64_t M,
+                                                             const uint16_t* __restrict__ B,
+                                                             int64_t K, float* __restrict__ dvals,
+                                                             int64_t brows) {
   constexpr int N = NT * 16, S = N / 32;
   const int64_t br = blockIdx.x;
   if (br >= brows) return;
@@ -230,7 +262,7 @@ __global__ __launch_bounds__(kTB) void bspmm_dvals(const int32_t* __restrict__ b
 #pragma unroll
   for (int s = 0; s < S; ++s)
     a[s] = arow < M ? *reinterpret_cast<const u4*>(dC + arow * N + 32 * s + 8 * kg) : u4{0, 0, 0, 0};
-  for (int b = browptr[br] + w; b < browptr[br + 1]; b += 4) {
+  for (int b = browptr[br] + w; b < browptr[br + 1]; b += kDvWaves) {
     const int64_t brow = (int64_t)bcol[b] * 16 + i;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -245,8 +277,8 @@ __global__ __launch_bounds__(kTB) void bspmm_dvals(const int32_t* __restrict__ b
 }
 
 template <bool BF, bool TRANS>
-static bsp_status launch_rows(bsp_bspmm_s* h, int64_t N, const uint16_t* opT, int64_t ldT,
-                              float* out, int64_t out_rows, hipStream_t s) {
+static bsp_status launch_rows(bsp_bspmm_s* h, int64_t N, const uint16_t* panel, float* out,
+                              int64_t out_rows, hipStream_t s) {
   const int64_t nouter = TRANS ? h->bcols : h->brows;
   const int32_t* ptr = TRANS ? h->tcolptr.get() : h->browptr;
   const int32_t* perm = TRANS ? h->tperm.get() : nullptr;
@@ -255,8 +287,8 @@ static bsp_status launch_rows(bsp_bspmm_s* h, int64_t N, const uint16_t* opT, in
   if (nouter == 0) return BSP_OK;
 #define BSPMM_CASE(NT)                                                                           \
   case NT:                                                                                       \
-    bspmm_rows<NT, BF, TRANS><<<nouter, kTB, 3 * NT * 4 * kWave * sizeof(float), s>>>(          \
-        ptr, perm, other, blocks, h->scale.get(), opT, ldT, out, out_rows, nouter);              \
+    bspmm_rows<NT, BF, TRANS><<<nouter, RowsShape<NT>::NW * kWave, 0, s>>>(                      \
+        ptr, perm, other, blocks, h->scale.get(), panel, out, out_rows, nouter);                 \
     break;
   switch (N / 16) {
     BSPMM_CASE(2)
@@ -277,8 +309,8 @@ static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, co
   if (h->brows == 0) return BSP_OK;
 #define DV_CASE(NT)                                                                             \
   case NT:                                                                                      \
-    bspmm_dvals<NT, BF><<<h->brows, kTB, 0, s>>>(h->browptr, h->bcol, dC, h->M, B, h->K, dvals, \
-                                                 h->brows);                                     \
+    bspmm_dvals<NT, BF><<<h->brows, kDvWaves * 64, 0, s>>>(h->browptr, h->bcol, dC, h->M, B,  \
+                                                           h->K, dvals, h->brows);              \
     break;
   switch (N / 16) {
     DV_CASE(2)
@@ -293,15 +325,22 @@ static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, co
   return BSP_OK;
 }
 
-static bsp_status transposed(bsp_bspmm_s* h, const uint16_t* src, int64_t R, int64_t Cn,
-                             int64_t Rp, hipStream_t s) {
-  if (Cn * Rp > h->work_cap) {
+// The dense operand (R x N) in panel layout, nslab 16-row slabs, in h->work.
+static bsp_status paneled(bsp_bspmm_s* h, const uint16_t* src, int64_t R, int64_t nslab,
+                          int64_t N, hipStream_t s) {
+  if (nslab * 16 * N > h->work_cap) {
     BSP_HIP(hipStreamSynchronize(s));
-    BSP_TRY(h->work.alloc(Cn * Rp));
-    h->work_cap = Cn * Rp;
+    BSP_TRY(h->work.alloc(nslab * 16 * N));
+    h->work_cap = nslab * 16 * N;
   }
-  dim3 grid((unsigned)cdiv(Cn, 64), (unsigned)cdiv(Rp, 64));
-  transpose_u16<<<grid, kTB, 0, s>>>(src, R, Cn, h->work.get(), Rp);
+  if (nslab == 0) return BSP_OK;
+  switch (N / 16) {
+    case 2: panelize<2><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
+    case 4: panelize<4><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
+    case 8: panelize<8><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
+    case 16: panelize<16><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
+    default: return fail(BSP_ERR_UNSUPPORTED, "block SpMM needs N in {32, 64, 128, 256}");
+  }
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }
@@ -384,12 +423,11 @@ bsp_status bsp_bspmm_view_get(bsp_bspmm h, const float** scale, const uint16_t**
 bsp_status bsp_bspmm_fwd(bsp_bspmm h, const void* B, int64_t N, float* Cout, bsp_stream s) {
   BSP_REQUIRE(h && B && Cout, "null argument");
   hipStream_t st = (hipStream_t)s;
-  const int64_t Kp = h->bcols * 16;
-  // B^T (N x Kp), k-contiguous for the MFMA B operand
-  BSP_TRY(transposed(h, static_cast<const uint16_t*>(B), h->K, N, Kp, st));
-  if (h->ht == BSP_BF16)
-    return launch_rows<true, false>(h, N, h->work.get(), Kp, Cout, h->M, st);
-  return launch_rows<false, false>(h, N, h->work.get(), Kp, Cout, h->M, st);
+  BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256, "block SpMM needs N in {32, 64, 128, 256}");
+  // B in panel layout: each block column's 16 x N slab in MFMA operand order
+  BSP_TRY(paneled(h, static_cast<const uint16_t*>(B), h->K, h->bcols, N, st));
+  if (h->ht == BSP_BF16) return launch_rows<true, false>(h, N, h->work.get(), Cout, h->M, st);
+  return launch_rows<false, false>(h, N, h->work.get(), Cout, h->M, st);
 }
 
 bsp_status bsp_bspmm_bwd(bsp_bspmm h, const void* B, const void* dC, int64_t N, float* dB,
@@ -404,12 +442,13 @@ bsp_status bsp_bspmm_bwd(bsp_bspmm h, const void* B, const void* dC, int64_t N, 
       BSP_TRY(launch_dvals<false>(h, N, (const uint16_t*)dC, (const uint16_t*)B, dvals, st));
   }
   if (dB) {
-    const int64_t Mp = h->brows * 16;
-    BSP_TRY(transposed(h, static_cast<const uint16_t*>(dC), h->M, N, Mp, st));
+    BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256,
+                "block SpMM needs N in {32, 64, 128, 256}");
+    BSP_TRY(paneled(h, static_cast<const uint16_t*>(dC), h->M, h->brows, N, st));
     if (bf)
-      BSP_TRY((launch_rows<true, true>(h, N, h->work.get(), Mp, dB, h->K, st)));
+      BSP_TRY((launch_rows<true, true>(h, N, h->work.get(), dB, h->K, st)));
     else
-      BSP_TRY((launch_rows<false, true>(h, N, h->work.get(), Mp, dB, h->K, st)));
+      BSP_TRY((launch_rows<false, true>(h, N, h->work.get(), dB, h->K, st)));
   }
   return BSP_OK;
 }
