@@ -55,6 +55,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--warm", action="store_true", help="single copy (cache-warm) timing")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB three times (counter calibration run)")
+    ap.add_argument("--traffic-summary", nargs=4, metavar="CSV",
+                    help="FETCH_SIZE, WRITE_SIZE, copy FETCH_SIZE, copy WRITE_SIZE counter CSVs "
+                         "(rocprofv3 --pmc) -> profiles/csr_adaptive_traffic.json")
     return ap.parse_args()
 
 
@@ -282,10 +287,64 @@ def batch_leg(args, dev):
     return out
 
 
+def copy_probe():
+    """A plain device copy of 256 MiB, three times: the calibration that turns
+    FETCH_SIZE / WRITE_SIZE readings into bytes."""
+    import torch
+    a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied 256 MiB x3", flush=True)
+
+
+SPMV_KERNELS = ("csr_adaptive", "sum_partials")
+
+
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
+    """Median FETCH_SIZE / WRITE_SIZE per launch of the SpMV-leg kernels (KiB
+    per dispatch), scaled by what the same counters read for the 256 MiB copy,
+    written to profiles/csr_adaptive_traffic.json."""
+    import csv
+    import statistics
+
+    def per_kernel(path):
+        rows = list(csv.DictReader(open(path)))
+        out = {}
+        for r in rows:
+            k = (r["Kernel_Name"], int(r["Grid_Size"]))
+            out.setdefault(k, []).append(float(r["Counter_Value"]))
+        return {k: statistics.median(v) for k, v in out.items()}
+
+    def copy_kib(path):
+        vals = [v for (name, _), v in per_kernel(path).items() if "copyBuffer" in name]
+        return max(vals)
+
+    copy_kib_total = 256 * 1024
+    rs = copy_kib_total / copy_kib(copy_fetch_csv)
+    ws = copy_kib_total / copy_kib(copy_write_csv)
+    fe, wr = per_kernel(fetch_csv), per_kernel(write_csv)
+    launches = {}
+    for (name, grid), f in sorted(fe.items(), key=lambda kv: -kv[1]):
+        if not any(k in name for k in SPMV_KERNELS):
+            continue
+        w = wr.get((name, grid), 0.0)
+        launches[f"{name.split('(')[0]} grid={grid}"] = {
+            "FETCH_SIZE_KiB": round(f, 1), "WRITE_SIZE_KiB": round(w, 1),
+            "hbm_bytes": int(round((rs * f + ws * w) * 1024))}
+    d = {"read_scale": round(rs, 4), "write_scale": round(ws, 4), "launches": launches}
+    out = os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
+    with open(out, "w") as fh:
+        json.dump(d, fh, indent=1)
+    print(json.dumps(d, indent=1))
+
+
 def measured_traffic():
-    """HBM bytes the adaptive kernel moves per step (27-pt launch + both
-    power-law segment launches), from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE measurement scaled by a plain copy (profiles/)."""
+    """HBM bytes per SpMV step (every launch of the step: the 27-pt adaptive
+    launch, the power-law segment launch and its partial-sum kernel), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement scaled by a plain
+    copy (profiles/csr_adaptive_traffic.json)."""
     p = os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
     if not os.path.exists(p):
         return None
@@ -300,6 +359,12 @@ def _progress(msg: str):
 
 def main():
     args = parse()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary)
+        return
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library "
               "runs one matrix shard per GPU and has no multi-GPU path.", file=sys.stderr)
