@@ -340,17 +340,39 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
     print(json.dumps(d, indent=1))
 
 
-def measured_traffic():
-    """HBM bytes per SpMV step (every launch of the step: the 27-pt adaptive
-    launch, the power-law segment launch and its partial-sum kernel), from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement scaled by a plain
-    copy (profiles/csr_adaptive_traffic.json)."""
+def measured_traffic(segmented: bool):
+    """HBM bytes of one SpMV launch of the adaptive plan, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement scaled by a plain copy
+    (profiles/csr_adaptive_traffic.json): the XCD-pinned segment kernel plus
+    its partial-sum kernel for a column-segmented plan, the single adaptive
+    kernel otherwise."""
     p = os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         d = json.load(f)
-    return int(sum(v["hbm_bytes"] for v in d["launches"].values()))
+    keys = ("csr_adaptive_xcd", "sum_partials") if segmented else ("csr_adaptive_f64<",)
+    return int(sum(v["hbm_bytes"] for k, v in d["launches"].items() if any(q in k for q in keys)))
+
+
+def roofline(per: dict, method: str) -> dict:
+    """The dominant launch of the step (the matrix whose SpMV takes longest):
+    its algorithmic bytes over its event-timed duration, and its measured HBM
+    bytes; the other matrix's launch is reported alongside."""
+    def one(name, m):
+        segmented = m["column_segments"] > 1
+        gbs = m["bytes"] / (m["us"] * 1e-6) / 1e9
+        r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(gbs / HBM_PEAK_GBS, 4),
+             "traffic": measured_traffic(segmented) if method == "adaptive" else None,
+             "kernel": ("csr_adaptive_xcd_f64<2048> + sum_partials" if segmented else
+                        "csr_adaptive_f64<1024>") if method == "adaptive" else method,
+             "launch": f"{name} SpMV: {m['bytes']} algorithmic bytes in {m['us']} us"}
+        return r
+    ordered = sorted(per.items(), key=lambda kv: -kv[1]["us"])
+    out = one(*ordered[0])
+    out["other_launches"] = {name: one(name, m) for name, m in ordered[1:]}
+    return out
 
 
 def _progress(msg: str):
@@ -393,13 +415,7 @@ def main():
                    "method": args.method, "cache": "warm" if args.warm else "cold (rotated copies)",
                    "pct_hbm_peak": round(100 * sp["value"] / HBM_PEAK_GBS, 1),
                    "matrices": sp["per"]},
-        "roofline": {"bound": "hbm", "achieved": round(sp["achieved"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(sp["achieved"] / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic() if args.method == "adaptive" else None,
-                     "traffic_basis": "HBM bytes per step (all launches of the kernel), "
-                                      "algorithmic bytes per step = "
-                                      f"{sum(v['bytes'] for v in sp['per'].values())}",
-                     "kernel": "csr_adaptive_f64" if args.method == "adaptive" else args.method},
+        "roofline": roofline(sp["per"], args.method),
     }
     if not args.no_spmm:
         out["spmm_fwd_bwd"] = spmm_leg(args, dev)
