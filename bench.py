@@ -19,6 +19,10 @@ dvals = dC*B^T on A's blocks) = 3 * 2*nnz*N FLOP, on FP16 (and BF16) MFMA
 with FP32 accumulation, against the same three products with the library's
 CSR SpMM in FP32.
 
+Formats leg: the same two matrices through every SpMV format and method of
+the library (scalar / balanced / split / adaptive CSR, 4x4 BSR on MFMA and
+VALU, DBSR, DASP) and rocSPARSE, as CSR-equivalent effective GB/s.
+
 Batched leg: 512 matrices of uneven size, an FP64 SpMV and an 8-column FP16
 SpMM each per step, plain per-matrix launches against the persistent
 autotuned schedule captured in a graph.
@@ -52,6 +56,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-formats", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--warm", action="store_true", help="single copy (cache-warm) timing")
@@ -205,6 +210,55 @@ def spmm_leg(args, dev):
     out["csr_fp32_GFLOPs"] = round(3 * flop / (ms * 1e-3) / 1e9, 1)
     out["csr_fp32_ms_per_step"] = round(ms, 4)
     out["fp16_over_fp32"] = round(out["block_float16_GFLOPs"] / out["csr_fp32_GFLOPs"], 2)
+    return out
+
+
+def formats_leg(args, dev, mats, xs):
+    """The reference's headline comparison on the same two matrices: every
+    SpMV format and method of the library plus rocSPARSE, each on rotated
+    copies (cold HBM), reported as CSR-equivalent effective GB/s of the
+    median event-timed launch."""
+    import numpy as np
+    import torch
+    from benchsp_amd.device import (Bsr, CsrSpmv, Dasp, Dbsr, DeviceCsr, RocsparseSpmv)
+
+    out = {}
+    for (name, A), x in zip(mats, xs):
+        xd = torch.from_numpy(x).to(dev)
+        y = torch.empty(A.nrows, dtype=torch.float64, device=dev)
+        ncopy = _copies(A.csr_bytes(), args.warm)
+        dAs = [DeviceCsr.from_host(A, dev) for _ in range(ncopy)]
+        cands = {
+            "csr_adaptive": lambda d: CsrSpmv(d, "adaptive"),
+            "csr_scalar": lambda d: CsrSpmv(d, "scalar"),
+            "csr_balanced": lambda d: CsrSpmv(d, "balanced"),
+            "csr_split": lambda d: CsrSpmv(d, "split"),
+            "dasp": lambda d: Dasp(d),
+            "dbsr": lambda d: Dbsr(d),
+            "rocsparse": lambda d: RocsparseSpmv(d, 0),
+        }
+        if A.nnz / max(1, A.nrows) >= 16:  # 4x4 blocks only where rows are dense
+            cands["bsr4_mfma"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=2)))(Bsr(d))
+            cands["bsr4_valu"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=1)))(Bsr(d))
+        res = {}
+        for key, make in cands.items():
+            ops = [make(d) for d in dAs]
+            torch.cuda.synchronize()
+            for i in range(2 * ncopy):
+                ops[i % ncopy](xd, y)
+            ts = []
+            for i in range(12):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                ops[i % ncopy](xd, y)
+                e1.record()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            us = float(np.median(ts)) * 1e3
+            res[key] = {"us": round(us, 2), "eff_GBs": round(A.csr_bytes() / (us * 1e-6) / 1e9, 1)}
+            del ops
+        out[name] = res
+        del dAs
     return out
 
 
@@ -366,7 +420,7 @@ def roofline(per: dict, method: str) -> dict:
              "frac": round(gbs / HBM_PEAK_GBS, 4),
              "traffic": measured_traffic(segmented) if method == "adaptive" else None,
              "kernel": ("csr_adaptive_xcd_f64<2048> + sum_partials" if segmented else
-                        "csr_adaptive_f64<1024>") if method == "adaptive" else method,
+                        "csr_adaptive_f64") if method == "adaptive" else method,
              "launch": f"{name} SpMV: {m['bytes']} algorithmic bytes in {m['us']} us"}
         return r
     ordered = sorted(per.items(), key=lambda kv: -kv[1]["us"])
@@ -417,6 +471,9 @@ def main():
                    "matrices": sp["per"]},
         "roofline": roofline(sp["per"], args.method),
     }
+    if not args.no_formats:
+        out["formats"] = formats_leg(args, dev, list(zip(sp["per"].keys(), sp["mats"])), sp["xs"])
+        _progress("formats leg done")
     if not args.no_spmm:
         out["spmm_fwd_bwd"] = spmm_leg(args, dev)
         _progress("block-spmm leg done")

@@ -91,9 +91,12 @@ static void launch_adaptive_t(const CsrSegment& g, bool accum, const double* x, 
     csr_adaptive_f64<CHUNK, false><<<g.nblk, kWG, 0, s>>>(g.rowptr, g.col, g.val, x, y, blk, g.nblk);
 }
 
+// The window (chunk) is a plan-time value up to 2048; the kernel's LDS
+// capacity is the next of 1024 / 2048 (a window of c units gives blocks of at
+// most c rows and 1.5c entries, or a single row).
 static void launch_adaptive(const CsrSegment& g, int chunk, bool accum, const double* x, double* y,
                             hipStream_t s) {
-  if (chunk == 2048)
+  if (chunk > 1024)
     launch_adaptive_t<2048>(g, accum, x, y, s);
   else
     launch_adaptive_t<1024>(g, accum, x, y, s);
@@ -449,8 +452,26 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     }
     BSP_HIP(hipStreamSynchronize(s));
   }
-  // measured on gfx950: 1024-unit blocks for one segment, 2048 for segmented
-  p->chunk = p->opt.chunk == 2048 || p->opt.chunk == 1024 ? p->opt.chunk : (nseg > 1 ? 2048 : 1024);
+  // Default window: the largest of 2048, 1024, 512, 256 units that still
+  // gives at least 8 blocks per CU (larger blocks amortise the per-block
+  // start-up; measured on the 27-point 64^3 stencil: 2048 -> 21.2 us,
+  // 1024 -> 22.9 us, 512 -> 29.8 us); segmented plans always use 2048.
+  if (p->opt.chunk >= 128 && p->opt.chunk <= 2048) {
+    p->chunk = p->opt.chunk;
+  } else if (nseg > 1) {
+    p->chunk = 2048;
+  } else {
+    int dev = 0, cus = 0;
+    BSP_HIP(hipGetDevice(&dev));
+    BSP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t units = n + A.nnz, want = 8 * (int64_t)std::max(cus, 1);
+    p->chunk = 256;
+    for (int c : {2048, 1024, 512})
+      if (units / c >= want) {
+        p->chunk = c;
+        break;
+      }
+  }
   p->nblk = 0;
   for (auto& g : p->seg) {
     BSP_TRY(adaptive_partition(g, n, p->chunk, s));
@@ -538,7 +559,7 @@ bsp_status csr_plan_run(const CsrPlan* p, const double* x, double* y, hipStream_
       if (p->xcd_mode) {
         const int nseg = (int)p->seg.size();
         const auto* tab = reinterpret_cast<const SegDesc*>(p->segtab.get());
-        if (p->chunk == 2048)
+        if (p->chunk > 1024)
           csr_adaptive_xcd_f64<2048><<<p->xcd_grid, kWG, 0, s>>>(tab, nseg, x);
         else
           csr_adaptive_xcd_f64<1024><<<p->xcd_grid, kWG, 0, s>>>(tab, nseg, x);

@@ -361,3 +361,29 @@ class Batch:
                 self._h = None
         except Exception:
             pass
+
+
+class RocsparseSpmv:
+    """rocSPARSE SpMV on a device CSR (bsp_rocsparse_*), the vendor comparator."""
+
+    def __init__(self, A: DeviceCsr, alg: int = 0):
+        self.A = A
+        self._desc = A.desc()
+        h = C.c_void_p()
+        lib().bsp_rocsparse_plan_create(C.byref(self._desc), alg, stream_handle(), C.byref(h))
+        self._h = h
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        if y is None:
+            y = torch.empty(self.A.nrows, dtype=torch.float32 if self.A.val.dtype != torch.float64
+                            else torch.float64, device=x.device)
+        lib().bsp_rocsparse_spmv(self._h, ptr(x), ptr(y), stream_handle())
+        return y
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_rocsparse_plan_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -88,7 +88,7 @@ typedef struct bsp_spmv_plan_s* bsp_spmv_plan;
  *   diagonal and rows are sorted), < 0 = never segment. */
 typedef struct bsp_csr_options {
   int64_t seg_cols;
-  /* adaptive block window in rows+entries: 1024 or 2048 (0 = 1024 for an
+  /* adaptive block window in rows+entries, 128..2048 (others: 1024 for an
    * unsegmented plan, 2048 for a segmented one) */
   int32_t chunk;
   /* segmented plans: 0 = automatic (2, 4 or 8 segments run in one launch,
