@@ -23,6 +23,9 @@ Formats leg: the same two matrices through every SpMV format and method of
 the library (scalar / balanced / split / adaptive CSR, 4x4 BSR on MFMA and
 VALU, DBSR, DASP) and rocSPARSE, as CSR-equivalent effective GB/s.
 
+GNN leg: the differentiable CSR SpMM (forward, transpose-free A^T dY,
+SDDMM) on a 200,000-node power-law graph with 64 features, FP32 and BF16.
+
 Batched leg: 512 matrices of uneven size, an FP64 SpMV and an 8-column FP16
 SpMM each per step, plain per-matrix launches against the persistent
 autotuned schedule captured in a graph.
@@ -57,6 +60,7 @@ def parse():
     ap.add_argument("--no-spmm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-formats", action="store_true")
+    ap.add_argument("--no-gnn", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--warm", action="store_true", help="single copy (cache-warm) timing")
@@ -210,6 +214,51 @@ def spmm_leg(args, dev):
     out["csr_fp32_GFLOPs"] = round(3 * flop / (ms * 1e-3) / 1e9, 1)
     out["csr_fp32_ms_per_step"] = round(ms, 4)
     out["fp16_over_fp32"] = round(out["block_float16_GFLOPs"] / out["csr_fp32_GFLOPs"], 2)
+    return out
+
+
+def gnn_leg(args, dev):
+    """The differentiable SpMM of a GNN layer: A = power-law graph (n=200,000,
+    Zipf 2.1, average degree ~50, symmetric-normalised), X and dY 200,000 x 64
+    N(0,1). One step = forward Y = A X, backward dX = A^T dY and dvals =
+    SDDMM(dY, X): 3 * 2*nnz*64 FLOP, in FP32 and in BF16 with FP32
+    accumulation. Bytes per product (FP32): 8*nnz + 4*(n+1) + 2*4*n*64."""
+    import numpy as np
+    import torch
+    from benchsp_amd import host
+    from benchsp_amd.autograd import SparsePattern
+
+    A = host.zipf_graph_200k()
+    F = 64
+    X32 = torch.from_numpy(host.normal(A.ncols * F, 400).astype(np.float32).reshape(A.ncols, F)).to(dev)
+    dY32 = torch.from_numpy(host.normal(A.nrows * F, 401).astype(np.float32).reshape(A.nrows, F)).to(dev)
+    v32 = torch.from_numpy(A.val.astype(np.float32)).to(dev)
+    pat = SparsePattern.from_host(A, dev)
+    flop = 3 * 2.0 * A.nnz * F
+    out = {"workload": f"GNN layer: n={A.nrows}, nnz={A.nnz}, F={F}; step = Y=A X, dX=A^T dY, "
+                       "dvals=SDDMM(dY, X)", "flop_per_step": flop}
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        X, dY, v = X32.to(dt), dY32.to(dt), v32.to(dt)
+
+        def step():
+            pat.spmm(v, X)
+            pat.spmm_t(v, dY)
+            pat.sddmm(dY, X)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        esz = X.element_size()
+        nbytes = 3 * ((4 + esz) * A.nnz + 4 * (A.nrows + 1) + (esz + 4) * A.nrows * F)
+        out[f"{name}_ms_per_step"] = round(ms, 4)
+        out[f"{name}_GFLOPs"] = round(flop / (ms * 1e-3) / 1e9, 1)
+        out[f"{name}_eff_GBs"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
     return out
 
 
@@ -477,6 +526,9 @@ def main():
     if not args.no_spmm:
         out["spmm_fwd_bwd"] = spmm_leg(args, dev)
         _progress("block-spmm leg done")
+    if not args.no_gnn:
+        out["gnn_spmm"] = gnn_leg(args, dev)
+        _progress("gnn leg done")
     if not args.no_batch:
         out["batched"] = batch_leg(args, dev)
         _progress("batched leg done")
