@@ -112,6 +112,23 @@ __global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ id
     }
     const int n = min(kWave, it.z - base);
     int t = 0;
+    // eight rows of X in flight per wave, then four, then one
+    for (; t + 8 <= n; t += 8) {
+      float xr[8][VEC];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        load_feats<TX, VEC>(X + (int64_t)rdlane_i(c, t + u) * F + lane * VEC, xr[u]);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          s0 += rdlane_f(a, t + u) * xr[u][v];
+          s1 += rdlane_f(a, t + 4 + u) * xr[4 + u][v];
+        }
+        acc[v] += s0 + s1;
+      }
+    }
     for (; t + 4 <= n; t += 4) {
       float x0[VEC], x1[VEC], x2[VEC], x3[VEC];
       const int c0 = rdlane_i(c, t), c1 = rdlane_i(c, t + 1), c2 = rdlane_i(c, t + 2),
@@ -147,9 +164,19 @@ __global__ __launch_bounds__(kTB) void spmm_combine(const int4* __restrict__ com
   if (w >= ncomb) return;
   const int4 c = comb[w];
   for (int64_t f = lane; f < F; f += kWave) {
-    float s = 0.f;
-    for (int k = 0; k < c.z; ++k) s += partial[(int64_t)(c.y + k) * F + f];
-    Y[(int64_t)c.x * F + f] = s;
+    // four interleaved running sums (chunk k goes to sum k % 4), added in a
+    // fixed order at the end: deterministic, four loads in flight
+    const float* p = partial + (int64_t)c.y * F + f;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int k = 0;
+    for (; k + 4 <= c.z; k += 4) {
+      s0 += p[(int64_t)k * F];
+      s1 += p[(int64_t)(k + 1) * F];
+      s2 += p[(int64_t)(k + 2) * F];
+      s3 += p[(int64_t)(k + 3) * F];
+    }
+    for (; k < c.z; ++k) s0 += p[(int64_t)k * F];
+    Y[(int64_t)c.x * F + f] = (s0 + s1) + (s2 + s3);
   }
 }
 
@@ -170,16 +197,28 @@ __global__ __launch_bounds__(kTB) void sddmm_items(const int32_t* __restrict__ c
   const int q = lane % G, sub = lane / G;
   float dy[4];
   load_feats<TX, 4>(dY + (int64_t)it.x * F + q * 4, dy);
-  for (int j0 = it.y; j0 < it.z; j0 += NZ) {
-    const int j = j0 + sub;
-    const bool ok = j < it.z;
-    const int c = ok ? col[j] : 0;
-    float xv[4];
-    load_feats<TX, 4>(X + (int64_t)c * F + q * 4, xv);
-    float p = dy[0] * xv[0] + dy[1] * xv[1] + dy[2] * xv[2] + dy[3] * xv[3];
+  // 64 column indices per wave load, then U steps of NZ entries with their X
+  // rows all in flight before the dot products
+  constexpr int U = 4;
+  for (int base = it.y; base < it.z; base += kWave) {
+    const int cl = base + lane < it.z ? ld_stream(&col[base + lane]) : 0;
+    const int n = min(kWave, it.z - base);
+    for (int t0 = 0; t0 < n; t0 += NZ * U) {
+      float xv[U][4];
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
-    if (q == 0 && ok) dvals[j] = p;
+      for (int u = 0; u < U; ++u) {
+        const int c = __shfl(cl, min(t0 + u * NZ + sub, kWave - 1), 64);
+        load_feats<TX, 4>(X + (int64_t)c * F + q * 4, xv[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float p = dy[0] * xv[u][0] + dy[1] * xv[u][1] + dy[2] * xv[u][2] + dy[3] * xv[u][3];
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+        const int jl = t0 + u * NZ + sub;
+        if (q == 0 && jl < n) dvals[base + jl] = p;
+      }
+    }
   }
 }
 
