@@ -214,6 +214,35 @@ def spmm_leg(args, dev):
     out["csr_fp32_GFLOPs"] = round(3 * flop / (ms * 1e-3) / 1e9, 1)
     out["csr_fp32_ms_per_step"] = round(ms, 4)
     out["fp16_over_fp32"] = round(out["block_float16_GFLOPs"] / out["csr_fp32_GFLOPs"], 2)
+    out["rocsparse"] = rocsparse_spmm_points(args, dev, A, X, dY, flop)
+    return out
+
+
+def rocsparse_spmm_points(args, dev, A, X, dY, flop):
+    """rocSPARSE CSR SpMM on the same matrix and dense operands: forward
+    C = A X and transposed A^T dY, FP32 and FP16-in/FP32-out (ms per product
+    and GFLOP/s), the vendor points beside the library's kernels."""
+    import torch
+    from benchsp_amd.device import DeviceCsr, RocsparseSpmm
+    out = {}
+    N = X.shape[1]
+    for name, dt in (("fp32", torch.float32), ("fp16", torch.float16)):
+        d = DeviceCsr.from_host(A, dev, dtype=dt)
+        Xd, dYd = X.to(dt).contiguous(), dY.to(dt).contiguous()
+        for kind, op, operand in (("fwd", RocsparseSpmm(d, N, False), Xd),
+                                  ("transposed", RocsparseSpmm(d, N, True), dYd)):
+            for _ in range(args.warmup):
+                op(operand)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                op(operand)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            out[f"{name}_{kind}_ms"] = round(ms, 4)
+            out[f"{name}_{kind}_GFLOPs"] = round(flop / (ms * 1e-3) / 1e9, 1)
     return out
 
 
@@ -259,6 +288,7 @@ def gnn_leg(args, dev):
         out[f"{name}_ms_per_step"] = round(ms, 4)
         out[f"{name}_GFLOPs"] = round(flop / (ms * 1e-3) / 1e9, 1)
         out[f"{name}_eff_GBs"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+    out["rocsparse"] = rocsparse_spmm_points(args, dev, A, X32, dY32, 2.0 * A.nnz * F)
     return out
 
 

@@ -91,6 +91,10 @@ VOID = "void"  # restype marker for the reference's void functions
 # name -> (restype, argtypes); restype None means bsp_status (checked).
 _SIGS: dict[str, tuple] = {
     "bsp_csr_spmv_f16": (None, [vp, vp, vp, vp]),
+    "bsp_rocsparse_spmm_plan_create": (None, [C.POINTER(CsrDesc), i64, C.c_int, C.c_int, vp,
+                                              C.POINTER(vp)]),
+    "bsp_rocsparse_spmm": (None, [vp, vp, vp, vp]),
+    "bsp_rocsparse_spmm_plan_destroy": (None, [vp]),
     "bsp_mtx_read": (None, [C.c_char_p, C.POINTER(vp)]),
     "bsp_coo_to_csr": (None, [i64, i64, i64, vp, vp, vp, C.POINTER(vp)]),
     "bsp_block_info": (None, [C.POINTER(CsrDesc), C.c_int, vp, C.POINTER(BlockInfo)]),

@@ -245,6 +245,76 @@ bsp_status bsp_rocsparse_plan_destroy(bsp_rocsparse_plan p) {
   return BSP_OK;
 }
 
+}  // extern "C"
+
+struct bsp_rocsparse_spmm_plan_s {
+  rocsparse_handle handle = nullptr;
+  rocsparse_spmat_descr mat = nullptr;
+  rocsparse_dnmat_descr B = nullptr, C = nullptr;
+  rocsparse_spmm_alg alg = rocsparse_spmm_alg_default;
+  rocsparse_operation op = rocsparse_operation_none;
+  DevBuf<char> buffer;
+  size_t bytes = 0;
+  ~bsp_rocsparse_spmm_plan_s() {
+    if (B) rocsparse_destroy_dnmat_descr(B);
+    if (C) rocsparse_destroy_dnmat_descr(C);
+    if (mat) rocsparse_destroy_spmat_descr(mat);
+    if (handle) rocsparse_destroy_handle(handle);
+  }
+  rocsparse_status call(rocsparse_spmm_stage stage, void* buf) {
+    const float one = 1.f, zero = 0.f;
+    return rocsparse_spmm(handle, op, rocsparse_operation_none, &one, mat, B, &zero, C,
+                          rocsparse_datatype_f32_r, alg, stage, &bytes, buf);
+  }
+};
+
+extern "C" {
+
+bsp_status bsp_rocsparse_spmm_plan_create(const bsp_csr* A, int64_t N, int transpose, int alg,
+                                          bsp_stream s, bsp_rocsparse_spmm_plan* out) {
+  BSP_REQUIRE(A && out && N > 0, "bad argument");
+  BSP_REQUIRE(A->dtype == BSP_F32 || A->dtype == BSP_F16 || A->dtype == BSP_BF16,
+              "rocSPARSE SpMM comparator takes FP32, FP16 or BF16 values");
+  auto p = std::make_unique<bsp_rocsparse_spmm_plan_s>();
+  p->alg = (rocsparse_spmm_alg)alg;
+  p->op = transpose ? rocsparse_operation_transpose : rocsparse_operation_none;
+  const rocsparse_datatype ta = A->dtype == BSP_F32   ? rocsparse_datatype_f32_r
+                                : A->dtype == BSP_F16 ? rocsparse_datatype_f16_r
+                                                      : rocsparse_datatype_bf16_r;
+  const int64_t brows = transpose ? A->nrows : A->ncols, crows = transpose ? A->ncols : A->nrows;
+  BSP_RS(rocsparse_create_handle(&p->handle));
+  BSP_RS(rocsparse_set_stream(p->handle, (hipStream_t)s));
+  BSP_RS(rocsparse_create_csr_descr(&p->mat, A->nrows, A->ncols, A->nnz, (void*)A->rowptr,
+                                    (void*)A->colidx, (void*)A->val, rocsparse_indextype_i32,
+                                    rocsparse_indextype_i32, rocsparse_index_base_zero, ta));
+  DevBuf<char> bd, cd;  // placeholders until the first run sets B and C
+  BSP_TRY(bd.alloc(brows * N * 4));
+  BSP_TRY(cd.alloc(crows * N * 4));
+  BSP_RS(rocsparse_create_dnmat_descr(&p->B, brows, N, N, bd.get(), ta, rocsparse_order_row));
+  BSP_RS(rocsparse_create_dnmat_descr(&p->C, crows, N, N, cd.get(), rocsparse_datatype_f32_r,
+                                      rocsparse_order_row));
+  BSP_RS(p->call(rocsparse_spmm_stage_buffer_size, nullptr));
+  BSP_TRY(p->buffer.alloc(p->bytes));
+  BSP_RS(p->call(rocsparse_spmm_stage_preprocess, p->buffer.get()));
+  BSP_HIP(hipStreamSynchronize((hipStream_t)s));
+  *out = p.release();
+  return BSP_OK;
+}
+
+bsp_status bsp_rocsparse_spmm(bsp_rocsparse_spmm_plan p, const void* B, float* C, bsp_stream s) {
+  BSP_REQUIRE(p && B && C, "null argument");
+  BSP_RS(rocsparse_set_stream(p->handle, (hipStream_t)s));
+  BSP_RS(rocsparse_dnmat_set_values(p->B, (void*)B));
+  BSP_RS(rocsparse_dnmat_set_values(p->C, C));
+  BSP_RS(p->call(rocsparse_spmm_stage_compute, p->buffer.get()));
+  return BSP_OK;
+}
+
+bsp_status bsp_rocsparse_spmm_plan_destroy(bsp_rocsparse_spmm_plan p) {
+  delete p;
+  return BSP_OK;
+}
+
 int bsp_cusparse_spmv_fp64(int64_t* hA_csrOffsets, int64_t* hA_columns, double* hA_values,
                            double* hX, double* hY, int64_t A_num_rows, int64_t A_num_cols,
                            int64_t A_nnz, int repeat, double* avg_time) {

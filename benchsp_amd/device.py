@@ -387,3 +387,32 @@ class RocsparseSpmv:
                 self._h = None
         except Exception:
             pass
+
+
+class RocsparseSpmm:
+    """rocSPARSE C = op(A) B (B row-major in A's value type, C FP32), the
+    vendor comparator for the SpMM kernels (bsp_rocsparse_spmm_*)."""
+
+    def __init__(self, A: DeviceCsr, N: int, transpose: bool = False, alg: int = 0):
+        self.A, self.N, self.transpose = A, N, transpose
+        self._desc = A.desc()
+        h = C.c_void_p()
+        lib().bsp_rocsparse_spmm_plan_create(C.byref(self._desc), N, int(transpose), alg,
+                                             stream_handle(), C.byref(h))
+        self._h = h
+
+    def __call__(self, B: torch.Tensor, Cout: torch.Tensor | None = None) -> torch.Tensor:
+        rows = self.A.ncols if self.transpose else self.A.nrows
+        assert B.dtype == self.A.val.dtype and B.is_contiguous() and B.shape[1] == self.N
+        if Cout is None:
+            Cout = torch.empty(rows, self.N, dtype=torch.float32, device=B.device)
+        lib().bsp_rocsparse_spmm(self._h, ptr(B), ptr(Cout), stream_handle())
+        return Cout
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_rocsparse_spmm_plan_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -493,6 +493,17 @@ bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
 bsp_status bsp_rocsparse_spmv(bsp_rocsparse_plan p, const void* x, void* y, bsp_stream s);
 bsp_status bsp_rocsparse_plan_destroy(bsp_rocsparse_plan p);
 
+/* Device-level rocSPARSE SpMM (counterpart of cusparse_spmm_fp16,
+ * src/cusparse.cu:288-390, generalised): C = op(A) B with A a device CSR of
+ * FP32, FP16 or BF16 values, B row-major (ncols x N, or nrows x N when
+ * transpose) of the same type, C row-major FP32, FP32 compute; prepared once,
+ * run per call. The vendor point next to the library's SpMM kernels. */
+typedef struct bsp_rocsparse_spmm_plan_s* bsp_rocsparse_spmm_plan;
+bsp_status bsp_rocsparse_spmm_plan_create(const bsp_csr* A, int64_t N, int transpose, int alg,
+                                          bsp_stream s, bsp_rocsparse_spmm_plan* out);
+bsp_status bsp_rocsparse_spmm(bsp_rocsparse_spmm_plan p, const void* B, float* C, bsp_stream s);
+bsp_status bsp_rocsparse_spmm_plan_destroy(bsp_rocsparse_spmm_plan p);
+
 /* Replaces int spmv_fp64_warpper(...) (include/csrspmv.h:2, src/csrspmv.cu:104-134):
  * thread-per-row CSR, 10 warmups, `repeat` timed launches; y = A*x. */
 int bsp_spmv_fp64_warpper(int32_t* rowPtr, int32_t* colIdx, int32_t rows, int32_t cols,

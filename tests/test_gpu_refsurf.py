@@ -293,3 +293,27 @@ def test_compat_driver_runs():
     r = subprocess.run([b.COMPAT_BIN, "96"], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("dtype_name", ["float32", "float16"])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_rocsparse_spmm_comparator(oracle, bsp_host, dtype_name, transpose):
+    import torch
+    from benchsp_amd.device import DeviceCsr, RocsparseSpmm
+    dt = getattr(torch, dtype_name)
+    A = bsp_host.powerlaw(3000, 12.0, 1.2, 400, seed=6, ncols=2500)
+    N = 64
+    rows_b = A.nrows if transpose else A.ncols
+    B = bsp_host.normal(rows_b * N, 19).reshape(rows_b, N)
+    d = DeviceCsr.from_host(A, dtype=dt)
+    op = RocsparseSpmm(d, N, transpose)
+    Bd = torch.from_numpy(B).to("cuda", dt)
+    Cm = op(Bd)
+    torch.cuda.synchronize()
+    a = d.val.double().cpu().numpy()  # the operands as stored
+    Bq = Bd.double().cpu().numpy()
+    if transpose:
+        ref = oracle.csr_spmm_t_f64(A.rowptr, A.col, a, Bq, A.ncols)
+    else:
+        ref = oracle.csr_spmm_f64(A.rowptr, A.col, a, Bq)
+    assert rel_fro(Cm.cpu().numpy(), ref) <= 1e-5
