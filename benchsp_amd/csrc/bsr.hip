@@ -175,15 +175,25 @@ __global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
   const int a = ai >> 2, r = ai & 3;
   const int s0 = rowptr[br], e0 = rowptr[br + 1];
   dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = s0; k0 < e0; k0 += 4) {
-    const int kb = k0 + a;
-    double av = 0.0, bv = 0.0;
-    if (kb < e0) {
-      av = ld_stream(&bval[(int64_t)kb * 16 + r * 4 + kk]);
-      const int64_t c = (int64_t)colidx[kb] * 4 + kk;
-      bv = c < ncols ? x[c] : 0.0;
+  // block column indices 64 at a time (one coalesced load), then four MFMA
+  // groups (16 blocks) per step with all their operand loads in flight
+  for (int base = s0; base < e0; base += kWave) {
+    const int nb = min(kWave, e0 - base);
+    const int cid = lane < nb ? colidx[base + lane] : 0;
+    for (int k0 = 0; k0 < nb; k0 += 16) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kb = k0 + 4 * u + a;
+        const int c4 = __shfl(cid, min(kb, kWave - 1), 64);
+        const int64_t c = (int64_t)c4 * 4 + kk;
+        const bool ok = kb < nb;
+        av[u] = ok ? ld_stream(&bval[(int64_t)(base + kb) * 16 + r * 4 + kk]) : 0.0;
+        bv[u] = ok && c < ncols ? x[c] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
   }
   // lane 16r + 4a + t keeps acc[a]; sum over a (lane bits 2..3)
   double t = a == 0 ? acc[0] : a == 1 ? acc[1] : a == 2 ? acc[2] : acc[3];
