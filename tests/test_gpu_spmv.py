@@ -90,6 +90,38 @@ def test_all_empty_rows(method):
     assert np.array_equal(y, np.zeros(1000))
 
 
+@pytest.mark.parametrize("method", METHODS)
+def test_degenerate_and_unsorted(oracle, method):
+    # a 1x1 matrix, one column, and rows whose columns are unsorted and
+    # repeated (duplicates are summed in order, as the CPU loop does); none of
+    # these are column-segmented, so every method accepts them
+    from benchsp_amd.host import Csr
+    rng = np.random.default_rng(12)
+    cases = [Csr(1, 1, np.array([0, 1]), np.array([0], np.int32), np.array([2.5])),
+             Csr(700, 1, np.arange(701, dtype=np.int64), np.zeros(700, np.int32),
+                 rng.uniform(-1, 1, 700))]
+    lens = rng.integers(0, 40, 500)
+    rp = np.zeros(501, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = rng.integers(0, 60, int(rp[-1])).astype(np.int32)  # unsorted, with repeats
+    cases.append(Csr(500, 60, rp, col, rng.uniform(-1, 1, int(rp[-1]))))
+    for A in cases:
+        x = rng.uniform(-1, 1, A.ncols)
+        _check(oracle, A, x, method, f"degenerate {A.nrows}x{A.ncols}")
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_zero_rows(method):
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    from benchsp_amd.host import Csr
+    A = Csr(0, 5, np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    op = CsrSpmv(DeviceCsr.from_host(A), method)
+    y = op(torch.ones(5, dtype=torch.float64, device="cuda"))
+    torch.cuda.synchronize()
+    assert y.numel() == 0
+
+
 def _run_seg(A, x, seg_cols, seg_mode=0):
     import torch
     from benchsp_amd.device import CsrSpmv, DeviceCsr
