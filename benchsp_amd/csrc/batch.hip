@@ -524,19 +524,30 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
     bsp_status st = BSP_OK;
     for (size_t m = 0; m < M && st == BSP_OK; ++m) {
       if (b->A[m].nrows == 0) continue;
-      float best_v = 1e30f, best_m = 1e30f;
+      // The timing is cache-warm (the copies share one matrix), so it ranks
+      // per-tile cost only: a candidate replaces the default window only
+      // when it is at least kMargin faster than the default.
+      constexpr float kMargin = 0.9f;
+      float best_v = 1e30f, best_m = 1e30f, def_v = 1e30f, def_m = 1e30f;
+      int win_v = kCap, win_m = kCapM;
       for (int c : kWindows) {
         std::vector<BTile> one;
         csr_tiles(rp[m], (int)m, c, one);
         float t = 0;
         st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st != BSP_OK) break;
-        if (t < best_v) best_v = t, b->win_spmv[m] = c, b->fmt[m] = BSP_BATCH_CSR;
+        if (c == kCap) def_v = t;
+        if (t < best_v) best_v = t, win_v = c;
         if (c > kCapM) continue;
         st = time_candidate(one, true, b->spmm.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st != BSP_OK) break;
-        if (t < best_m) best_m = t, b->win_spmm[m] = c;
+        if (c == kCapM) def_m = t;
+        if (t < best_m) best_m = t, win_m = c;
       }
+      b->fmt[m] = BSP_BATCH_CSR;
+      b->win_spmv[m] = best_v < kMargin * def_v ? win_v : kCap;
+      b->win_spmm[m] = best_m < kMargin * def_m ? win_m : kCapM;
+      best_v = b->win_spmv[m] == kCap ? def_v : best_v;
       if (st == BSP_OK && b->bsr[m]) {
         bsp_bsr_view v;
         bsp_bsr_view_get(b->bsr[m], &v);
@@ -546,7 +557,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         bsr_tiles(brp, (int)m, one);
         float t = 0;
         st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
-        if (st == BSP_OK && t < best_v) best_v = t, b->fmt[m] = BSP_BATCH_BSR4;
+        if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = BSP_BATCH_BSR4;
       }
       if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR4 && b->bsr[m]) {
         bsp_bsr_destroy(b->bsr[m]);
