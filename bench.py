@@ -458,10 +458,10 @@ def copy_probe():
 SPMV_KERNELS = ("csr_adaptive", "sum_partials")
 
 
-def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=None):
     """Median FETCH_SIZE / WRITE_SIZE per launch of the SpMV-leg kernels (KiB
     per dispatch), scaled by what the same counters read for the 256 MiB copy,
-    written to profiles/csr_adaptive_traffic.json."""
+    written to `out` (default profiles/csr_adaptive_traffic.json)."""
     import csv
     import statistics
 
@@ -490,7 +490,7 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
             "FETCH_SIZE_KiB": round(f, 1), "WRITE_SIZE_KiB": round(w, 1),
             "hbm_bytes": int(round((rs * f + ws * w) * 1024))}
     d = {"read_scale": round(rs, 4), "write_scale": round(ws, 4), "launches": launches}
-    out = os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
+    out = out or os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
     with open(out, "w") as fh:
         json.dump(d, fh, indent=1)
     print(json.dumps(d, indent=1))
