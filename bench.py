@@ -21,14 +21,17 @@ CSR SpMM in FP32.
 
 Formats leg: the same two matrices through every SpMV format and method of
 the library (scalar / balanced / split / adaptive CSR, 4x4 BSR on MFMA and
-VALU, DBSR, DASP) and rocSPARSE, as CSR-equivalent effective GB/s.
+VALU, DBSR, DASP) and rocSPARSE, as CSR-equivalent effective GB/s, with the
+wall time of each GPU format build (plan, analysis or format conversion from
+a device CSR). The CPU baseline leg times the restated host builders of the
+BSR4, DBSR and DASP formats on one core beside it.
 
 GNN leg: the differentiable CSR SpMM (forward, transpose-free A^T dY,
 SDDMM) on a 200,000-node power-law graph with 64 features, FP32 and BF16.
 
 Batched leg: 512 matrices of uneven size, an FP64 SpMV and an 8-column FP16
-SpMM each per step, plain per-matrix launches against the persistent
-autotuned schedule captured in a graph.
+SpMM each per step, plain per-matrix launches against the tiled
+single-launch schedules (fixed and autotuned) captured in a graph.
 
 Prints one JSON line. `--gpus N` with N > 1 is refused: this build measures
 one GPU.
@@ -102,6 +105,10 @@ def cpu_baseline(mats, xs, seconds: float):
             ts.append(time.perf_counter() - t0)
         ts.sort()
         one[name] = round(A.csr_bytes() / ts[2] / 1e9, 2)
+    # the restated host builders of the formats with a GPU builder here
+    builders = {name: {k: round(O.build_seconds(k, A.nrows, A.ncols, A.rowptr, A.col, A.val) * 1e3, 1)
+                       for k in ("bsr4", "dbsr", "dasp")}
+                for name, A in zip(("27pt-64^3", "powerlaw-1M"), mats)}
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -112,7 +119,8 @@ def cpu_baseline(mats, xs, seconds: float):
             "kind": "port",
             "sample": f"{reps} x (SpMV 27pt-64^3 + SpMV powerlaw-1M), {t_total:.1f} s on "
                       f"{threads} threads, same CSR-equivalent bytes as the GPU value",
-            "one_core_GBs": one, "host": {"nproc": os.cpu_count(), "cpu_model": model}}
+            "one_core_GBs": one, "builders_ms_one_core": builders,
+            "host": {"nproc": os.cpu_count(), "cpu_model": model}}
 
 
 def spmv_leg(args, dev):
@@ -339,7 +347,13 @@ def formats_leg(args, dev, mats, xs):
             cands["bsr4_valu"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=1)))(Bsr(d))
         res = {}
         for key, make in cands.items():
-            ops = [make(d) for d in dAs]
+            ops, build = [], []
+            for d in dAs:  # one build per copy, each from a synchronized start
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ops.append(make(d))
+                torch.cuda.synchronize()
+                build.append(time.perf_counter() - t0)
             torch.cuda.synchronize()
             for i in range(2 * ncopy):
                 ops[i % ncopy](xd, y)
@@ -352,7 +366,8 @@ def formats_leg(args, dev, mats, xs):
                 torch.cuda.synchronize()
                 ts.append(e0.elapsed_time(e1))
             us = float(np.median(ts)) * 1e3
-            res[key] = {"us": round(us, 2), "eff_GBs": round(A.csr_bytes() / (us * 1e-6) / 1e9, 1)}
+            res[key] = {"us": round(us, 2), "eff_GBs": round(A.csr_bytes() / (us * 1e-6) / 1e9, 1),
+                        "build_ms": round(float(np.median(build)) * 1e3, 2)}
             del ops
         out[name] = res
         del dAs
@@ -364,7 +379,8 @@ def batch_leg(args, dev):
     3-D 7-pt and lognormal-random by index mod 3), each with an FP64 SpMV and
     an 8-column FP16 SpMM per step. Schedules: plain (a thread-per-row launch
     per matrix and operation, on the stream and captured in a graph) and the
-    persistent tile launches (fixed 1024 windows, and autotuned), captured."""
+    tiled single launches (fixed windows 1024 / 512 at the default launch
+    size, and autotuned), captured."""
     import numpy as np
     import torch
     from benchsp_amd import host
@@ -432,6 +448,7 @@ def batch_leg(args, dev):
            "speedup_vs_plain_stream": round(res["plain_stream_ms"] / best, 2),
            "speedup_vs_plain_graph": round(res["plain_graph_ms"] / best, 2),
            "tuned_over_fixed": round(res["persistent_fixed_graph_ms"] / best, 3),
+           "grid": {"fixed": fixed.info()["grid"], "tuned": tuned.info()["grid"]},
            "tuner_choice": {
                "bsr4": int((ch["format"] == 1).sum()),
                "spmv_windows": {int(k): int(v) for k, v in

@@ -262,62 +262,131 @@ __global__ void dbsr_map(const int32_t* __restrict__ slen, const int32_t* __rest
 }
 
 // balanceDbsr (include/dbsr.h:94-169) with WARP_CAPACITY 64, both passes as
-// written. Mode 0 counts (out[0] = pass-1 warps, out[1] = pass-2 warps);
-// mode 1 fills per-warp row counts into cnt.
-__global__ void dbsr_balance(const int32_t* __restrict__ blcptr, int64_t brows, int mode,
-                             int32_t* __restrict__ out, int32_t* __restrict__ cnt) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  const int C = 64;
-  int now = 0, warpcount = 0;
-  for (int64_t i = 0; i < brows; ++i) {
-    const int len = blcptr[i + 1] - blcptr[i];
-    if (now + len <= C) {
-      now += len;
-    } else {
-      if (now != 0) {
-        warpcount++;
-        now = 0;
-      }
-      if (len <= C)
-        now = len;
-      else
-        warpcount += (len + C - 1) / C;
-    }
+// written: pass 2 starts from the fill pass 1 ended with, and a block row
+// that overflows the open warp in pass 2 opens the next warp without adding
+// its length to the fill. Each pass is a greedy packing whose only state
+// between block rows is the fill of the open warp (0..64), so it runs as a
+// parallel scan over chunks of block rows: dbsr_chunk_tables records, for
+// each of the 65 fills a chunk can be entered with, the fill it leaves and
+// the warps it opens; dbsr_chunk_walk composes those tables in order (one
+// wave, 65 table entries per chunk); dbsr_chunk_fill replays pass 2 over each
+// chunk from its entry fill and adds the per-warp block-row counts.
+constexpr int kDbCap = 64;
+constexpr int kDbStates = kDbCap + 1;
+constexpr int kDbChunk = 1024;
+constexpr int kDbTB = 128;
+
+// pass 1 step: the warps it closes or opens go to wc
+__device__ inline void db_pass1(int& now, int len, int& wc) {
+  if (now + len <= kDbCap) {
+    now += len;
+    return;
   }
-  if (now) warpcount++;
-  int now_warp = 0;
-  for (int64_t i = 0; i < brows; ++i) {
-    const int len = blcptr[i + 1] - blcptr[i];
-    if (now + len <= C) {
-      if (mode) cnt[now_warp]++;
-      now += len;
-    } else {
-      now = 0;
-      if (len <= C) {
-        now_warp++;
-        if (mode) cnt[now_warp]++;
-      } else {
-        for (int j = 1; j <= (len + C - 1) / C; ++j) {
-          now_warp++;
-          if (mode) cnt[now_warp]++;
-        }
-      }
-    }
+  if (now != 0) {
+    ++wc;
+    now = 0;
   }
-  if (!mode) {
-    out[0] = warpcount;
-    out[1] = brows > 0 ? now_warp + 1 : 0;
+  if (len <= kDbCap)
+    now = len;
+  else
+    wc += (len + kDbCap - 1) / kDbCap;
+}
+
+// pass 2 step: returns how many warps the block row advances the open warp by
+// (0: it joins the open warp)
+__device__ inline int db_pass2(int& now, int len) {
+  if (now + len <= kDbCap) {
+    now += len;
+    return 0;
+  }
+  now = 0;
+  return len <= kDbCap ? 1 : (len + kDbCap - 1) / kDbCap;
+}
+
+__device__ inline int db_chunk_lens(const int32_t* __restrict__ blcptr, int64_t brows, int* lens) {
+  const int64_t b0 = (int64_t)blockIdx.x * kDbChunk;
+  const int m = (int)min((int64_t)kDbChunk, brows - b0);
+  for (int i = threadIdx.x; i < m; i += blockDim.x) lens[i] = blcptr[b0 + i + 1] - blcptr[b0 + i];
+  __syncthreads();
+  return m;
+}
+
+// tab[c][k][s], k = 0..3: pass-1 exit fill, pass-1 warps, pass-2 exit fill,
+// pass-2 warp advance, for chunk c entered with fill s
+__global__ __launch_bounds__(kDbTB) void dbsr_chunk_tables(const int32_t* __restrict__ blcptr,
+                                                           int64_t brows, int32_t* __restrict__ tab) {
+  __shared__ int lens[kDbChunk];
+  const int m = db_chunk_lens(blcptr, brows, lens);
+  const int s = threadIdx.x;
+  if (s >= kDbStates) return;
+  int n1 = s, wc = 0, n2 = s, adv = 0;
+  for (int i = 0; i < m; ++i) {
+    const int len = lens[i];
+    db_pass1(n1, len, wc);
+    adv += db_pass2(n2, len);
+  }
+  int32_t* t = tab + (int64_t)blockIdx.x * 4 * kDbStates;
+  t[s] = n1;
+  t[kDbStates + s] = wc;
+  t[2 * kDbStates + s] = n2;
+  t[3 * kDbStates + s] = adv;
+}
+
+// the table entry of state st (0..64) from one wave holding entries 0..63 in
+// lanes and entry 64 in every lane
+__device__ inline int db_pick(int lo, int hi, int st) { return st < kWave ? __shfl(lo, st) : hi; }
+
+// one wave: out[0] = pass-1 warps, out[1] = pass-2 warps; entry[c] = pass-2
+// fill and first warp of chunk c
+__global__ __launch_bounds__(kWave) void dbsr_chunk_walk(const int32_t* __restrict__ tab,
+                                                         int64_t nchunks, int64_t brows,
+                                                         int32_t* __restrict__ out,
+                                                         int2* __restrict__ entry) {
+  const int lane = threadIdx.x;
+  int st = 0, wc = 0;
+  for (int64_t c = 0; c < nchunks; ++c) {
+    const int32_t* t = tab + c * 4 * kDbStates;
+    const int w = db_pick(t[kDbStates + lane], t[2 * kDbStates - 1], st);
+    st = db_pick(t[lane], t[kDbStates - 1], st);
+    wc += w;
+  }
+  if (st) ++wc;
+  int wv = 0;
+  for (int64_t c = 0; c < nchunks; ++c) {
+    const int32_t* t = tab + c * 4 * kDbStates;
+    if (lane == 0) entry[c] = make_int2(st, wv);
+    const int a = db_pick(t[3 * kDbStates + lane], t[4 * kDbStates - 1], st);
+    st = db_pick(t[2 * kDbStates + lane], t[3 * kDbStates - 1], st);
+    wv += a;
+  }
+  if (lane == 0) {
+    out[0] = wc;
+    out[1] = brows > 0 ? wv + 1 : 0;
   }
 }
 
-__global__ void serial_scan_i32(const int32_t* __restrict__ cnt, int n, int32_t* __restrict__ out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int s = 0;
-  out[0] = 0;
-  for (int i = 0; i < n; ++i) {
-    s += cnt[i];
-    out[i + 1] = s;
+// replays pass 2 over chunk c from entry[c]; cnt[w] += block rows of warp w
+// (atomics: the first and last warp of a chunk can be shared with neighbours)
+__global__ __launch_bounds__(kDbTB) void dbsr_chunk_fill(const int32_t* __restrict__ blcptr,
+                                                         int64_t brows, const int2* __restrict__ entry,
+                                                         int32_t* __restrict__ cnt) {
+  __shared__ int lens[kDbChunk];
+  const int m = db_chunk_lens(blcptr, brows, lens);
+  if (threadIdx.x != 0) return;
+  const int2 e = entry[blockIdx.x];
+  int now = e.x, w = e.y, run = 0;
+  for (int i = 0; i < m; ++i) {
+    const int adv = db_pass2(now, lens[i]);
+    if (adv == 0) {
+      ++run;
+      continue;
+    }
+    if (run) atomicAdd(cnt + w, run);
+    for (int j = 1; j < adv; ++j) atomicAdd(cnt + w + j, 1);
+    w += adv;
+    run = 1;
   }
+  if (run) atomicAdd(cnt + w, run);
 }
 
 // Wave per 4-row group; lane = slot*4 + r, 16 blocks per step; each lane
@@ -487,9 +556,15 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
                                                 h->map.get());
     BSP_HIP(hipGetLastError());
   }
-  DevBuf<int32_t> counts;
+  const int64_t nchunks = cdiv(v.brows, kDbChunk);
+  DevBuf<int32_t> counts, tab;
+  DevBuf<int2> entry;
   BSP_TRY(counts.alloc(2));
-  dbsr_balance<<<1, 1, 0, s>>>(h->blcptr.get(), v.brows, 0, counts.get(), nullptr);
+  BSP_TRY(tab.alloc(nchunks * 4 * kDbStates));
+  BSP_TRY(entry.alloc(nchunks));
+  if (nchunks > 0)
+    dbsr_chunk_tables<<<nchunks, kDbTB, 0, s>>>(h->blcptr.get(), v.brows, tab.get());
+  dbsr_chunk_walk<<<1, kWave, 0, s>>>(tab.get(), nchunks, v.brows, counts.get(), entry.get());
   BSP_HIP(hipGetLastError());
   int32_t c[2] = {0, 0};
   BSP_HIP(hipMemcpyAsync(c, counts.get(), sizeof(c), hipMemcpyDeviceToHost, s));
@@ -501,9 +576,11 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
   BSP_TRY(cnt.alloc(warps + 1));
   BSP_TRY(h->prefix.alloc(warps + 1));
   BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * (warps + 1), s));
-  dbsr_balance<<<1, 1, 0, s>>>(h->blcptr.get(), v.brows, 1, nullptr, cnt.get());
-  serial_scan_i32<<<1, 1, 0, s>>>(cnt.get(), warps, h->prefix.get());
+  if (nchunks > 0)
+    dbsr_chunk_fill<<<nchunks, kDbTB, 0, s>>>(h->blcptr.get(), v.brows, entry.get(), cnt.get());
   BSP_HIP(hipGetLastError());
+  // prefix[i] = block rows of warps 0..i-1, i = 0..warps (cnt[warps] is 0)
+  BSP_TRY(exclusive_scan_i32(cnt.get(), h->prefix.get(), (int64_t)warps + 1, s));
   BSP_HIP(hipStreamSynchronize(s));
   v.recorder = h->recorder.get();
   v.blcptr = h->blcptr.get();

@@ -296,7 +296,7 @@ def block_info(A: DeviceCsr, block_len: int = 4) -> dict:
 
 class BatchOptions(C.Structure):
     """struct bsp_batch_options."""
-    _fields_ = [("tune", C.c_int32), ("reserved0", C.c_int32), ("reserved", i64 * 7)]
+    _fields_ = [("tune", C.c_int32), ("grid", C.c_int32), ("reserved", i64 * 7)]
 
 
 PERSISTENT, PLAIN = 0, 1
@@ -309,10 +309,10 @@ class Batch:
 
     COLS = 8
 
-    def __init__(self, mats: list[DeviceCsr], tune: bool = True):
+    def __init__(self, mats: list[DeviceCsr], tune: int = 1, grid: int = 0):
         self.mats = mats  # keeps the borrowed device arrays alive
         descs = (CsrDesc * len(mats))(*[m.desc() for m in mats])
-        opt = BatchOptions(int(tune), 0)
+        opt = BatchOptions(int(tune), int(grid))
         h = C.c_void_p()
         lib().bsp_batch_create(descs, len(mats), C.byref(opt), stream_handle(), C.byref(h))
         self._h = h

@@ -416,21 +416,27 @@ bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_st
  * is y = A_m x_m (FP64) and Y_m = A_m X_m (A in FP16, X 8 columns of FP16,
  * FP32 accumulation and output) for every m. Vectors are concatenated: x_m
  * starts at col_off[m] of x, y_m at row_off[m] of y; X and Y are row-major
- * with 8 entries per row (X: 16-byte aligned). The persistent schedule runs
- * each operation as one launch over tiles of all matrices; the plain
- * schedule launches a thread-per-row kernel per matrix and operation. With
- * tune = 1 the builder times, per matrix, the CSR tile windows {128, 256,
- * 512, 1024} rows+entries for the SpMV and {128, 256, 512} for the SpMM
- * (which set rows per workgroup and lanes per row) and, where 4x4 blocks are
- * at least 11/16 full, the 4x4 BSR format for the SpMV, each in a full
- * persistent launch, and keeps the fastest; tune = 0 uses windows of 1024
- * (SpMV) and 512 (SpMM); tune = 2 puts every eligible matrix in 4x4 BSR
- * without timing.
+ * with 8 entries per row (X: 16-byte aligned). The tiled schedule
+ * (BSP_BATCH_PERSISTENT) runs each operation as one launch over tiles of all
+ * matrices, each workgroup a contiguous range of tiles; the plain schedule
+ * launches a thread-per-row kernel per matrix and operation. With tune = 1
+ * the builder times, per matrix, the CSR tile windows {128, 256, 512, 1024}
+ * rows+entries for the SpMV and {128, 256, 512} for the SpMM (which set rows
+ * per workgroup and lanes per row) and, where 4x4 blocks are at least 11/16
+ * full, the 4x4 BSR format for the SpMV, and keeps a candidate 10% faster
+ * than the default; then, per operation, it times the whole batch with the
+ * default and with the per-matrix tiles at several launch sizes and keeps
+ * the fastest. tune = 0 uses windows of 1024 (SpMV) and 512 (SpMM); tune = 2
+ * puts every eligible matrix in 4x4 BSR without timing.
  * A captured graph holds one step; replaying it needs no host work. */
 typedef struct bsp_batch_s* bsp_batch;
 typedef struct bsp_batch_options {
   int32_t tune;
-  int32_t reserved0;
+  /* workgroups of each tiled launch, capped at the tile count: 0 = 16x as
+   * many as the device holds at once (tune = 1: the fastest of 1x, 4x, 16x
+   * and one per tile); the resident count gives the persistent form, each
+   * workgroup one long range of tiles */
+  int32_t grid;
   int64_t reserved[7];
 } bsp_batch_options;
 enum { BSP_BATCH_CSR = 0, BSP_BATCH_BSR4 = 1 };

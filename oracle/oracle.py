@@ -238,6 +238,25 @@ def dasp(rowptr, col, val, threshold: float = 0.75, block_longest: int = 256):
     return out
 
 
+def build_seconds(kind: str, nrows, ncols, rowptr, col, val) -> float:
+    """Wall time of one restated host builder ("bsr4", "dbsr" or "dasp"),
+    the C call alone (inputs converted beforehand, nothing copied out)."""
+    import time
+    rp64, c32, v = _c64(rowptr), _c32(col), _cf64(val)
+    rp32 = _c32(rowptr)
+    L = lib()
+    t0 = time.perf_counter()
+    if kind == "bsr4":
+        L.orc_bcsr_free(L.orc_bcsr_build(int(nrows), int(ncols), rp64, c32, v, 4, 4))
+    elif kind == "dbsr":
+        L.orc_dbsr_free(L.orc_dbsr_build(int(nrows), int(ncols), rp64, c32, v))
+    elif kind == "dasp":
+        L.orc_dasp_free(L.orc_dasp_build(int(nrows), rp32, c32, v, 0.75, 256))
+    else:
+        raise ValueError(kind)
+    return time.perf_counter() - t0
+
+
 # --------------------------------------------------------------------------
 # The reference's own DBSR builder (compiled from include/dbsr.h)
 

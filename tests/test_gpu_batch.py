@@ -136,3 +136,30 @@ def test_batch_rejects_bad_input(bsp_host):
     d = DeviceCsr.from_host(A, dtype=torch.float32)
     with pytest.raises(BspError):
         Batch([d])
+
+
+def test_batch_grid_sizes(oracle, bsp_host):
+    # the launch size changes only which workgroup runs a tile, never a
+    # tile's arithmetic: every grid gives the same bits
+    import torch
+    from benchsp_amd._lib import BspError
+    from benchsp_amd.device import Batch
+    mats = _mix(bsp_host)
+    xs, Xs = _vectors(bsp_host, mats, 61)
+    dm, x, X = _device(mats, xs, Xs)
+    outs = []
+    for g in (0, 1, 3, 64, 1 << 30):
+        B = Batch(dm, tune=0, grid=g)
+        info = B.info()
+        tiles = max(info["spmv_tiles"], info["spmm_tiles"])
+        assert info["grid"] == (min(g, tiles) if g else info["grid"]) and info["grid"] >= 1
+        y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+        Y = torch.full((B.nrows, COLS), float("nan"), dtype=torch.float32, device="cuda")
+        B.run(x, y, X, Y)
+        torch.cuda.synchronize()
+        outs.append((y.cpu().numpy(), Y.cpu().numpy()))
+    _check(oracle, mats, xs, Xs, outs[0][0], outs[0][1], B)
+    for y, Y in outs[1:]:
+        assert np.array_equal(y, outs[0][0]) and np.array_equal(Y, outs[0][1])
+    with pytest.raises(BspError):
+        Batch(dm, tune=0, grid=-1)

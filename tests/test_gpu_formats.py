@@ -82,6 +82,13 @@ def _dbsr_inputs(bsp_host):
                                        g["rowptr"].astype(np.int64), g["col"], g["val"])
     rng = np.random.default_rng(0)
     yield "overflow-case", _rand_csr(rng, [0, 1, 2, 3, 4, 5, 9, 16, 17, 33, 70, 0, 2, 64, 65, 8, 1], 90)
+    # ~4,600 block rows (5 chunks of the parallel warp packing) whose block
+    # counts sit around the 64-block warp capacity, below it and far above it
+    kind = rng.choice(3, 18_500, p=[0.6, 0.3, 0.1])
+    lens = np.where(kind == 0, rng.integers(0, 61, kind.size),
+                    np.where(kind == 1, rng.integers(100, 261, kind.size),
+                             rng.integers(256, 1101, kind.size)))
+    yield "warp-capacity-mix", _rand_csr(rng, lens, 1500)
     yield "stencil27-64", bsp_host.stencil27_64()
     yield "powerlaw-1M", bsp_host.powerlaw_1m()
 
