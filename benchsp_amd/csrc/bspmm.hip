@@ -173,11 +173,15 @@ __global__ __launch_bounds__(kTB) void panelize(const uint16_t* __restrict__ src
 // row (block column for dB) with NW waves; wave w owns column tiles
 // [w*TPW, (w+1)*TPW) across all of the row's blocks, so its accumulators are
 // final (no cross-wave reduction) and blocks are added in index order. The
-// block loop runs eight blocks at a time so their operand loads overlap.
+// row's blocks pass through LDS in chunks of 8*NW: their indices and scales
+// first, then their A fragments in lane order (each block read from HBM once
+// per workgroup rather than once per wave); each wave then streams its panel
+// fragments, U blocks in flight.
 template <int NT>
 struct RowsShape {
   static constexpr int NW = NT < 8 ? NT : 8;  // waves
   static constexpr int TPW = NT / NW;         // 16-column tiles per wave
+  static constexpr int CH = 8 * NW;           // blocks per LDS chunk
 };
 
 template <int NT, bool BF, bool TRANS>
@@ -189,46 +193,58 @@ __global__ __launch_bounds__(512) void bspmm_rows(const int32_t* __restrict__ pt
                                                   const uint16_t* __restrict__ panel,
                                                   float* __restrict__ out, int64_t out_rows,
                                                   int64_t nouter) {
-  constexpr int TPW = RowsShape<NT>::TPW;
-  constexpr int U = 8;
+  constexpr int NW = RowsShape<NT>::NW, TPW = RowsShape<NT>::TPW, CH = RowsShape<NT>::CH;
+  constexpr int U = 16 / TPW;
+  __shared__ u2 sa[CH][kWave];
+  __shared__ int32_t sb[CH], so[CH];
+  __shared__ float ss[CH];
   const int64_t r = blockIdx.x;
   if (r >= nouter) return;
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
   const int i = lane & 15, kg = lane >> 4;
   const uint16_t* pw = panel + ((int64_t)w * TPW * 64 + lane) * 4;
   f4 acc[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-  const int k1 = ptr[r + 1];
-  int k = ptr[r];
-  for (; k + U <= k1; k += U) {
-    u2 a[U], bo[U][TPW];
-    float s[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = TRANS ? perm[k + u] : k + u;
-      const int64_t o = other[k + u];
-      a[u] = *reinterpret_cast<const u2*>(blocks + (int64_t)b * 256 + i * 16 + 4 * kg);
-      s[u] = scale[b];
-#pragma unroll
-      for (int t = 0; t < TPW; ++t)
-        bo[u][t] = *reinterpret_cast<const u2*>(pw + (o * NT + t) * 256);
+  const int k0 = ptr[r], k1 = ptr[r + 1];
+  for (int c0 = k0; c0 < k1; c0 += CH) {
+    const int cn = min(CH, k1 - c0);
+    if (tid < cn) {
+      const int b = TRANS ? perm[c0 + tid] : c0 + tid;
+      sb[tid] = b;
+      so[tid] = other[c0 + tid];
+      ss[tid] = scale[b];
     }
+    __syncthreads();
+    // unconditional loads (slots past cn repeat the last block and are never
+    // read), so all CH/NW of a wave's loads are in flight together
+    u2 stage[CH / NW];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int q = 0; q < CH / NW; ++q)
+      stage[q] = *reinterpret_cast<const u2*>(blocks + (int64_t)sb[min(w + q * NW, cn - 1)] * 256 +
+                                              i * 16 + 4 * kg);
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) acc[t] += s[u] * mma16<BF>(a[u], bo[u][t], f4{0.f, 0.f, 0.f, 0.f});
-  }
-  for (; k < k1; ++k) {
-    const int b = TRANS ? perm[k] : k;
-    const int64_t o = other[k];
-    const u2 a = *reinterpret_cast<const u2*>(blocks + (int64_t)b * 256 + i * 16 + 4 * kg);
-    const float s = scale[b];
+    for (int q = 0; q < CH / NW; ++q) sa[w + q * NW][lane] = stage[q];
+    __syncthreads();
+    for (int j = 0; j < cn; j += U) {
+      u2 bo[U][TPW];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      const u2 bo = *reinterpret_cast<const u2*>(pw + (o * NT + t) * 256);
-      acc[t] += s * mma16<BF>(a, bo, f4{0.f, 0.f, 0.f, 0.f});
+      for (int u = 0; u < U; ++u) {
+        const int64_t o = so[min(j + u, cn - 1)];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+          bo[u][t] = *reinterpret_cast<const u2*>(pw + (o * NT + t) * 256);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < cn) {
+          const u2 a = sa[j + u][lane];
+          const float sc = ss[j + u];
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) acc[t] += sc * mma16<BF>(a, bo[u][t], f4{0.f, 0.f, 0.f, 0.f});
+        }
     }
+    __syncthreads();  // the next chunk overwrites the LDS arrays
   }
   constexpr int N = NT * 16;
 #pragma unroll
@@ -259,20 +275,37 @@ __global__ __launch_bounds__(kDvWaves * 64) void bspmm_dvals(const int32_t* __re
   const int i = lane & 15, kg = lane >> 4;
   u4 a[S];
   const int64_t arow = br * 16 + i;
+  // clamped unconditional loads, rows past M (K) zeroed afterwards: no branch
+  // splits a batch of loads
 #pragma unroll
-  for (int s = 0; s < S; ++s)
-    a[s] = arow < M ? *reinterpret_cast<const u4*>(dC + arow * N + 32 * s + 8 * kg) : u4{0, 0, 0, 0};
-  for (int b = browptr[br] + w; b < browptr[br + 1]; b += kDvWaves) {
-    const int64_t brow = (int64_t)bcol[b] * 16 + i;
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < S; ++s) {
+    const u4 v = *reinterpret_cast<const u4*>(dC + min(arow, M - 1) * N + 32 * s + 8 * kg);
+    a[s] = arow < M ? v : u4{0, 0, 0, 0};
+  }
+  // two blocks per step (b and b + kDvWaves), their B loads in flight together
+  const int e = browptr[br + 1];
+  for (int b = browptr[br] + w; b < e; b += 2 * kDvWaves) {
+    const int b2 = b + kDvWaves;
+    const bool two = b2 < e;
+    const int64_t brow[2] = {(int64_t)bcol[b] * 16 + i, (int64_t)bcol[two ? b2 : b] * 16 + i};
+    u4 bo[2][S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const u4 bo =
-          brow < K ? *reinterpret_cast<const u4*>(B + brow * N + 32 * s + 8 * kg) : u4{0, 0, 0, 0};
-      acc = mma32<BF>(a[s], bo, acc);
-    }
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) dvals[(int64_t)b * 256 + (4 * kg + v) * 16 + i] = acc[v];
+      for (int s = 0; s < S; ++s) {
+        const u4 v = *reinterpret_cast<const u4*>(B + min(brow[h], K - 1) * N + 32 * s + 8 * kg);
+        bo[h][s] = brow[h] < K ? v : u4{0, 0, 0, 0};
+      }
+    f4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int s = 0; s < S; ++s) acc[h] = mma32<BF>(a[s], bo[h][s], acc[h]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dvals[(int64_t)b * 256 + (4 * kg + v) * 16 + i] = acc[0][v];
+    if (two)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dvals[(int64_t)b2 * 256 + (4 * kg + v) * 16 + i] = acc[1][v];
   }
 }
 
