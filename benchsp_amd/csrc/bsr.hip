@@ -228,19 +228,25 @@ __global__ void dbsr_widths(const int32_t* __restrict__ slen, int64_t nrows, int
   w[g] = m;
 }
 
-__global__ void dbsr_place(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                           const double* __restrict__ val, const int32_t* __restrict__ rec,
-                           const int32_t* __restrict__ blcptr, int64_t nrows,
-                           int32_t* __restrict__ ocol, double* __restrict__ oval) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// wave per (sorted) row, lanes striding the row's entries
+__global__ __launch_bounds__(kTB) void dbsr_place(const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val,
+                                                  const int32_t* __restrict__ rec,
+                                                  const int32_t* __restrict__ blcptr, int64_t nrows,
+                                                  int32_t* __restrict__ ocol,
+                                                  double* __restrict__ oval) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (i >= nrows) return;
   const int o = rec[i];
   const int64_t b0 = blcptr[i / 4];
   const int rr = (int)(i % 4) * 4;
-  for (int j = rowptr[o], z = 0, e = rowptr[o + 1]; j < e; ++j, ++z) {
+  const int j0 = rowptr[o], e = rowptr[o + 1];
+  for (int z = lane; z < e - j0; z += kWave) {
     const int64_t d = (b0 + z / 4) * 16 + rr + (z % 4);
-    ocol[d] = col[j];
-    oval[d] = val[j];
+    ocol[d] = col[j0 + z];
+    oval[d] = val[j0 + z];
   }
 }
 
@@ -342,22 +348,47 @@ __global__ __launch_bounds__(kWave) void dbsr_chunk_walk(const int32_t* __restri
                                                          int64_t nchunks, int64_t brows,
                                                          int32_t* __restrict__ out,
                                                          int2* __restrict__ entry) {
+  // The table loads do not depend on the running fill, so eight chunks'
+  // entries are loaded together before the eight dependent picks.
+  constexpr int G = 8;
   const int lane = threadIdx.x;
   int st = 0, wc = 0;
-  for (int64_t c = 0; c < nchunks; ++c) {
-    const int32_t* t = tab + c * 4 * kDbStates;
-    const int w = db_pick(t[kDbStates + lane], t[2 * kDbStates - 1], st);
-    st = db_pick(t[lane], t[kDbStates - 1], st);
-    wc += w;
+  for (int64_t c0 = 0; c0 < nchunks; c0 += G) {
+    int el[G], eh[G], wl[G], wh[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int32_t* t = tab + min(c0 + q, nchunks - 1) * 4 * kDbStates;
+      el[q] = t[lane];
+      eh[q] = t[kDbStates - 1];
+      wl[q] = t[kDbStates + lane];
+      wh[q] = t[2 * kDbStates - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+      if (c0 + q < nchunks) {
+        wc += db_pick(wl[q], wh[q], st);
+        st = db_pick(el[q], eh[q], st);
+      }
   }
   if (st) ++wc;
   int wv = 0;
-  for (int64_t c = 0; c < nchunks; ++c) {
-    const int32_t* t = tab + c * 4 * kDbStates;
-    if (lane == 0) entry[c] = make_int2(st, wv);
-    const int a = db_pick(t[3 * kDbStates + lane], t[4 * kDbStates - 1], st);
-    st = db_pick(t[2 * kDbStates + lane], t[3 * kDbStates - 1], st);
-    wv += a;
+  for (int64_t c0 = 0; c0 < nchunks; c0 += G) {
+    int el[G], eh[G], al[G], ah[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int32_t* t = tab + min(c0 + q, nchunks - 1) * 4 * kDbStates;
+      el[q] = t[2 * kDbStates + lane];
+      eh[q] = t[3 * kDbStates - 1];
+      al[q] = t[3 * kDbStates + lane];
+      ah[q] = t[4 * kDbStates - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+      if (c0 + q < nchunks) {
+        if (lane == 0) entry[c0 + q] = make_int2(st, wv);
+        wv += db_pick(al[q], ah[q], st);
+        st = db_pick(el[q], eh[q], st);
+      }
   }
   if (lane == 0) {
     out[0] = wc;
@@ -549,9 +580,9 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
   BSP_HIP(hipMemsetAsync(h->colidx.get(), 0, sizeof(int32_t) * (nb ? nb * 16 : 1), s));
   BSP_HIP(hipMemsetAsync(h->val.get(), 0, sizeof(double) * (nb ? nb * 16 : 1), s));
   if (n > 0) {
-    dbsr_place<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, static_cast<const double*>(A.val),
-                                            h->recorder.get(), h->blcptr.get(), n, h->colidx.get(),
-                                            h->val.get());
+    dbsr_place<<<cdiv(n * kWave, kTB), kTB, 0, s>>>(
+        A.rowptr, A.colidx, static_cast<const double*>(A.val), h->recorder.get(),
+        h->blcptr.get(), n, h->colidx.get(), h->val.get());
     dbsr_map<<<cdiv(v.brows, kTB), kTB, 0, s>>>(slen.get(), h->blcptr.get(), n, v.brows,
                                                 h->map.get());
     BSP_HIP(hipGetLastError());

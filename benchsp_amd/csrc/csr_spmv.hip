@@ -296,55 +296,85 @@ static bsp_status stable_partition(const int32_t* flags, int64_t n, DevBuf<int32
 constexpr int64_t kDefaultSegCols = int64_t(1) << 19;  // 4 MB of FP64 x (one XCD's L2)
 constexpr int64_t kXcdSegCols = int64_t(1) << 18;      // 2 MB slices when XCDs are pinned
 
+// The three plan-builder passes below run a wave per row (lanes stride the
+// row's entries), so a 5,000-entry row is 79 steps of one wave rather than
+// one thread's loop.
+
 // Counts entries farther than `half` columns from the (scaled) diagonal and
 // rows whose columns are not strictly ascending.
-__global__ void csr_scatter_stats(const int32_t* __restrict__ rowptr,
-                                  const int32_t* __restrict__ col, int64_t nrows, int64_t ncols,
-                                  int64_t half, unsigned long long* far_count, int* unsorted) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const int64_t d = r * ncols / nrows;
+__global__ __launch_bounds__(kWG) void csr_scatter_stats(const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ col,
+                                                         int64_t nrows, int64_t ncols, int64_t half,
+                                                         unsigned long long* far_count,
+                                                         int* unsorted) {
+  // a fixed grid of waves strides over the rows, so the totals take one
+  // atomic per wave, not one per row
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
   unsigned long long far = 0;
   int bad = 0;
-  int prev = -1;
-  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) {
-    const int c = col[j];
-    far += (c > d + half || c < d - half);
-    bad |= c <= prev;
-    prev = c;
-  }
-  if (far) atomicAdd(far_count, far);
-  if (bad) atomicOr(unsorted, 1);
-}
-
-// cnt[s*(nrows+1) + r] = entries of row r in column segment s.
-__global__ void seg_count(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                          int64_t nrows, int64_t width, int32_t* __restrict__ cnt) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j)
-    cnt[(int64_t)(col[j] / width) * (nrows + 1) + r] += 1;
-}
-
-// Copies each row's run of segment-s entries (contiguous: columns ascend)
-// to segment s's sub-CSR.
-__global__ void seg_scatter(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                            const double* __restrict__ val, int64_t nrows, int64_t width,
-                            const int32_t* __restrict__ segrp, const int64_t* __restrict__ base,
-                            int32_t* __restrict__ ocol, double* __restrict__ oval) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  int64_t cur = -1, dst = 0;
-  for (int j = rowptr[r], e = rowptr[r + 1]; j < e; ++j) {
-    const int c = col[j];
-    const int64_t sgm = c / width;
-    if (sgm != cur) {
-      cur = sgm;
-      dst = base[sgm] + segrp[sgm * (nrows + 1) + r];
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < nrows; r += nw) {
+    const int64_t d = r * ncols / nrows;
+    const int j0 = rowptr[r], e = rowptr[r + 1];
+    for (int j = j0 + lane; j < e; j += kWave) {
+      const int c = col[j];
+      far += (c > d + half || c < d - half);
+      bad |= j > j0 && c <= col[j - 1];
     }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) far += __shfl_xor(far, o, 64);
+  bad = __any(bad);
+  if (lane == 0 && far) atomicAdd(far_count, far);
+  if (lane == 0 && bad) atomicOr(unsorted, 1);
+}
+
+static int scatter_stats_grid(int64_t nrows) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(nrows * kWave, kWG), 2048));
+}
+
+// cnt[s*(nrows+1) + r] = entries of row r in column segment s (columns
+// ascend, so they form one run): the lane at the run's first entry j adds -j,
+// the lane at its last entry adds j + 1 (cnt starts at zero).
+__global__ __launch_bounds__(kWG) void seg_count(const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ col, int64_t nrows,
+                                                 int64_t width, int32_t* __restrict__ cnt) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= nrows) return;
+  const int j0 = rowptr[r], e = rowptr[r + 1];
+  for (int j = j0 + lane; j < e; j += kWave) {
+    const int64_t sg = col[j] / width;
+    int32_t* c = cnt + sg * (nrows + 1) + r;
+    if (j == j0 || col[j - 1] / width != sg) atomicAdd(c, -j);
+    if (j == e - 1 || col[j + 1] / width != sg) atomicAdd(c, j + 1);
+  }
+}
+
+// Copies each row's run of segment-s entries (contiguous: columns ascend) to
+// segment s's sub-CSR: entry j of the row goes to base[s] + segrp[s][r] plus
+// its offset in the run, the run starting after the row's entries of the
+// segments before s.
+__global__ __launch_bounds__(kWG) void seg_scatter(const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col,
+                                                   const double* __restrict__ val, int64_t nrows,
+                                                   int64_t width, const int32_t* __restrict__ segrp,
+                                                   const int64_t* __restrict__ base,
+                                                   int32_t* __restrict__ ocol,
+                                                   double* __restrict__ oval) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= nrows) return;
+  const int j0 = rowptr[r], e = rowptr[r + 1];
+  for (int j = j0 + lane; j < e; j += kWave) {
+    const int c = col[j];
+    const int64_t sg = c / width;
+    int64_t before = 0;
+    for (int64_t q = 0; q < sg; ++q)
+      before += segrp[q * (nrows + 1) + r + 1] - segrp[q * (nrows + 1) + r];
+    const int64_t dst = base[sg] + segrp[sg * (nrows + 1) + r] + (j - j0 - before);
     ocol[dst] = c;
     oval[dst] = val[j];
-    ++dst;
   }
 }
 
@@ -374,8 +404,9 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     BSP_TRY(unsorted.alloc(1));
     BSP_HIP(hipMemsetAsync(far.get(), 0, sizeof(unsigned long long), s));
     BSP_HIP(hipMemsetAsync(unsorted.get(), 0, sizeof(int), s));
-    csr_scatter_stats<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, A.ncols,
-                                                   kDefaultSegCols / 2, far.get(), unsorted.get());
+    csr_scatter_stats<<<scatter_stats_grid(n), kWG, 0, s>>>(A.rowptr, A.colidx, n, A.ncols,
+                                                            kDefaultSegCols / 2, far.get(),
+                                                            unsorted.get());
     BSP_HIP(hipGetLastError());
     unsigned long long nfar = 0;
     int uns = 0;
@@ -396,8 +427,8 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     BSP_TRY(unsorted.alloc(1));
     BSP_HIP(hipMemsetAsync(far.get(), 0, sizeof(unsigned long long), s));
     BSP_HIP(hipMemsetAsync(unsorted.get(), 0, sizeof(int), s));
-    csr_scatter_stats<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, A.ncols, W / 2,
-                                                   far.get(), unsorted.get());
+    csr_scatter_stats<<<scatter_stats_grid(n), kWG, 0, s>>>(A.rowptr, A.colidx, n, A.ncols, W / 2,
+                                                            far.get(), unsorted.get());
     BSP_HIP(hipGetLastError());
     int uns = 0;
     BSP_TRY(fetch(unsorted.get(), &uns, s));
@@ -419,7 +450,7 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     DevBuf<int32_t> cnt;
     BSP_TRY(cnt.alloc(nseg * (n + 1)));
     BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * nseg * (n + 1), s));
-    seg_count<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, W, cnt.get());
+    seg_count<<<cdiv(n * kWave, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, W, cnt.get());
     BSP_HIP(hipGetLastError());
     DevBuf<int32_t> segrp;
     BSP_TRY(segrp.alloc(nseg * (n + 1)));
@@ -436,9 +467,9 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
                            hipMemcpyHostToDevice, s));
     BSP_TRY(p->seg_col.alloc(A.nnz));
     BSP_TRY(p->seg_val.alloc(A.nnz));
-    seg_scatter<<<cdiv(n, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, static_cast<const double*>(A.val),
-                                             n, W, segrp.get(), dbase.get(), p->seg_col.get(),
-                                             p->seg_val.get());
+    seg_scatter<<<cdiv(n * kWave, kWG), kWG, 0, s>>>(
+        A.rowptr, A.colidx, static_cast<const double*>(A.val), n, W, segrp.get(), dbase.get(),
+        p->seg_col.get(), p->seg_val.get());
     BSP_HIP(hipGetLastError());
     for (int64_t k = 0; k < nseg; ++k) {
       CsrSegment& g = p->seg[k];

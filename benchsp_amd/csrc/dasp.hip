@@ -202,25 +202,26 @@ __global__ void block_slabs(const int32_t* __restrict__ blen, int32_t nb, int32_
 
 // regular part, slab-major (dasp.cu:1442-1487): row g's entry t of block b
 // goes to block_ptr[b] + (t/4)*32 + g*4 + t%4; irregular tail after it.
-__global__ void block_fill(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cid,
-                           const double* __restrict__ val, const int32_t* __restrict__ ridA,
-                           int32_t nb, const int32_t* __restrict__ bptr,
-                           const int32_t* __restrict__ irpt, double* __restrict__ rv,
-                           int32_t* __restrict__ rc, double* __restrict__ iv,
-                           int32_t* __restrict__ ic) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kTB) void block_fill(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cid,
+    const double* __restrict__ val, const int32_t* __restrict__ ridA, int32_t nb,
+    const int32_t* __restrict__ bptr, const int32_t* __restrict__ irpt, double* __restrict__ rv,
+    int32_t* __restrict__ rc, double* __restrict__ iv, int32_t* __restrict__ ic) {
+  // wave per slot k, lanes striding its entries
+  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (k >= nb) return;
   const int b = (int)(k / BS), g = (int)(k % BS);
   const int blocklen = (bptr[b + 1] - bptr[b]) / BS;
   const int r = ridA[k];
   const int a0 = rowptr[r], L = rowptr[r + 1] - a0;
-  for (int t = 0; t < blocklen && t < L; ++t) {
+  for (int t = lane; t < blocklen && t < L; t += kWave) {
     const int64_t d = bptr[b] + (t / MK) * (BS * MK) + g * MK + t % MK;
     rv[d] = val[a0 + t];
     rc[d] = cid[a0 + t];
   }
   const int i0 = irpt[k], il = irpt[k + 1] - i0;
-  for (int t = 0; t < il; ++t) {
+  for (int t = lane; t < il; t += kWave) {
     iv[i0 + t] = val[a0 + L - il + t];
     ic[i0 + t] = cid[a0 + L - il + t];
   }
@@ -490,10 +491,9 @@ static bsp_status build_dasp(const bsp_csr& A, double threshold, int32_t block_l
   BSP_HIP(hipMemsetAsync(h->reg_val.get(), 0, sizeof(double) * (freg ? freg : 1), s));
   BSP_HIP(hipMemsetAsync(h->reg_cid.get(), 0, sizeof(int32_t) * (freg ? freg : 1), s));
   if (nb > 0)
-    block_fill<<<cdiv(nb, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, aval, h->ridA.get(), nb,
-                                             h->block_ptr.get(), h->irreg_rpt.get(),
-                                             h->reg_val.get(), h->reg_cid.get(),
-                                             h->irreg_val.get(), h->irreg_cid.get());
+    block_fill<<<cdiv((int64_t)nb * kWave, kTB), kTB, 0, s>>>(
+        A.rowptr, A.colidx, aval, h->ridA.get(), nb, h->block_ptr.get(), h->irreg_rpt.get(),
+        h->reg_val.get(), h->reg_cid.get(), h->irreg_val.get(), h->irreg_cid.get());
   BSP_HIP(hipGetLastError());
   BSP_HIP(hipStreamSynchronize(s));
   v.order_rid = h->order_rid.get();

@@ -255,23 +255,32 @@ __global__ void fill_items(const int32_t* __restrict__ ptr, int64_t n,
   if (multi[r]) comb[combpos[r]] = make_int4((int)r, slotptr[r], c, 0);
 }
 
-__global__ void csc_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                         int64_t n, int64_t ncols, int64_t* __restrict__ key,
-                         int32_t* __restrict__ idx) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// wave per row (lanes stride the row), so a long row is not one thread's loop
+__global__ __launch_bounds__(kTB) void csc_keys(const int32_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ col, int64_t n,
+                                                int64_t* __restrict__ key,
+                                                int32_t* __restrict__ idx) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (r >= n) return;
-  for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+  for (int j = rowptr[r] + lane, e = rowptr[r + 1]; j < e; j += kWave) {
     key[j] = (int64_t)col[j] * n + r;
     idx[j] = j;
   }
 }
 
-__global__ void csc_fill(const int64_t* __restrict__ skey, int64_t nnz, int64_t n,
-                         int32_t* __restrict__ rowidx, int32_t* __restrict__ colcnt) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nnz) return;
-  rowidx[k] = (int32_t)(skey[k] % n);
-  atomicAdd(&colcnt[skey[k] / n], 1);
+// Row indices and column pointers from the (column, row)-sorted keys:
+// colptr[c] is the first k whose column is >= c, written by the thread at the
+// boundary where the column changes (k = nnz closes the last columns), so
+// every entry is written once and no counters are needed.
+__global__ void csc_fill(const int64_t* __restrict__ skey, int64_t nnz, int64_t n, int64_t ncols,
+                         int32_t* __restrict__ rowidx, int32_t* __restrict__ colptr) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nnz) return;
+  const int64_t c = k < nnz ? skey[k] / n : ncols;
+  const int64_t cp = k > 0 ? skey[k - 1] / n : -1;
+  if (k < nnz) rowidx[k] = (int32_t)(skey[k] % n);
+  for (int64_t q = cp + 1; q <= c; ++q) colptr[q] = (int32_t)k;
 }
 
 }  // namespace
@@ -316,26 +325,24 @@ static bsp_status build_spmm_plan(bsp_spmm_plan_s* p, hipStream_t s) {
   // CSC of the pattern: stable sort of (col, row) keys
   const int64_t nnz = A.nnz;
   DevBuf<int64_t> key, skey;
-  DevBuf<int32_t> idx, colcnt;
+  DevBuf<int32_t> idx;
   BSP_TRY(key.alloc(nnz));
   BSP_TRY(skey.alloc(nnz));
   BSP_TRY(idx.alloc(nnz));
   BSP_TRY(p->perm.alloc(nnz));
   BSP_TRY(p->rowidx.alloc(nnz));
-  BSP_TRY(colcnt.alloc(A.ncols + 1));
   BSP_TRY(p->colptr.alloc(A.ncols + 1));
-  BSP_HIP(hipMemsetAsync(colcnt.get(), 0, sizeof(int32_t) * (A.ncols + 1), s));
   if (nnz > 0) {
-    csc_keys<<<cdiv(A.nrows, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, A.nrows, A.ncols, key.get(),
-                                                idx.get());
+    csc_keys<<<cdiv(A.nrows * kWave, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, A.nrows, key.get(),
+                                                        idx.get());
     BSP_HIP(hipGetLastError());
     int bits = 1;
     while (bits < 63 && (int64_t(1) << bits) <= A.ncols * A.nrows) ++bits;
     BSP_TRY(sort_pairs_i64key(key.get(), skey.get(), idx.get(), p->perm.get(), nnz, bits, s));
-    csc_fill<<<cdiv(nnz, kTB), kTB, 0, s>>>(skey.get(), nnz, A.nrows, p->rowidx.get(), colcnt.get());
-    BSP_HIP(hipGetLastError());
   }
-  BSP_TRY(exclusive_scan_i32(colcnt.get(), p->colptr.get(), A.ncols + 1, s));
+  csc_fill<<<cdiv(nnz + 1, kTB), kTB, 0, s>>>(skey.get(), nnz, A.nrows, A.ncols, p->rowidx.get(),
+                                               p->colptr.get());
+  BSP_HIP(hipGetLastError());
   p->bwd.ptr = p->colptr.get();
   p->bwd.idx = p->rowidx.get();
   p->bwd.perm = p->perm.get();
