@@ -79,19 +79,23 @@ __device__ __forceinline__ void bsr4_rows(const int32_t* __restrict__ brp,
 // Rows [r0, r1) with entries [p0, p1) of Y = A X (at most kCapM rows and
 // 1.5 kCapM entries, or one row), X rows of 8 halves, Y rows of 8 floats.
 // Entries are streamed coalesced, one lane per entry: a 16-byte load brings
-// the entry's X row, the 8 products go to LDS column-major; then q lanes per
+// the entry's X row. The products go through LDS four columns at a time
+// (column-major, odd stride: columns in distinct banks), and q lanes per
 // (row, column) pair add a row's products (q = the largest power of two
-// <= 8 with rows*8*q <= 256). One long row takes the whole workgroup.
+// <= 8 with rows*4*q <= 128); half-width staging keeps the LDS per
+// workgroup at 14 KB, so eight workgroups fit a CU. One long row takes the
+// whole workgroup.
 constexpr int kCapM = 512;
 constexpr int kMaxM = kCapM + kCapM / 2;
 constexpr int kItM = kMaxM / kWG;
+constexpr int kHalf = kCols / 2;
 
 __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
                                            const int32_t* __restrict__ col,
                                            const uint16_t* __restrict__ hval,
                                            const uint4* __restrict__ X, float* __restrict__ Y,
                                            int r0, int p0, int r1, int p1) {
-  __shared__ float prod[kCols][kMaxM + 1];  // odd stride: columns in distinct banks
+  __shared__ float prod[kHalf][kMaxM + 1];
   __shared__ int32_t rptr[kCapM + 1];
   __shared__ float red[kWG / kWave][kCols];
   const int tid = threadIdx.x;
@@ -140,28 +144,34 @@ __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
     for (int k = 0; k < kItM; ++k)
       if (k < iters) xq[k] = X[ci[k]];
   }
-  if (nnz > 0) {
+  int q = 1;
+  while (q < 8 && nrows * kHalf * q * 2 <= kWG) q <<= 1;
+  const int sub = tid & (q - 1);
 #pragma unroll
-    for (int k = 0; k < kItM; ++k) {
-      const int i = tid + k * kWG;
-      if (k < iters && i < nnz) {
-        float acc[kCols] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        unpack(xq[k], half_to_f32(hv[k]), acc);
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();  // the first half's sums have read prod
+    if (nnz > 0) {
 #pragma unroll
-        for (int c = 0; c < kCols; ++c) prod[c][i] = acc[c];
+      for (int k = 0; k < kItM; ++k) {
+        const int i = tid + k * kWG;
+        if (k < iters && i < nnz) {
+          const float v = half_to_f32(hv[k]);
+          const uint32_t w0 = h ? xq[k].z : xq[k].x, w1 = h ? xq[k].w : xq[k].y;
+          prod[0][i] = v * half_to_f32((uint16_t)(w0 & 0xffff));
+          prod[1][i] = v * half_to_f32((uint16_t)(w0 >> 16));
+          prod[2][i] = v * half_to_f32((uint16_t)(w1 & 0xffff));
+          prod[3][i] = v * half_to_f32((uint16_t)(w1 >> 16));
+        }
       }
     }
-  }
-  __syncthreads();
-  int q = 1;
-  while (q < 8 && nrows * kCols * q * 2 <= kWG) q <<= 1;
-  const int sub = tid & (q - 1);
-  for (int pr = tid / q; pr < nrows * kCols; pr += kWG / q) {
-    const int rr = pr >> 3, c = pr & 7;
-    float sum = 0.f;
-    for (int i = rptr[rr] + sub, e = rptr[rr + 1]; i < e; i += q) sum += prod[c][i];
-    for (int o = q >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-    if (sub == 0) Y[(int64_t)(r0 + rr) * kCols + c] = sum;
+    __syncthreads();
+    for (int pr = tid / q; pr < nrows * kHalf; pr += kWG / q) {
+      const int rr = pr >> 2, c = pr & 3;
+      float sum = 0.f;
+      for (int i = rptr[rr] + sub, e = rptr[rr + 1]; i < e; i += q) sum += prod[c][i];
+      for (int o = q >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (sub == 0) Y[(int64_t)(r0 + rr) * kCols + h * kHalf + c] = sum;
+    }
   }
 }
 
@@ -169,11 +179,11 @@ __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
 // run of tiles) runs tiles [wg_start[w], wg_start[w+1]) in order; the cuts
 // balance rows+entries plus a fixed cost per tile. No counters, no atomics:
 // every row of every matrix is written by exactly one tile.
-__global__ __launch_bounds__(kWG) void batch_spmv_f64(const BTile* __restrict__ tiles,
-                                                      const int32_t* __restrict__ wg_start,
-                                                      const BMat* __restrict__ mats,
-                                                      const double* __restrict__ x,
-                                                      double* __restrict__ y) {
+// (The SpMV kernel is held to 64 VGPRs, eight waves per SIMD; unconstrained
+// it takes 66 and seven.)
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) void batch_spmv_f64(
+    const BTile* __restrict__ tiles, const int32_t* __restrict__ wg_start,
+    const BMat* __restrict__ mats, const double* __restrict__ x, double* __restrict__ y) {
   const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
   const int t1 = wg_start[w + 1];
   for (int t = wg_start[w]; t < t1; ++t) {
@@ -431,21 +441,22 @@ bsp_status time_candidate(const std::vector<BTile>& one, bool spmm, int resident
   return BSP_OK;
 }
 
-// Tiles of every matrix for one operation: per-matrix windows (and, for the
-// SpMV, formats) when `tuned`, else the default window.
+// Tiles of every matrix for one operation: with uniform > 0 every matrix in
+// CSR with that window; otherwise the per-matrix windows (and, for the SpMV,
+// formats).
 bsp_status op_tiles(const bsp_batch_s* b, const std::vector<std::vector<int32_t>>& rp, bool spmm,
-                    bool tuned, std::vector<BTile>& out) {
+                    int uniform, std::vector<BTile>& out) {
   out.clear();
   for (size_t m = 0; m < b->A.size(); ++m) {
     if (b->A[m].nrows == 0) continue;
-    if (!spmm && tuned && b->fmt[m] == BSP_BATCH_BSR4) {
+    if (!spmm && uniform == 0 && b->fmt[m] == BSP_BATCH_BSR4) {
       bsp_bsr_view v;
       bsp_bsr_view_get(b->bsr[m], &v);
       std::vector<int32_t> brp(v.brows + 1);
       BSP_HIP(hipMemcpy(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost));
       bsr_tiles(brp, (int)m, out);
     } else {
-      const int c = spmm ? (tuned ? b->win_spmm[m] : kCapM) : (tuned ? b->win_spmv[m] : kCap);
+      const int c = uniform > 0 ? uniform : spmm ? b->win_spmm[m] : b->win_spmv[m];
       csr_tiles(rp[m], (int)m, c, out);
     }
   }
@@ -474,21 +485,26 @@ bsp_status time_schedule(const Schedule& S, bool spmm, const BMat* mats, const d
   return BSP_OK;
 }
 
-// Whole-operation choice between the default tiles and the per-matrix tuned
-// ones, each at launch sizes of 1, 4 and 16 resident waves and one workgroup
-// per tile (or at the caller's grid): the step itself decides, since the
-// per-matrix timings are cache-warm proxies. *tuned_out / *grid_out get the
-// fastest.
+// The step itself decides, since the per-matrix timings are cache-warm
+// proxies: each operation is timed over the whole batch with every uniform
+// window and with the per-matrix choice, each at launch sizes of 1, 4 and 16
+// resident waves and one workgroup per tile (or at the caller's grid).
+// *uniform_out gets the fastest window (0: the per-matrix choice), *grid_out
+// its launch size.
 bsp_status pick_schedule(const bsp_batch_s* b, const std::vector<std::vector<int32_t>>& rp,
                          bool spmm, int resident, int grid_opt, const double* x, double* y,
                          const uint16_t* X, float* Y, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                         bool* tuned_out, int* grid_out) {
+                         int* uniform_out, int* grid_out) {
   float best = 1e30f;
-  *tuned_out = false;
+  *uniform_out = spmm ? kCapM : kCap;
   *grid_out = grid_opt;
-  for (int tuned = 0; tuned < 2; ++tuned) {
+  std::vector<int> cands;
+  for (int c : kWindows)
+    if (c <= (spmm ? kCapM : kCap)) cands.push_back(c);
+  cands.push_back(0);
+  for (int u : cands) {
     std::vector<BTile> t;
-    BSP_TRY(op_tiles(b, rp, spmm, tuned == 1, t));
+    BSP_TRY(op_tiles(b, rp, spmm, u, t));
     std::vector<int> grids;
     if (grid_opt > 0)
       grids = {grid_opt};
@@ -503,7 +519,7 @@ bsp_status pick_schedule(const bsp_batch_s* b, const std::vector<std::vector<int
       BSP_TRY(time_schedule(S, spmm, b->mats.get(), x, y, X, Y, s, e0, e1, &ms));
       if (ms < best) {
         best = ms;
-        *tuned_out = tuned == 1;
+        *uniform_out = u;
         *grid_out = g;
       }
     }
@@ -655,34 +671,33 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       }
     }
     if (st == BSP_OK) st = upload_mats(b.get(), s);
-    bool tuned_v = true, tuned_m = true;
-    int grid_v = grid, grid_m = grid;
+    int uni_v = 0, uni_m = 0, grid_v = grid, grid_m = grid;
     if (st == BSP_OK)
       st = pick_schedule(b.get(), rp, false, b->spmv.resident, grid, x.get(), y.get(), X.get(),
-                         Y.get(), s, e0, e1, &tuned_v, &grid_v);
+                         Y.get(), s, e0, e1, &uni_v, &grid_v);
     if (st == BSP_OK)
       st = pick_schedule(b.get(), rp, true, b->spmm.resident, grid, x.get(), y.get(), X.get(),
-                         Y.get(), s, e0, e1, &tuned_m, &grid_m);
+                         Y.get(), s, e0, e1, &uni_m, &grid_m);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     BSP_TRY(st);
     b->spmv.grid_cap = grid_v;
     b->spmm.grid_cap = grid_m;
-    if (!tuned_v) {  // the default SpMV tiles won: CSR with the default window
+    if (uni_v > 0) {  // a uniform SpMV window won: every matrix in CSR with it
       for (size_t m = 0; m < M; ++m) {
         b->fmt[m] = BSP_BATCH_CSR;
-        b->win_spmv[m] = kCap;
+        b->win_spmv[m] = uni_v;
         if (b->bsr[m]) bsp_bsr_destroy(b->bsr[m]);
         b->bsr[m] = nullptr;
       }
     }
-    if (!tuned_m) std::fill(b->win_spmm.begin(), b->win_spmm.end(), kCapM);
+    if (uni_m > 0) std::fill(b->win_spmm.begin(), b->win_spmm.end(), uni_m);
   }
   BSP_TRY(upload_mats(b.get(), s));
 
   std::vector<BTile> tv, tm;
-  BSP_TRY(op_tiles(b.get(), rp, false, true, tv));
-  BSP_TRY(op_tiles(b.get(), rp, true, true, tm));
+  BSP_TRY(op_tiles(b.get(), rp, false, 0, tv));
+  BSP_TRY(op_tiles(b.get(), rp, true, 0, tm));
   BSP_TRY(b->spmv.upload(tv, s));
   BSP_TRY(b->spmm.upload(tm, s));
   *out = b.release();

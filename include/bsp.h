@@ -424,9 +424,9 @@ bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_st
  * rows+entries for the SpMV and {128, 256, 512} for the SpMM (which set rows
  * per workgroup and lanes per row) and, where 4x4 blocks are at least 11/16
  * full, the 4x4 BSR format for the SpMV, and keeps a candidate 10% faster
- * than the default; then, per operation, it times the whole batch with the
- * default and with the per-matrix tiles at several launch sizes and keeps
- * the fastest. tune = 0 uses windows of 1024 (SpMV) and 512 (SpMM); tune = 2
+ * than the default; then, per operation, it times the whole batch with each
+ * uniform window and with the per-matrix choice, at several launch sizes,
+ * and keeps the fastest. tune = 0 uses windows of 1024 (SpMV) and 512 (SpMM); tune = 2
  * puts every eligible matrix in 4x4 BSR without timing.
  * A captured graph holds one step; replaying it needs no host work. */
 typedef struct bsp_batch_s* bsp_batch;
