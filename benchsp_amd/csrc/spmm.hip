@@ -47,61 +47,43 @@ __device__ __forceinline__ float to_f32<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ float to_f32<uint16_t>(uint16_t v) { return bf16_to_f32(v); }
 
-// VEC consecutive features of a row of X starting at p, as floats.
-template <class T, int VEC>
-__device__ __forceinline__ void load_feats(const T* p, float (&f)[VEC]);
+// Four consecutive features of a row of X starting at p, as floats (one
+// 16-byte FP32 or 8-byte BF16 load).
+template <class T>
+__device__ __forceinline__ void load_feats4(const T* p, float (&f)[4]);
 template <>
-__device__ __forceinline__ void load_feats<float, 1>(const float* p, float (&f)[1]) { f[0] = *p; }
-template <>
-__device__ __forceinline__ void load_feats<float, 2>(const float* p, float (&f)[2]) {
-  const float2 v = *reinterpret_cast<const float2*>(p);
-  f[0] = v.x; f[1] = v.y;
-}
-template <>
-__device__ __forceinline__ void load_feats<float, 4>(const float* p, float (&f)[4]) {
+__device__ __forceinline__ void load_feats4<float>(const float* p, float (&f)[4]) {
   const float4 v = *reinterpret_cast<const float4*>(p);
   f[0] = v.x; f[1] = v.y; f[2] = v.z; f[3] = v.w;
 }
 template <>
-__device__ __forceinline__ void load_feats<uint16_t, 1>(const uint16_t* p, float (&f)[1]) {
-  f[0] = bf16_to_f32(*p);
-}
-template <>
-__device__ __forceinline__ void load_feats<uint16_t, 2>(const uint16_t* p, float (&f)[2]) {
-  const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
-  f[0] = __uint_as_float(v << 16); f[1] = __uint_as_float(v & 0xffff0000u);
-}
-template <>
-__device__ __forceinline__ void load_feats<uint16_t, 4>(const uint16_t* p, float (&f)[4]) {
+__device__ __forceinline__ void load_feats4<uint16_t>(const uint16_t* p, float (&f)[4]) {
   const uint2 v = *reinterpret_cast<const uint2*>(p);
   f[0] = __uint_as_float(v.x << 16); f[1] = __uint_as_float(v.x & 0xffff0000u);
   f[2] = __uint_as_float(v.y << 16); f[3] = __uint_as_float(v.y & 0xffff0000u);
 }
 
-__device__ __forceinline__ int rdlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ float rdlane_f(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-// Y[row, :] (or a partial) = sum over the chunk of a * X[col, :]; wave per
-// work item, lane l owns features [l*VEC, l*VEC+VEC) (F = 64*VEC). The
-// chunk's (col, a) pairs are loaded 64 at a time, one per lane, and
-// broadcast with readlane.
-template <class TV, class TX, int VEC, bool GATHER>
+// Y[row, :] (or a partial) = sum over the item's entries of a * X[col, :];
+// wave per work item. G = F/4 lanes share an entry, each owning four
+// features (one 16-byte FP32 or 8-byte BF16 load), so NZ = 64/G entries sit
+// side by side in a wave and U steps of them are in flight at once; the
+// item's (col, a) pairs are loaded 64 at a time, one per lane, and handed
+// out with lane shuffles. Lane group `sub` sums the entries e = sub (mod NZ)
+// in order; the NZ group sums are added with a fixed shuffle tree.
+template <class TV, class TX, int F, bool GATHER>
 __global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ idx,
                                                   const TV* __restrict__ val,
                                                   const int32_t* __restrict__ perm,
                                                   const TX* __restrict__ X, float* __restrict__ Y,
                                                   const int4* __restrict__ items, int64_t nitems,
                                                   float* __restrict__ partial) {
-  constexpr int F = kWave * VEC;
+  constexpr int G = F / 4, NZ = kWave / G, U = 4;
   const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (w >= nitems) return;
   const int4 it = items[w];
-  float acc[VEC];
-#pragma unroll
-  for (int v = 0; v < VEC; ++v) acc[v] = 0.f;
+  const int q = lane % G, sub = lane / G;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
   for (int base = it.y; base < it.z; base += kWave) {
     const int j = base + lane;
     int c = 0;
@@ -111,48 +93,29 @@ __global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ id
       a = to_f32<TV>(GATHER ? val[perm[j]] : ld_stream(&val[j]));
     }
     const int n = min(kWave, it.z - base);
-    int t = 0;
-    // eight rows of X in flight per wave, then four, then one
-    for (; t + 8 <= n; t += 8) {
-      float xr[8][VEC];
+    for (int t0 = 0; t0 < n; t0 += NZ * U) {
+      float xv[U][4], av[U];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        load_feats<TX, VEC>(X + (int64_t)rdlane_i(c, t + u) * F + lane * VEC, xr[u]);
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          s0 += rdlane_f(a, t + u) * xr[u][v];
-          s1 += rdlane_f(a, t + 4 + u) * xr[4 + u][v];
-        }
-        acc[v] += s0 + s1;
+      for (int u = 0; u < U; ++u) {
+        const int e = min(t0 + u * NZ + sub, kWave - 1);
+        av[u] = __shfl(a, e, 64);
+        load_feats4<TX>(X + (int64_t)__shfl(c, e, 64) * F + q * 4, xv[u]);
       }
-    }
-    for (; t + 4 <= n; t += 4) {
-      float x0[VEC], x1[VEC], x2[VEC], x3[VEC];
-      const int c0 = rdlane_i(c, t), c1 = rdlane_i(c, t + 1), c2 = rdlane_i(c, t + 2),
-                c3 = rdlane_i(c, t + 3);
-      load_feats<TX, VEC>(X + (int64_t)c0 * F + lane * VEC, x0);
-      load_feats<TX, VEC>(X + (int64_t)c1 * F + lane * VEC, x1);
-      load_feats<TX, VEC>(X + (int64_t)c2 * F + lane * VEC, x2);
-      load_feats<TX, VEC>(X + (int64_t)c3 * F + lane * VEC, x3);
-      const float a0 = rdlane_f(a, t), a1 = rdlane_f(a, t + 1), a2 = rdlane_f(a, t + 2),
-                  a3 = rdlane_f(a, t + 3);
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) acc[v] += a0 * x0[v] + a1 * x1[v] + a2 * x2[v] + a3 * x3[v];
-    }
-    for (; t < n; ++t) {
-      float x0[VEC];
-      load_feats<TX, VEC>(X + (int64_t)rdlane_i(c, t) * F + lane * VEC, x0);
-      const float a0 = rdlane_f(a, t);
+      for (int u = 0; u < U; ++u)
+        if (t0 + u * NZ + sub < n)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) acc[v] += a0 * x0[v];
+          for (int v = 0; v < 4; ++v) acc[v] += av[u] * xv[u][v];
     }
   }
-  float* dst = it.w < 0 ? Y + (int64_t)it.x * F : partial + (int64_t)it.w * F;
 #pragma unroll
-  for (int v = 0; v < VEC; ++v) dst[lane * VEC + v] = acc[v];
+  for (int o = G; o < kWave; o <<= 1)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] += __shfl_xor(acc[v], o, 64);
+  if (sub == 0) {
+    float* dst = it.w < 0 ? Y + (int64_t)it.x * F : partial + (int64_t)it.w * F;
+    *reinterpret_cast<float4*>(dst + q * 4) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  }
 }
 
 // Y[row, :] = sum of its chunk partials, in chunk order.
@@ -196,7 +159,7 @@ __global__ __launch_bounds__(kTB) void sddmm_items(const int32_t* __restrict__ c
   const int4 it = items[w];
   const int q = lane % G, sub = lane / G;
   float dy[4];
-  load_feats<TX, 4>(dY + (int64_t)it.x * F + q * 4, dy);
+  load_feats4<TX>(dY + (int64_t)it.x * F + q * 4, dy);
   // 64 column indices per wave load, then U steps of NZ entries with their X
   // rows all in flight before the dot products
   constexpr int U = 4;
@@ -208,7 +171,7 @@ __global__ __launch_bounds__(kTB) void sddmm_items(const int32_t* __restrict__ c
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int c = __shfl(cl, min(t0 + u * NZ + sub, kWave - 1), 64);
-        load_feats<TX, 4>(X + (int64_t)c * F + q * 4, xv[u]);
+        load_feats4<TX>(X + (int64_t)c * F + q * 4, xv[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -366,15 +329,15 @@ static bsp_status run_spmm_t(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd
   if (sd.nitems == 0) return BSP_OK;
   switch (F) {
     case 64:
-      spmm_items<TV, TX, 1, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
+      spmm_items<TV, TX, 64, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
                                                          sd.nitems, p->partial.get());
       break;
     case 128:
-      spmm_items<TV, TX, 2, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
+      spmm_items<TV, TX, 128, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
                                                          sd.nitems, p->partial.get());
       break;
     case 256:
-      spmm_items<TV, TX, 4, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
+      spmm_items<TV, TX, 256, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
                                                          sd.nitems, p->partial.get());
       break;
     default:
@@ -421,9 +384,12 @@ bsp_status bsp_spmm_plan_create(const bsp_csr* A, bsp_stream s, bsp_spmm_plan* o
   return BSP_OK;
 }
 
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 bsp_status bsp_spmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* X,
                     bsp_dtype xdtype, int64_t F, float* Y, bsp_stream s) {
   BSP_REQUIRE(p && (X || p->A.ncols == 0) && (Y || p->A.nrows == 0), "null argument");
+  BSP_REQUIRE(aligned16(X) && aligned16(Y), "X and Y must be 16-byte aligned");
   BSP_REQUIRE(val || p->A.nnz == 0, "null values");
   BSP_REQUIRE(xdtype == BSP_F32 || xdtype == BSP_BF16, "X must be FP32 or BF16");
   BSP_REQUIRE(vdtype == BSP_F32 || vdtype == BSP_BF16, "values must be FP32 or BF16");
@@ -433,6 +399,7 @@ bsp_status bsp_spmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const vo
 bsp_status bsp_spmm_t(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* dY,
                       bsp_dtype dtype, int64_t F, float* dX, bsp_stream s) {
   BSP_REQUIRE(p && (dY || p->A.nrows == 0) && (dX || p->A.ncols == 0), "null argument");
+  BSP_REQUIRE(aligned16(dY) && aligned16(dX), "dY and dX must be 16-byte aligned");
   BSP_REQUIRE(val || p->A.nnz == 0, "null values");
   BSP_REQUIRE(dtype == BSP_F32 || dtype == BSP_BF16, "dY must be FP32 or BF16");
   BSP_REQUIRE(vdtype == BSP_F32 || vdtype == BSP_BF16, "values must be FP32 or BF16");
@@ -442,6 +409,7 @@ bsp_status bsp_spmm_t(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const 
 bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype dtype, int64_t F,
                      float* dvals, bsp_stream s) {
   BSP_REQUIRE(p && dvals, "null argument");
+  BSP_REQUIRE(aligned16(dY) && aligned16(X), "dY and X must be 16-byte aligned");
   BSP_REQUIRE(dtype == BSP_F32 || dtype == BSP_BF16, "operands must be FP32 or BF16");
   const auto& sd = p->fwd;
   if (sd.nitems == 0 || p->A.nnz == 0) return BSP_OK;

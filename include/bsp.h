@@ -240,8 +240,9 @@ bsp_status bsp_dasp_destroy(bsp_dasp h);
  *   bsp_sddmm   dvals[j] = dY[row_j, :] . X[col_j, :]   (A's pattern, CSR order)
  * The plan (work lists, CSC permutation) is built on the GPU from the
  * pattern alone (A.val may be NULL); the values (nnz entries in CSR order,
- * FP32 or BF16) are passed to every call. Results are deterministic (no
- * atomics). */
+ * FP32 or BF16) are passed to every call. Dense operands and outputs must
+ * be 16-byte aligned (BSP_ERR_INVALID otherwise). Results are deterministic
+ * (no atomics). */
 typedef struct bsp_spmm_plan_s* bsp_spmm_plan;
 bsp_status bsp_spmm_plan_create(const bsp_csr* A, bsp_stream s, bsp_spmm_plan* out);
 bsp_status bsp_spmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* X,

@@ -114,3 +114,22 @@ def test_spmm_gnn_200k(oracle, bsp_host, dtype_name):
         assert rel_fro(dv[js], ref) <= tol
         # deterministic: a second run is bit-identical
         assert np.array_equal(P.spmm(tv, tX).cpu().numpy(), Y)
+
+
+def test_spmm_rejects_unaligned_operands(bsp_host):
+    # rows of four features are read with 16-byte loads: a dense operand or
+    # output that is not 16-byte aligned is an argument error
+    import torch
+    from benchsp_amd._lib import BspError
+    A = bsp_host.graph(500, 10.0, 2.1, True, seed=3)
+    P, tv, tX, tdY, _, _ = _setup(A, 64, torch.float32)
+    Xu = torch.zeros(A.ncols * 64 + 1, device="cuda")[1:].view(A.ncols, 64)
+    Xu.copy_(tX)
+    assert Xu.data_ptr() % 16 != 0
+    with pytest.raises(BspError):
+        P.spmm(tv, Xu)
+    with pytest.raises(BspError):
+        P.sddmm(tdY, Xu)
+    dYu = torch.zeros(A.nrows * 64 + 1, device="cuda")[1:].view(A.nrows, 64)
+    with pytest.raises(BspError):
+        P.spmm_t(tv, dYu)
