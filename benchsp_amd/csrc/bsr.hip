@@ -421,7 +421,11 @@ __global__ __launch_bounds__(kDbTB) void dbsr_chunk_fill(const int32_t* __restri
 }
 
 // Wave per 4-row group; lane = slot*4 + r, 16 blocks per step; each lane
-// gathers x through the per-slot column indices.
+// reads its row's four slots of a block with one 32-byte value load and one
+// 16-byte index load, and gathers x through the per-slot column indices.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef int int4v __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
     const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
@@ -431,10 +435,27 @@ __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
   if (g >= brows) return;
   const int slot = lane >> 2, r = lane & 3;
   double acc = 0.0;
-  for (int k = blcptr[g] + slot, e = blcptr[g + 1]; k < e; k += 16) {
+  auto add = [&](const dbl4& v, const int4v& c) {
+    acc += v.x * x[c.x];
+    acc += v.y * x[c.y];
+    acc += v.z * x[c.z];
+    acc += v.w * x[c.w];
+  };
+  int k = blcptr[g] + slot;
+  const int e = blcptr[g + 1];
+  for (; k + 16 < e; k += 32) {  // two steps' loads together, added in step order
+    const int64_t o0 = (int64_t)k * 16 + r * 4, o1 = o0 + 16 * 16;
+    const dbl4 v0 = __builtin_nontemporal_load(reinterpret_cast<const dbl4*>(bval + o0));
+    const int4v c0 = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(colidx + o0));
+    const dbl4 v1 = __builtin_nontemporal_load(reinterpret_cast<const dbl4*>(bval + o1));
+    const int4v c1 = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(colidx + o1));
+    add(v0, c0);
+    add(v1, c1);
+  }
+  if (k < e) {
     const int64_t o = (int64_t)k * 16 + r * 4;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc += ld_stream(&bval[o + c]) * x[ld_stream(&colidx[o + c])];
+    add(__builtin_nontemporal_load(reinterpret_cast<const dbl4*>(bval + o)),
+        __builtin_nontemporal_load(reinterpret_cast<const int4v*>(colidx + o)));
   }
   acc += __shfl_xor(acc, 4, 64);
   acc += __shfl_xor(acc, 8, 64);
