@@ -600,6 +600,11 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
     for (size_t m = 0; m < M; ++m) {
       const bsp_csr& A = b->A[m];
       if (A.nnz < 16 * 64) continue;
+      // block statistics first (key sort only): most matrices are screened
+      // out without building their BSR
+      bsp_blockinfo bi;
+      BSP_TRY(bsp_block_info(&A, 4, stream, &bi));
+      if (bi.avg < 11.0) continue;
       bsp_bsr h = nullptr;
       BSP_TRY(bsp_bsr_build(&A, 4, 4, stream, &h));
       bsp_bsr_view v;

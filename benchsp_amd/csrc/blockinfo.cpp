@@ -2,7 +2,8 @@
 // of b x b blocks holding at least one entry, and the number of blocks when
 // each row's entries are packed to the left (entry k of a row goes to block
 // column k / b), i.e. per group of b rows the longest row's ceil(len / b).
-// Both on the GPU; the first through the BSR builder.
+// Both on the GPU from the pattern alone; the first with the BSR builder's
+// key sort, no values placed.
 #include <hip/hip_runtime.h>
 
 #include "bsp_hip.hpp"
@@ -31,16 +32,17 @@ extern "C" bsp_status bsp_block_info(const bsp_csr* A, int block_len, bsp_stream
                                      bsp_blockinfo* out) {
   BSP_REQUIRE(A && out, "null argument");
   BSP_REQUIRE(block_len >= 1 && block_len <= 64, "block_len must be in [1, 64]");
+  BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
+  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
+      A->ncols >= (int64_t(1) << 31))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
+  BSP_REQUIRE(A->nnz == 0 || A->colidx, "null colidx");
   hipStream_t s = (hipStream_t)stream;
   bsp_blockinfo r{};
   r.nnz = A->nnz;
   if (A->nrows > 0 && A->nnz > 0) {
-    bsp_bsr h = nullptr;
-    BSP_TRY(bsp_bsr_build(A, block_len, block_len, stream, &h));
-    bsp_bsr_view v;
-    bsp_bsr_view_get(h, &v);
-    r.blocks = v.nblocks;
-    bsp_bsr_destroy(h);
+    BSP_TRY(bsr_count_blocks(*A, block_len, block_len, s, &r.blocks));
     const int64_t ng = (A->nrows + block_len - 1) / block_len;
     DevBuf<int64_t> per, sum;
     BSP_TRY(per.alloc(ng));

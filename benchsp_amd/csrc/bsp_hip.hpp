@@ -75,6 +75,10 @@ bsp_status sort_pairs_i64key(const int64_t* keys_in, int64_t* keys_out, const in
 // Sum of n int32 into *out (device).
 bsp_status reduce_sum_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s);
 
+// Number of occupied tm x tk blocks of a CSR (bsr.hip): the key sort of the
+// BSR builder without placing any value.
+bsp_status bsr_count_blocks(const bsp_csr& A, int tm, int tk, hipStream_t s, int64_t* nblocks);
+
 // Copies one value from device to host (synchronising the stream).
 template <class T>
 bsp_status fetch(const T* dptr, T* host, hipStream_t s) {

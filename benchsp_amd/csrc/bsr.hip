@@ -477,6 +477,36 @@ static int bits_for(int64_t v) {
   return b;
 }
 
+bsp_status bsr_count_blocks(const bsp_csr& A, int tm, int tk, hipStream_t s, int64_t* nblocks) {
+  const int64_t n = A.nrows, nnz = A.nnz;
+  *nblocks = 0;
+  if (n == 0 || nnz == 0) return BSP_OK;
+  const int64_t brows = cdiv(n, tm), bcols = cdiv(A.ncols, tk);
+  DevBuf<int32_t> rid, idx, sidx, flag, pos;
+  DevBuf<int64_t> key, skey;
+  BSP_TRY(rid.alloc(nnz));
+  BSP_TRY(idx.alloc(nnz));
+  BSP_TRY(sidx.alloc(nnz));
+  BSP_TRY(flag.alloc(nnz));
+  BSP_TRY(pos.alloc(nnz));
+  BSP_TRY(key.alloc(nnz));
+  BSP_TRY(skey.alloc(nnz));
+  row_ids<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, n, rid.get());
+  bsr_keys<<<cdiv(nnz, kTB), kTB, 0, s>>>(rid.get(), A.colidx, nnz, tm, tk, bcols, key.get(),
+                                          idx.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(sort_pairs_i64key(key.get(), skey.get(), idx.get(), sidx.get(), nnz,
+                            bits_for(brows * bcols), s));
+  key_flags<<<cdiv(nnz, kTB), kTB, 0, s>>>(skey.get(), nnz, flag.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(exclusive_scan_i32(flag.get(), pos.get(), nnz, s));
+  int32_t lp = 0, lf = 0;
+  BSP_TRY(fetch(pos.get() + nnz - 1, &lp, s));
+  BSP_TRY(fetch(flag.get() + nnz - 1, &lf, s));
+  *nblocks = (int64_t)lp + lf;
+  return BSP_OK;
+}
+
 static bsp_status build_bsr(const bsp_csr& A, int tm, int tk, hipStream_t s, bsp_bsr_s* h) {
   const int64_t n = A.nrows, nnz = A.nnz;
   bsp_bsr_view& v = h->v;

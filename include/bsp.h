@@ -305,7 +305,9 @@ bsp_status bsp_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32
 /* Replaces bench/blockinfo.cpp:96-119: b x b blocks holding an entry
  * (`blocks`, avg = nnz / blocks) and blocks after packing each row's entries
  * to the left (`blocks_packed` = sum over groups of b rows of the longest
- * row's ceil(len / b)). Computed on the GPU from a device CSR. */
+ * row's ceil(len / b)). Computed on the GPU from a device CSR's pattern
+ * (values are not read). The batch autotuner uses it to decide which
+ * matrices get a 4x4 BSR candidate. */
 typedef struct bsp_blockinfo {
   int64_t nnz, blocks, blocks_packed;
   double avg, avg_packed;
