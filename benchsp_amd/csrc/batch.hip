@@ -547,12 +547,9 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   int64_t total_nnz = 0, max_rows = 0, max_cols = 0;
   for (size_t m = 0; m < M; ++m) {
     const bsp_csr& A = b->A[m];
+    BSP_TRY(check_csr_pattern(&A));
     BSP_REQUIRE(A.dtype == BSP_F64, "batch matrices need FP64 values");
-    BSP_REQUIRE(A.nrows >= 0 && A.ncols >= 0 && A.nnz >= 0 && A.nnz < (int64_t(1) << 31) &&
-                    A.nrows < (int64_t(1) << 31) && A.ncols < (int64_t(1) << 31),
-                "batch matrix sizes must fit int32");
-    BSP_REQUIRE(A.nrows == 0 || A.rowptr, "null rowptr");
-    BSP_REQUIRE(A.nnz == 0 || (A.colidx && A.val), "null colidx/val");
+    BSP_REQUIRE(A.nnz == 0 || A.val, "null values");
     b->row_off[m + 1] = b->row_off[m] + A.nrows;
     b->col_off[m + 1] = b->col_off[m] + A.ncols;
     total_nnz += A.nnz;

@@ -32,12 +32,7 @@ extern "C" bsp_status bsp_block_info(const bsp_csr* A, int block_len, bsp_stream
                                      bsp_blockinfo* out) {
   BSP_REQUIRE(A && out, "null argument");
   BSP_REQUIRE(block_len >= 1 && block_len <= 64, "block_len must be in [1, 64]");
-  BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
-  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
-      A->ncols >= (int64_t(1) << 31))
-    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
-  BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
-  BSP_REQUIRE(A->nnz == 0 || A->colidx, "null colidx");
+  BSP_TRY(check_csr_pattern(A));
   hipStream_t s = (hipStream_t)stream;
   bsp_blockinfo r{};
   r.nnz = A->nnz;

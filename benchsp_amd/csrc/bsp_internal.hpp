@@ -27,3 +27,20 @@ bsp_status fail(bsp_status s, const std::string& msg);
   do {                                                             \
     if (!(cond)) return ::bsp::fail(BSP_ERR_INVALID, (msg));       \
   } while (0)
+
+namespace bsp {
+
+// Sizes and index pointers of a CSR descriptor (values not checked): sizes
+// must be non-negative and fit int32 indices (BSP_ERR_TOO_LARGE otherwise).
+inline bsp_status check_csr_pattern(const bsp_csr* A) {
+  BSP_REQUIRE(A, "null CSR descriptor");
+  BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
+  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
+      A->ncols >= (int64_t(1) << 31))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
+  BSP_REQUIRE(A->nnz == 0 || A->colidx, "null colidx");
+  return BSP_OK;
+}
+
+}  // namespace bsp

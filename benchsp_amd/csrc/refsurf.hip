@@ -204,6 +204,7 @@ extern "C" {
 bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
                                      bsp_rocsparse_plan* out) {
   BSP_REQUIRE(A && out, "null argument");
+  BSP_TRY(check_csr_pattern(A));
   BSP_REQUIRE(A->dtype == BSP_F64 || A->dtype == BSP_F32 || A->dtype == BSP_F16,
               "rocSPARSE comparator takes FP64, FP32 or FP16 values");
   auto p = std::make_unique<bsp_rocsparse_plan_s>();
@@ -273,6 +274,7 @@ extern "C" {
 bsp_status bsp_rocsparse_spmm_plan_create(const bsp_csr* A, int64_t N, int transpose, int alg,
                                           bsp_stream s, bsp_rocsparse_spmm_plan* out) {
   BSP_REQUIRE(A && out && N > 0, "bad argument");
+  BSP_TRY(check_csr_pattern(A));
   BSP_REQUIRE(A->dtype == BSP_F32 || A->dtype == BSP_F16 || A->dtype == BSP_BF16,
               "rocSPARSE SpMM comparator takes FP32, FP16 or BF16 values");
   auto p = std::make_unique<bsp_rocsparse_spmm_plan_s>();

@@ -142,3 +142,39 @@ def test_compat_header_driver_builds(tmp_path):
                         b.COMPAT_SRC, "-L", b.HERE, "-lbsp", "-o", str(out)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+@pytest.mark.parametrize("which", ["bsr", "dbsr", "dasp", "spmm", "blockinfo", "rocsparse",
+                                   "rocsparse_spmm", "batch"])
+@pytest.mark.parametrize("size", ["nnz", "nrows", "ncols", "negative"])
+def test_builders_reject_bad_sizes_before_any_gpu_work(which, size):
+    # every builder validates the descriptor first: sizes past int32 indices
+    # give BSP_ERR_TOO_LARGE (6), negative sizes BSP_ERR_INVALID (1), and no
+    # device call is made (this runs without a GPU)
+    from benchsp_amd._lib import BlockInfo, BspError, CsrDesc, lib
+    from benchsp_amd.device import BatchOptions
+    big = 1 << 31
+    n, m, z = {"nnz": (10, 10, big), "nrows": (big, 10, 5), "ncols": (10, big, 5),
+               "negative": (10, -1, 5)}[size]
+    if which == "dasp" and size == "ncols":
+        n, m, z = 10, 10, big  # DASP indexes rows and entries only
+    d = CsrDesc(n, m, z, None, None, None, 0)  # FP64 values
+    if which in ("rocsparse_spmm",):
+        d = CsrDesc(n, m, z, None, None, None, 1)  # FP32 values
+    h = C.c_void_p()
+    L = lib()
+    calls = {
+        "bsr": lambda: L.bsp_bsr_build(C.byref(d), 4, 4, None, C.byref(h)),
+        "dbsr": lambda: L.bsp_dbsr_build(C.byref(d), None, C.byref(h)),
+        "dasp": lambda: L.bsp_dasp_build(C.byref(d), 0.75, 256, None, C.byref(h)),
+        "spmm": lambda: L.bsp_spmm_plan_create(C.byref(d), None, C.byref(h)),
+        "blockinfo": lambda: L.bsp_block_info(C.byref(d), 4, None, C.byref(BlockInfo())),
+        "rocsparse": lambda: L.bsp_rocsparse_plan_create(C.byref(d), 0, None, C.byref(h)),
+        "rocsparse_spmm": lambda: L.bsp_rocsparse_spmm_plan_create(C.byref(d), 64, 0, 0, None,
+                                                                    C.byref(h)),
+        "batch": lambda: L.bsp_batch_create(C.byref(d), 1, C.byref(BatchOptions(0, 0)), None,
+                                            C.byref(h)),
+    }
+    with pytest.raises(BspError) as e:
+        calls[which]()
+    assert e.value.status == (1 if size == "negative" else 6), (which, size, str(e.value))
