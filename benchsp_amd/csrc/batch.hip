@@ -445,7 +445,7 @@ bsp_status time_candidate(const std::vector<BTile>& one, bool spmm, int resident
 // CSR with that window; otherwise the per-matrix windows (and, for the SpMV,
 // formats).
 bsp_status op_tiles(const bsp_batch_s* b, const std::vector<std::vector<int32_t>>& rp, bool spmm,
-                    int uniform, std::vector<BTile>& out) {
+                    int uniform, hipStream_t s, std::vector<BTile>& out) {
   out.clear();
   for (size_t m = 0; m < b->A.size(); ++m) {
     if (b->A[m].nrows == 0) continue;
@@ -453,7 +453,9 @@ bsp_status op_tiles(const bsp_batch_s* b, const std::vector<std::vector<int32_t>
       bsp_bsr_view v;
       bsp_bsr_view_get(b->bsr[m], &v);
       std::vector<int32_t> brp(v.brows + 1);
-      BSP_HIP(hipMemcpy(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost));
+      BSP_HIP(hipMemcpyAsync(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(),
+                             hipMemcpyDeviceToHost, s));
+      BSP_HIP(hipStreamSynchronize(s));
       bsr_tiles(brp, (int)m, out);
     } else {
       const int c = uniform > 0 ? uniform : spmm ? b->win_spmm[m] : b->win_spmv[m];
@@ -504,7 +506,7 @@ bsp_status pick_schedule(const bsp_batch_s* b, const std::vector<std::vector<int
   cands.push_back(0);
   for (int u : cands) {
     std::vector<BTile> t;
-    BSP_TRY(op_tiles(b, rp, spmm, u, t));
+    BSP_TRY(op_tiles(b, rp, spmm, u, s, t));
     std::vector<int> grids;
     if (grid_opt > 0)
       grids = {grid_opt};
@@ -626,9 +628,16 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
     BSP_TRY(Y.alloc(b->row_off[M] * kCols));
     BSP_HIP(hipMemsetAsync(x.get(), 0, sizeof(double) * std::max<int64_t>(1, b->col_off[M]), s));
     BSP_HIP(hipMemsetAsync(X.get(), 0, sizeof(uint16_t) * std::max<int64_t>(1, b->col_off[M] * kCols), s));
-    hipEvent_t e0, e1;
-    BSP_HIP(hipEventCreate(&e0));
-    BSP_HIP(hipEventCreate(&e1));
+    struct Events {  // destroyed on every exit path
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      ~Events() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+      }
+    } ev;
+    BSP_HIP(hipEventCreate(&ev.e0));
+    BSP_HIP(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     bsp_status st = BSP_OK;
     for (size_t m = 0; m < M && st == BSP_OK; ++m) {
       if (b->A[m].nrows == 0) continue;
@@ -660,7 +669,9 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         bsp_bsr_view v;
         bsp_bsr_view_get(b->bsr[m], &v);
         std::vector<int32_t> brp(v.brows + 1);
-        BSP_HIP(hipMemcpy(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost));
+        BSP_HIP(hipMemcpyAsync(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(),
+                               hipMemcpyDeviceToHost, s));
+        BSP_HIP(hipStreamSynchronize(s));
         std::vector<BTile> one;
         bsr_tiles(brp, (int)m, one);
         float t = 0;
@@ -680,8 +691,6 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
     if (st == BSP_OK)
       st = pick_schedule(b.get(), rp, true, b->spmm.resident, grid, x.get(), y.get(), X.get(),
                          Y.get(), s, e0, e1, &uni_m, &grid_m);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     BSP_TRY(st);
     b->spmv.grid_cap = grid_v;
     b->spmm.grid_cap = grid_m;
@@ -698,8 +707,8 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   BSP_TRY(upload_mats(b.get(), s));
 
   std::vector<BTile> tv, tm;
-  BSP_TRY(op_tiles(b.get(), rp, false, 0, tv));
-  BSP_TRY(op_tiles(b.get(), rp, true, 0, tm));
+  BSP_TRY(op_tiles(b.get(), rp, false, 0, s, tv));
+  BSP_TRY(op_tiles(b.get(), rp, true, 0, s, tm));
   BSP_TRY(b->spmv.upload(tv, s));
   BSP_TRY(b->spmm.upload(tm, s));
   *out = b.release();
