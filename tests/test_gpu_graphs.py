@@ -53,3 +53,73 @@ def test_spmv_plans_replay_in_graphs(oracle, bsp_host):
                               abs_spmv(oracle, A, xh), what=f"{name} replay {rep}")
             _ = (x * 2).sum().item()  # eager work between replays
         del g
+
+
+def test_spmm_and_block_spmm_replay_in_graphs(bsp_host):
+    # deterministic kernels: every replay equals the eager call on the same
+    # operands bit for bit (after one eager call has sized the workspaces)
+    import torch
+    from benchsp_amd.autograd import SparsePattern
+    from benchsp_amd.device import BlockSpmm, Bsr, DeviceCsr
+    rng = np.random.default_rng(4)
+    A = bsp_host.graph(20_000, 30.0, 2.1, False, seed=9)
+    P = SparsePattern.from_host(A)
+    for dt in (torch.float32, torch.bfloat16):
+        print(f"[graph] spmm {dt}", flush=True)
+        vals = torch.from_numpy(A.val.astype(np.float32)).cuda().to(dt)
+        X = torch.zeros(A.ncols, 64, device="cuda", dtype=dt)
+        dY = torch.zeros(A.nrows, 64, device="cuda", dtype=dt)
+        outs = {}
+
+        def step():
+            outs["Y"] = P.spmm(vals, X)
+            outs["dX"] = P.spmm_t(vals, dY)
+            outs["dv"] = P.sddmm(dY, X)
+        step()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            step()
+        static = dict(outs)
+        for rep in range(3):
+            X.copy_(torch.from_numpy(rng.normal(size=(A.ncols, 64)).astype(np.float32)).to(dt))
+            dY.copy_(torch.from_numpy(rng.normal(size=(A.nrows, 64)).astype(np.float32)).to(dt))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            got = {k: v.clone() for k, v in static.items()}
+            step()  # eager, same operands
+            torch.cuda.synchronize()
+            for k in got:
+                assert torch.equal(got[k], outs[k]), (str(dt), k, rep)
+        del g
+    B16 = bsp_host.blocksparse(64, 48, 16, 700, seed=2)
+    bsr = Bsr(DeviceCsr.from_host(B16), 16, 16)
+    for half in (torch.float16, torch.bfloat16):
+        print(f"[graph] block spmm {half}", flush=True)
+        S = BlockSpmm(bsr, half)
+        Bm = torch.zeros(B16.ncols, 128, device="cuda", dtype=half)
+        dC = torch.zeros(B16.nrows, 128, device="cuda", dtype=half)
+        outs = {}
+
+        def bstep():
+            outs["C"] = S.fwd(Bm)
+            outs["dB"], outs["dv"] = S.bwd(Bm, dC)
+        bstep()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            bstep()
+        static = dict(outs)
+        for rep in range(3):
+            Bm.copy_(torch.from_numpy(rng.normal(size=(B16.ncols, 128)).astype(np.float32)).to(half))
+            dC.copy_(torch.from_numpy(rng.normal(size=(B16.nrows, 128)).astype(np.float32)).to(half))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            got = {k: v.clone() for k, v in static.items()}
+            bstep()
+            torch.cuda.synchronize()
+            for k in got:
+                assert torch.equal(got[k], outs[k]), (str(half), k, rep)
+        del g
