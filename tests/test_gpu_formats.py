@@ -27,6 +27,15 @@ def _mats(bsp_host):
     yield "stencil27-64", bsp_host.stencil27_64()
     yield "powerlaw-100k", bsp_host.powerlaw(100_000, 20.0, 1.2, 2000, seed=4)
     yield "empty-rows", _rand_csr(rng, [0] * 7 + [3] + [0] * 9, 50)
+    yield from _degenerate(rng)
+
+
+def _degenerate(rng):
+    # no entries at all; one row; one column; rows not a multiple of any block
+    yield "all-empty", Csr(37, 29, np.zeros(38, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    yield "one-row", _rand_csr(rng, [50], 100)
+    yield "one-col", _rand_csr(rng, rng.integers(0, 2, 203), 1)
+    yield "ragged-5x7", _rand_csr(rng, [3, 0, 7, 1, 2], 7)
 
 
 def _same(a, b, what):
@@ -82,6 +91,7 @@ def _dbsr_inputs(bsp_host):
                                        g["rowptr"].astype(np.int64), g["col"], g["val"])
     rng = np.random.default_rng(0)
     yield "overflow-case", _rand_csr(rng, [0, 1, 2, 3, 4, 5, 9, 16, 17, 33, 70, 0, 2, 64, 65, 8, 1], 90)
+    yield from _degenerate(rng)
     # ~4,600 block rows (5 chunks of the parallel warp packing) whose block
     # counts sit around the 64-block warp capacity, below it and far above it
     kind = rng.choice(3, 18_500, p=[0.6, 0.3, 0.1])
@@ -131,6 +141,7 @@ def _dasp_inputs(bsp_host):
     yield "stencil27-64", bsp_host.stencil27_64()
     yield "powerlaw-1M", bsp_host.powerlaw_1m()
     yield "poisson5-256", bsp_host.poisson5_256()
+    yield from _degenerate(rng)
 
 
 DASP_KEYS = ["order_rid", "short_val", "short_cid", "long_rpt", "long_val", "long_cid",
