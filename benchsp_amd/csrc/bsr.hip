@@ -17,6 +17,7 @@ struct bsp_dbsr_s {
   bsp::DevBuf<int32_t> recorder, blcptr, colidx, prefix;
   bsp::DevBuf<double> val;
   bsp::DevBuf<uint16_t> map;
+  int64_t narrow = 0;  // 4-row groups at most 8 blocks wide (a prefix: rows sorted by length)
 };
 
 namespace bsp {
@@ -420,20 +421,23 @@ __global__ __launch_bounds__(kDbTB) void dbsr_chunk_fill(const int32_t* __restri
   if (run) atomicAdd(cnt + w, run);
 }
 
-// Wave per 4-row group; lane = slot*4 + r, 16 blocks per step; each lane
-// reads its row's four slots of a block with one 32-byte value load and one
-// 16-byte index load, and gathers x through the per-slot column indices.
+// Lane = slot*4 + r; each lane reads its row's four slots of a block with
+// one 32-byte value load and one 16-byte index load and gathers x through
+// the per-slot column indices. Rows are sorted by length, so the groups at
+// most 8 blocks wide come first (`narrow` of them, counted when the handle is
+// made): there a wave serves two groups, each on half the wave with 8 blocks
+// per step (a pair with a wider group runs its two groups one after the
+// other); every later group has a wave of its own, 16 blocks per step.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef int int4v __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
-    const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
-    const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
-    double* __restrict__ y, int64_t brows, int64_t nrows, int sorted_order) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (g >= brows) return;
-  const int slot = lane >> 2, r = lane & 3;
+// Partial sum of lane l (l < 4*SL) for group g; blocks added in step order.
+template <int SL>
+__device__ __forceinline__ double dbsr_group(const int32_t* __restrict__ blcptr,
+                                             const int32_t* __restrict__ colidx,
+                                             const double* __restrict__ bval,
+                                             const double* __restrict__ x, int64_t g, int l) {
+  const int slot = l >> 2, r = l & 3;
   double acc = 0.0;
   auto add = [&](const dbl4& v, const int4v& c) {
     acc += v.x * x[c.x];
@@ -443,8 +447,8 @@ __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
   };
   int k = blcptr[g] + slot;
   const int e = blcptr[g + 1];
-  for (; k + 16 < e; k += 32) {  // two steps' loads together, added in step order
-    const int64_t o0 = (int64_t)k * 16 + r * 4, o1 = o0 + 16 * 16;
+  for (; k + SL < e; k += 2 * SL) {  // two steps' loads together
+    const int64_t o0 = (int64_t)k * 16 + r * 4, o1 = o0 + SL * 16;
     const dbl4 v0 = __builtin_nontemporal_load(reinterpret_cast<const dbl4*>(bval + o0));
     const int4v c0 = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(colidx + o0));
     const dbl4 v1 = __builtin_nontemporal_load(reinterpret_cast<const dbl4*>(bval + o1));
@@ -457,17 +461,76 @@ __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
     add(__builtin_nontemporal_load(reinterpret_cast<const dbl4*>(bval + o)),
         __builtin_nontemporal_load(reinterpret_cast<const int4v*>(colidx + o)));
   }
-  acc += __shfl_xor(acc, 4, 64);
-  acc += __shfl_xor(acc, 8, 64);
-  acc += __shfl_xor(acc, 16, 64);
-  acc += __shfl_xor(acc, 32, 64);
-  if (lane < 4) {
-    const int64_t i = 4 * g + lane;
-    if (i < nrows) y[sorted_order ? i : rec[i]] = acc;
+  return acc;
+}
+
+__global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
+    const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
+    double* __restrict__ y, int64_t brows, int64_t nrows, int64_t narrow, int sorted_order) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t npair = (narrow + 1) / 2;
+  auto store = [&](int64_t g, int r, double v) {
+    const int64_t i = 4 * g + r;
+    if (i < nrows) y[sorted_order ? i : rec[i]] = v;
+  };
+  if (w >= npair) {  // a wide group, one wave
+    const int64_t g = narrow + (w - npair);
+    if (g >= brows) return;
+    double acc = dbsr_group<16>(blcptr, colidx, bval, x, g, lane);
+    acc += __shfl_xor(acc, 4, 64);
+    acc += __shfl_xor(acc, 8, 64);
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    if (lane < 4) store(g, lane, acc);
+    return;
+  }
+  const int64_t g0 = 2 * w;
+  const bool pair = g0 + 1 < narrow && blcptr[g0 + 1] - blcptr[g0] <= 8 &&
+                    blcptr[g0 + 2] - blcptr[g0 + 1] <= 8;
+  if (pair) {
+    const int h = lane >> 5, l = lane & 31;
+    double acc = dbsr_group<8>(blcptr, colidx, bval, x, g0 + h, l);
+    acc += __shfl_xor(acc, 4, 64);
+    acc += __shfl_xor(acc, 8, 64);
+    acc += __shfl_xor(acc, 16, 64);
+    if (l < 4) store(g0 + h, l, acc);
+    return;
+  }
+  for (int64_t g = g0; g < g0 + 2 && g < narrow; ++g) {
+    double acc = dbsr_group<16>(blcptr, colidx, bval, x, g, lane);
+    acc += __shfl_xor(acc, 4, 64);
+    acc += __shfl_xor(acc, 8, 64);
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    if (lane < 4) store(g, lane, acc);
   }
 }
 
+// f[g] = 1 when 4-row group g is at most 8 blocks wide
+__global__ void dbsr_narrow_flags(const int32_t* __restrict__ blcptr, int64_t brows,
+                                  int64_t* __restrict__ f) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < brows) f[g] = blcptr[g + 1] - blcptr[g] <= 8;
+}
+
 }  // namespace
+
+// Groups of a DBSR handle that take the two-groups-per-wave path.
+static bsp_status dbsr_count_narrow(bsp_dbsr_s* h, hipStream_t s) {
+  const int64_t brows = h->v.brows;
+  h->narrow = 0;
+  if (brows == 0) return BSP_OK;
+  DevBuf<int64_t> f, sum;
+  BSP_TRY(f.alloc(brows));
+  BSP_TRY(sum.alloc(1));
+  dbsr_narrow_flags<<<cdiv(brows, kTB), kTB, 0, s>>>(h->blcptr.get(), brows, f.get());
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(reduce_sum_i64(f.get(), sum.get(), brows, s));
+  BSP_TRY(fetch(sum.get(), &h->narrow, s));
+  return BSP_OK;
+}
 
 // ---- builders -----------------------------------------------------------------
 
@@ -670,6 +733,7 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
   v.val = h->val.get();
   v.map = h->map.get();
   v.prefix = h->prefix.get();
+  BSP_TRY(dbsr_count_narrow(h, s));
   return BSP_OK;
 }
 
@@ -757,8 +821,9 @@ bsp_status bsp_dbsr_spmv(bsp_dbsr h, const double* x, double* y, int sorted_orde
   const bsp_dbsr_view& v = h->v;
   BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
   if (v.brows == 0) return BSP_OK;
-  dbsr_spmv_f64<<<cdiv(v.brows * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
-      v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, sorted_order);
+  const int64_t waves = (h->narrow + 1) / 2 + (v.brows - h->narrow);
+  dbsr_spmv_f64<<<cdiv(waves * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
+      v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, h->narrow, sorted_order);
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }
@@ -814,6 +879,7 @@ bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out)
   v.val = h->val.get();
   v.map = h->map.get();
   v.prefix = h->prefix.get();
+  BSP_TRY(dbsr_count_narrow(h.get(), st));
   *out = h.release();
   return BSP_OK;
 }
