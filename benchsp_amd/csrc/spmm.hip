@@ -11,6 +11,8 @@
 // plan); values are read through that permutation, never copied.
 #include <memory>
 
+#include <type_traits>
+
 #include "bsp_hip.hpp"
 
 struct bsp_spmm_plan_s {
@@ -415,22 +417,24 @@ bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype d
   if (sd.nitems == 0 || p->A.nnz == 0) return BSP_OK;
   const int64_t grid = cdiv(sd.nitems * kWave, kTB);
   hipStream_t st = (hipStream_t)s;
-  if (F == 64 && dtype == BSP_F32)
-    sddmm_items<float, 64><<<grid, kTB, 0, st>>>(p->A.colidx, (const float*)dY, (const float*)X,
-                                                 sd.items.get(), sd.nitems, dvals);
-  else if (F == 128 && dtype == BSP_F32)
-    sddmm_items<float, 128><<<grid, kTB, 0, st>>>(p->A.colidx, (const float*)dY, (const float*)X,
-                                                  sd.items.get(), sd.nitems, dvals);
-  else if (F == 64)
-    sddmm_items<uint16_t, 64><<<grid, kTB, 0, st>>>(p->A.colidx, (const uint16_t*)dY,
-                                                    (const uint16_t*)X, sd.items.get(), sd.nitems,
-                                                    dvals);
-  else if (F == 128)
-    sddmm_items<uint16_t, 128><<<grid, kTB, 0, st>>>(p->A.colidx, (const uint16_t*)dY,
-                                                     (const uint16_t*)X, sd.items.get(), sd.nitems,
-                                                     dvals);
-  else
-    return fail(BSP_ERR_UNSUPPORTED, "SDDMM feature width must be 64 or 128");
+  auto launch = [&](auto tag, auto fw) -> bsp_status {
+    using T = decltype(tag);
+    constexpr int FW = decltype(fw)::value;
+    sddmm_items<T, FW><<<grid, kTB, 0, st>>>(p->A.colidx, static_cast<const T*>(dY),
+                                             static_cast<const T*>(X), sd.items.get(), sd.nitems,
+                                             dvals);
+    return BSP_OK;
+  };
+  using F64c = std::integral_constant<int, 64>;
+  using F128c = std::integral_constant<int, 128>;
+  using F256c = std::integral_constant<int, 256>;
+  const bool bf = dtype == BSP_BF16;
+  switch (F) {
+    case 64: bf ? launch(uint16_t{}, F64c{}) : launch(float{}, F64c{}); break;
+    case 128: bf ? launch(uint16_t{}, F128c{}) : launch(float{}, F128c{}); break;
+    case 256: bf ? launch(uint16_t{}, F256c{}) : launch(float{}, F256c{}); break;
+    default: return fail(BSP_ERR_UNSUPPORTED, "SDDMM feature width must be 64, 128 or 256");
+  }
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

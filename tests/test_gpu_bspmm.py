@@ -82,7 +82,7 @@ def _quantized_dense(A, g, o, half):
 
 
 @pytest.mark.parametrize("half", ["float16", "bfloat16"])
-@pytest.mark.parametrize("N", [32, 128, 256])
+@pytest.mark.parametrize("N", [32, 64, 128, 256])
 @pytest.mark.parametrize("shape", ["blocks", "ragged"])
 def test_fwd_bwd_small(oracle, bsp_host, half, N, shape):
     import torch

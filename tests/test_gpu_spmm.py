@@ -31,7 +31,7 @@ def _setup(A, F, dtype, seed=0):
 
 
 @pytest.mark.parametrize("dtype_name", ["float32", "bfloat16"])
-@pytest.mark.parametrize("F", [64, 128])
+@pytest.mark.parametrize("F", [64, 128, 256])
 @pytest.mark.parametrize("symmetric", [True, False])
 def test_spmm_fwd_bwd_small(oracle, bsp_host, dtype_name, F, symmetric):
     import torch
