@@ -38,7 +38,7 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
 
 // Rows [r0, r1) holding entries [p0, p1): at most CHUNK rows and at most
 // MAXNNZ entries, or one row (of any length).
-template <int CHUNK, bool ACCUM>
+template <int CHUNK, bool ACCUM, bool NT_Y = true>
 __device__ __forceinline__ void adaptive_rows(const int32_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ col,
                                               const double* __restrict__ val,
@@ -114,10 +114,13 @@ __device__ __forceinline__ void adaptive_rows(const int32_t* __restrict__ rowptr
     for (int i = rs + l; i < re; i += lpr) s += prod[i];
     s = group_sum_dyn(s, lpr);
     if (l == 0) {
-      if (!ACCUM)
-        st_stream(&y[r0 + rr], s);
-      else if (re > rs)
-        st_stream(&y[r0 + rr], yb[rr] + s);
+      const double out = ACCUM ? yb[rr] + s : s;
+      if (!ACCUM || re > rs) {
+        if (NT_Y)
+          st_stream(&y[r0 + rr], out);
+        else
+          y[r0 + rr] = out;
+      }
     }
   }
 }
@@ -125,14 +128,14 @@ __device__ __forceinline__ void adaptive_rows(const int32_t* __restrict__ rowptr
 // One block per workgroup. The block's first row and first entry come from
 // one 8-byte load each ({row, entry offset} pairs built with the plan), so a
 // workgroup's stream loads start after a single round trip.
-template <int CHUNK, bool ACCUM>
+template <int CHUNK, bool ACCUM, bool NT_Y = true>
 __device__ __forceinline__ void adaptive_block(const int32_t* __restrict__ rowptr,
                                                const int32_t* __restrict__ col,
                                                const double* __restrict__ val,
                                                const double* __restrict__ x, double* __restrict__ y,
                                                const int2* __restrict__ blk, int64_t b) {
   const int2 s0 = blk[b], s1 = blk[b + 1];
-  adaptive_rows<CHUNK, ACCUM>(rowptr, col, val, x, y, s0.x, s0.y, s1.x, s1.y);
+  adaptive_rows<CHUNK, ACCUM, NT_Y>(rowptr, col, val, x, y, s0.x, s0.y, s1.x, s1.y);
 }
 
 }  // namespace bsp

@@ -69,7 +69,8 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_xcd_f64(const SegDesc* __res
   const SegDesc d = segs[s];
   const int64_t b = local * reps + q;
   if (b >= d.nblk) return;
-  adaptive_block<CHUNK, false>(d.rowptr, d.col, d.val, x, d.out, d.blk, b);
+  // partial sums stay cached (sum_partials reads them back at once)
+  adaptive_block<CHUNK, false, false>(d.rowptr, d.col, d.val, x, d.out, d.blk, b);
 }
 
 __global__ void sum_partials(const double* __restrict__ part, int nseg, int64_t n,
