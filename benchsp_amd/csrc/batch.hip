@@ -345,10 +345,18 @@ struct bsp_batch_s {
   hipStream_t cap = nullptr;  // capture stream
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // The SpMM of a step runs beside the SpMV: forked onto `side` after an
+  // event on the caller's stream and joined back with another (default
+  // priority, like every stream here).
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   ~bsp_batch_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap) (void)hipStreamDestroy(cap);
+    if (side) (void)hipStreamDestroy(side);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
     for (bsp_bsr h : bsr)
       if (h) bsp_bsr_destroy(h);
   }
@@ -564,6 +572,9 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   b->win_spmv.assign(M, kCap);
   b->win_spmm.assign(M, kCapM);
   b->bsr.assign(M, nullptr);
+  BSP_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  BSP_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  BSP_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
   BSP_TRY(resident_grid(batch_spmv_f64, &b->spmv.resident));
   BSP_TRY(resident_grid(batch_spmm8_f16, &b->spmm.resident));
   b->spmv.grid_cap = b->spmm.grid_cap = grid;
@@ -746,8 +757,17 @@ bsp_status bsp_batch_run(bsp_batch b, int schedule, const double* x, double* y, 
               "x/y and X/Y come in pairs");
   hipStream_t s = (hipStream_t)stream;
   if (schedule == BSP_BATCH_PERSISTENT) {
-    if (x) launch_spmv(b->spmv, b->mats.get(), x, y, s);
-    if (X) launch_spmm(b->spmm, b->mats.get(), X, Y, s);
+    if (x && X) {  // independent operations: the SpMM on the side stream
+      BSP_HIP(hipEventRecord(b->fork, s));
+      BSP_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
+      launch_spmm(b->spmm, b->mats.get(), X, Y, b->side);
+      launch_spmv(b->spmv, b->mats.get(), x, y, s);
+      BSP_HIP(hipEventRecord(b->join, b->side));
+      BSP_HIP(hipStreamWaitEvent(s, b->join, 0));
+    } else {
+      if (x) launch_spmv(b->spmv, b->mats.get(), x, y, s);
+      if (X) launch_spmm(b->spmm, b->mats.get(), X, Y, s);
+    }
   } else if (schedule == BSP_BATCH_PLAIN) {
     int64_t hoff = 0;
     for (size_t m = 0; m < b->A.size(); ++m) {

@@ -431,6 +431,8 @@ bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_st
  * uniform window and with the per-matrix choice, at several launch sizes,
  * and keeps the fastest. tune = 0 uses windows of 1024 (SpMV) and 512 (SpMM); tune = 2
  * puts every eligible matrix in 4x4 BSR without timing.
+ * When a step has both operations, the SpMM runs on a side stream of the
+ * batch, forked from and joined back to the caller's stream with events.
  * A captured graph holds one step; replaying it needs no host work. */
 typedef struct bsp_batch_s* bsp_batch;
 typedef struct bsp_batch_options {
