@@ -43,7 +43,10 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
     const int2* __restrict__ blk, int64_t nblk) {
-  adaptive_block<CHUNK, ACCUM>(rowptr, col, val, x, y, blk, xcd_remap(blockIdx.x, nblk));
+  // y through the caches (a caller's next use of y, or its next SpMV into
+  // the same buffer, finds it there; non-temporal stores cost 0.6 us more on
+  // the 27-point 64^3 launch)
+  adaptive_block<CHUNK, ACCUM, false>(rowptr, col, val, x, y, blk, xcd_remap(blockIdx.x, nblk));
 }
 
 // All column segments in one launch, each pinned to a set of XCDs:
