@@ -190,8 +190,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     const BTile T = tiles[t];
     const BMat* d = mats + T.m;
     if (T.kind == kKindCsr)
-      adaptive_rows<kCap, false>(d->rowptr, d->col, d->val, x + d->xoff, y + d->yoff, T.r0, T.p0,
-                                 T.r1, T.p1);
+      adaptive_rows<kCap, false, false>(d->rowptr, d->col, d->val, x + d->xoff, y + d->yoff,
+                                        T.r0, T.p0, T.r1, T.p1);
     else
       bsr4_rows(d->brp, d->bci, d->bval, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
                 d->ncols);
