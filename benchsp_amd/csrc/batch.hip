@@ -566,8 +566,8 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
     max_rows = std::max(max_rows, A.nrows);
     max_cols = std::max(max_cols, A.ncols);
   }
-  BSP_REQUIRE(b->row_off[M] < (int64_t(1) << 31) && b->col_off[M] < (int64_t(1) << 31),
-              "batch total rows/columns must fit int32");
+  if (!sizes_fit(b->row_off[M], b->col_off[M], 0))
+    return fail(BSP_ERR_TOO_LARGE, "batch total rows/columns must fit int32 (at most 2^31 - 2^20)");
   b->fmt.assign(M, BSP_BATCH_CSR);
   b->win_spmv.assign(M, kCap);
   b->win_spmm.assign(M, kCapM);

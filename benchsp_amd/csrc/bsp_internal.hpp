@@ -30,14 +30,22 @@ bsp_status fail(bsp_status s, const std::string& msg);
 
 namespace bsp {
 
+// Largest row count, column count and entry count the int32 kernels accept:
+// 2^31 - 2^20, so that an index plus any in-kernel offset (a window of
+// entries, a lane or a workgroup stride, all below 2^20) still fits int32.
+constexpr int64_t kMaxIndex = (int64_t(1) << 31) - (int64_t(1) << 20);
+
+inline bool sizes_fit(int64_t nrows, int64_t ncols, int64_t nnz) {
+  return nrows <= kMaxIndex && ncols <= kMaxIndex && nnz <= kMaxIndex;
+}
+
 // Sizes and index pointers of a CSR descriptor (values not checked): sizes
-// must be non-negative and fit int32 indices (BSP_ERR_TOO_LARGE otherwise).
+// must be non-negative and at most kMaxIndex (BSP_ERR_TOO_LARGE otherwise).
 inline bsp_status check_csr_pattern(const bsp_csr* A) {
   BSP_REQUIRE(A, "null CSR descriptor");
   BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
-  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
-      A->ncols >= (int64_t(1) << 31))
-    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  if (!sizes_fit(A->nrows, A->ncols, A->nnz))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices (at most 2^31 - 2^20)");
   BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
   BSP_REQUIRE(A->nnz == 0 || A->colidx, "null colidx");
   return BSP_OK;

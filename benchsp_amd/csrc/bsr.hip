@@ -745,9 +745,8 @@ namespace {
 bsp_status check_csr_f64(const bsp_csr* A) {
   BSP_REQUIRE(A, "null CSR descriptor");
   BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
-  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
-      A->ncols >= (int64_t(1) << 31))
-    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  if (!sizes_fit(A->nrows, A->ncols, A->nnz))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices (at most 2^31 - 2^20)");
   BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
   BSP_REQUIRE(A->nnz == 0 || (A->colidx && A->val), "null colidx/val");
   BSP_REQUIRE(A->dtype == BSP_F64, "FP64 values required");

@@ -522,8 +522,8 @@ bsp_status bsp_dasp_build(const bsp_csr* A, double threshold, int32_t block_long
                           bsp_dasp* out) {
   BSP_REQUIRE(A && out, "null argument");
   BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
-  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31))
-    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  if (!sizes_fit(A->nrows, A->ncols, A->nnz))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices (at most 2^31 - 2^20)");
   BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
   BSP_REQUIRE(A->nnz == 0 || (A->colidx && A->val), "null colidx/val");
   BSP_REQUIRE(A->dtype == BSP_F64, "FP64 values required");

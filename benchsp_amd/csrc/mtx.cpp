@@ -59,8 +59,8 @@ bsp_status bsp_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32
                           const int32_t* cols, const double* vals, bsp_host_csr* out) {
   BSP_REQUIRE(out && nrows >= 0 && ncols >= 0 && nnz >= 0, "bad COO arguments");
   BSP_REQUIRE(nnz == 0 || (rows && cols), "null COO arrays");
-  if (nrows >= (int64_t(1) << 31) || ncols >= (int64_t(1) << 31) || nnz >= (int64_t(1) << 31))
-    return fail(BSP_ERR_TOO_LARGE, "COO sizes must fit int32");
+  if (!sizes_fit(nrows, ncols, nnz))
+    return fail(BSP_ERR_TOO_LARGE, "COO sizes must fit int32 indices (at most 2^31 - 2^20)");
   std::vector<int32_t> r(rows, rows + nnz), c(cols, cols + nnz);
   std::vector<double> v(nnz, 1.0);
   if (vals) std::copy(vals, vals + nnz, v.begin());
@@ -115,8 +115,8 @@ bsp_status bsp_mtx_read(const char* path, bsp_host_csr* out) {
   const long long N = std::strtoll(end, &end, 10);
   const long long L = std::strtoll(end, &end, 10);
   if (M < 0 || N < 0 || L < 0 || end == p) return fail(BSP_ERR_INVALID, "bad size line");
-  if (M >= (1ll << 31) || N >= (1ll << 31) || (symm || skew ? 2 * L : L) >= (1ll << 31))
-    return fail(BSP_ERR_TOO_LARGE, "MatrixMarket sizes must fit int32");
+  if (!sizes_fit(M, N, symm || skew ? 2 * L : L))
+    return fail(BSP_ERR_TOO_LARGE, "MatrixMarket sizes must fit int32 indices (at most 2^31 - 2^20)");
   std::vector<int32_t> r, c;
   std::vector<double> v;
   const size_t cap = (size_t)(symm || skew ? 2 * L : L);

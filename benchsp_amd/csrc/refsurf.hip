@@ -109,7 +109,8 @@ struct DevCsr {
 
 bsp_status upload_i32(DevCsr& c, const int32_t* rp, const int32_t* ci, const double* v,
                       int64_t rows, int64_t cols, int64_t nnz) {
-  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && nnz < (int64_t(1) << 31), "bad sizes");
+  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "bad sizes");
+  if (!sizes_fit(rows, cols, nnz)) return fail(BSP_ERR_TOO_LARGE, "sizes must fit 32-bit indices");
   BSP_REQUIRE(rp && (nnz == 0 || (ci && v)), "null host pointer");
   BSP_TRY(up(c.rp, rp, rows + 1));
   BSP_TRY(up(c.ci, ci, nnz));
@@ -120,9 +121,8 @@ bsp_status upload_i32(DevCsr& c, const int32_t* rp, const int32_t* ci, const dou
 
 bsp_status upload_i64(DevCsr& c, const int64_t* rp, const int64_t* ci, const double* v,
                       int64_t rows, int64_t cols, int64_t nnz) {
-  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && nnz < (int64_t(1) << 31) &&
-                  rows < (int64_t(1) << 31) && cols < (int64_t(1) << 31),
-              "sizes must fit 32-bit indices");
+  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "bad sizes");
+  if (!sizes_fit(rows, cols, nnz)) return fail(BSP_ERR_TOO_LARGE, "sizes must fit 32-bit indices");
   BSP_REQUIRE(rp && (nnz == 0 || (ci && v)), "null host pointer");
   DevBuf<int64_t> t;
   BSP_TRY(up(t, rp, rows + 1));
@@ -537,7 +537,7 @@ void bsp_smatSpmmfp16(int* rowPtr, int* colIdx, int row, int col, int n, int nnz
 int bsp_cusparse_spmv(int* hA_csrOffsets, int64_t* hA_columns, float* hA_values, float* hX,
                       float* hY, int64_t A_num_rows, int64_t A_num_cols, int64_t A_nnz) {
   if (!hA_csrOffsets || A_num_rows < 0 || A_num_cols < 0 || A_nnz < 0 ||
-      A_nnz >= (int64_t(1) << 31) || A_num_cols >= (int64_t(1) << 31) ||
+      !sizes_fit(A_num_rows, A_num_cols, A_nnz) ||
       (A_nnz && (!hA_columns || !hA_values)) || !hX || !hY)
     return 1;
   DevBuf<int32_t> rp, ci;

@@ -375,9 +375,8 @@ extern "C" {
 bsp_status bsp_spmm_plan_create(const bsp_csr* A, bsp_stream s, bsp_spmm_plan* out) {
   BSP_REQUIRE(A && out, "null argument");
   BSP_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, "negative CSR size");
-  if (A->nnz >= (int64_t(1) << 31) || A->nrows >= (int64_t(1) << 31) ||
-      A->ncols >= (int64_t(1) << 31))
-    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices");
+  if (!sizes_fit(A->nrows, A->ncols, A->nnz))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices (at most 2^31 - 2^20)");
   BSP_REQUIRE(A->nrows == 0 || A->rowptr, "null rowptr");
   BSP_REQUIRE(A->nnz == 0 || A->colidx, "null colidx");
   auto p = std::make_unique<bsp_spmm_plan_s>();

@@ -52,7 +52,9 @@ const char* bsp_version(void);
 
 /* ------------------------------------------------------------------------ */
 /* Device CSR descriptor (all pointers are device memory, not owned).        */
-/* rowptr has nrows+1 int32 entries (nnz < 2^31); columns ascending per row. */
+/* rowptr has nrows+1 int32 entries; nrows, ncols and nnz are at most
+ * 2^31 - 2^20 (BSP_ERR_TOO_LARGE beyond), which leaves in-kernel index
+ * offsets room within int32. Columns ascending per row. */
 typedef struct bsp_csr {
   int64_t nrows, ncols, nnz;
   const int32_t* rowptr;

@@ -148,16 +148,15 @@ def test_compat_header_driver_builds(tmp_path):
                                    "rocsparse_spmm", "batch"])
 @pytest.mark.parametrize("size", ["nnz", "nrows", "ncols", "negative"])
 def test_builders_reject_bad_sizes_before_any_gpu_work(which, size):
-    # every builder validates the descriptor first: sizes past int32 indices
-    # give BSP_ERR_TOO_LARGE (6), negative sizes BSP_ERR_INVALID (1), and no
-    # device call is made (this runs without a GPU)
+    # every builder validates the descriptor first: sizes past 2^31 - 2^20
+    # (int32 indices with room for in-kernel offsets) give BSP_ERR_TOO_LARGE
+    # (6), negative sizes BSP_ERR_INVALID (1), and no device call is made
+    # (this runs without a GPU)
     from benchsp_amd._lib import BlockInfo, BspError, CsrDesc, lib
     from benchsp_amd.device import BatchOptions
-    big = 1 << 31
+    big = (1 << 31) - (1 << 20) + 1
     n, m, z = {"nnz": (10, 10, big), "nrows": (big, 10, 5), "ncols": (10, big, 5),
                "negative": (10, -1, 5)}[size]
-    if which == "dasp" and size == "ncols":
-        n, m, z = 10, 10, big  # DASP indexes rows and entries only
     d = CsrDesc(n, m, z, None, None, None, 0)  # FP64 values
     if which in ("rocsparse_spmm",):
         d = CsrDesc(n, m, z, None, None, None, 1)  # FP32 values
