@@ -808,8 +808,11 @@ bsp_status bsp_batch_graph_capture(bsp_batch b, int schedule, const double* x, d
   bsp_status st = bsp_batch_run(b, schedule, x, y, X, Y, b->cap);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->cap, &g);
-  BSP_TRY(st);
-  if (e != hipSuccess) return fail(BSP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  if (st != BSP_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (st != BSP_OK) return st;
+    return fail(BSP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
   b->graph = g;
   BSP_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   return BSP_OK;
