@@ -61,6 +61,19 @@ struct DevBuf {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// One dispatch holds fewer than 2^32 work-items. A kernel that gives a wave
+// or a workgroup to each item (a row, a block row, a block) takes the index
+// of its first item, and the host issues it in slices of at most 2^31
+// work-items: `launch(first, count)` per slice, one slice at every size that
+// fits (a wave per row stays one dispatch up to 2^25 rows).
+constexpr int64_t kSliceThreads = int64_t(1) << 31;
+template <class Launch>
+inline void in_slices(int64_t items, int64_t threads_per_item, Launch&& launch) {
+  const int64_t per = kSliceThreads / threads_per_item;
+  for (int64_t first = 0; first < items; first += per)
+    launch(first, items - first < per ? items - first : per);
+}
+
 // Device-wide primitives (prims.hip): stream-ordered, temporary storage
 // allocated internally.
 bsp_status exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, hipStream_t s);

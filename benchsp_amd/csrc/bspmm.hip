@@ -85,8 +85,9 @@ template <bool BF>
 __global__ __launch_bounds__(kTB) void block_scale_convert(const double* __restrict__ bval,
                                                            int64_t nblocks, float* __restrict__ scale,
                                                            uint16_t* __restrict__ hv,
-                                                           uint16_t* __restrict__ hvt) {
-  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+                                                           uint16_t* __restrict__ hvt,
+                                                           int64_t first) {
+  const int64_t b = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (b >= nblocks) return;
   const double* a = bval + b * 256;
@@ -140,10 +141,10 @@ __global__ void gather_i32(const int32_t* __restrict__ src, const int32_t* __res
 // (rows >= R read as zero), so a wave's fragment load is 512 contiguous bytes.
 template <int NT>
 __global__ __launch_bounds__(kTB) void panelize(const uint16_t* __restrict__ src, int64_t R,
-                                                uint16_t* __restrict__ dst) {
+                                                uint16_t* __restrict__ dst, int64_t first) {
   constexpr int N = NT * 16;
   __shared__ uint16_t t[16][N + 2];
-  const int64_t o = blockIdx.x;
+  const int64_t o = first + blockIdx.x;
   for (int idx = threadIdx.x; idx < 16 * N / 8; idx += kTB) {
     const int row = idx / (N / 8), c8 = (idx % (N / 8)) * 8;
     const int64_t r = o * 16 + row;
@@ -192,13 +193,13 @@ __global__ __launch_bounds__(512) void bspmm_rows(const int32_t* __restrict__ pt
                                                   const float* __restrict__ scale,
                                                   const uint16_t* __restrict__ panel,
                                                   float* __restrict__ out, int64_t out_rows,
-                                                  int64_t nouter) {
+                                                  int64_t nouter, int64_t first) {
   constexpr int NW = RowsShape<NT>::NW, TPW = RowsShape<NT>::TPW, CH = RowsShape<NT>::CH;
   constexpr int U = 16 / TPW;
   __shared__ u2 sa[CH][kWave];
   __shared__ int32_t sb[CH], so[CH];
   __shared__ float ss[CH];
-  const int64_t r = blockIdx.x;
+  const int64_t r = first + blockIdx.x;
   if (r >= nouter) return;
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
   const int i = lane & 15, kg = lane >> 4;
@@ -267,9 +268,9 @@ __global__ __launch_bounds__(kDvWaves * 64) void bspmm_dvals(const int32_t* __re
                                                              int64_t M,
                                                              const uint16_t* __restrict__ B,
                                                              int64_t K, float* __restrict__ dvals,
-                                                             int64_t brows) {
+                                                             int64_t brows, int64_t first) {
   constexpr int N = NT * 16, S = N / 32;
-  const int64_t br = blockIdx.x;
+  const int64_t br = first + blockIdx.x;
   if (br >= brows) return;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const int i = lane & 15, kg = lane >> 4;
@@ -320,8 +321,10 @@ static bsp_status launch_rows(bsp_bspmm_s* h, int64_t N, const uint16_t* panel, 
   if (nouter == 0) return BSP_OK;
 #define BSPMM_CASE(NT)                                                                           \
   case NT:                                                                                       \
-    bspmm_rows<NT, BF, TRANS><<<nouter, RowsShape<NT>::NW * kWave, 0, s>>>(                      \
-        ptr, perm, other, blocks, h->scale.get(), panel, out, out_rows, nouter);                 \
+    in_slices(nouter, RowsShape<NT>::NW * kWave, [&](int64_t f, int64_t c) {                     \
+      bspmm_rows<NT, BF, TRANS><<<c, RowsShape<NT>::NW * kWave, 0, s>>>(                         \
+          ptr, perm, other, blocks, h->scale.get(), panel, out, out_rows, nouter, f);            \
+    });                                                                                          \
     break;
   switch (N / 16) {
     BSPMM_CASE(2)
@@ -342,8 +345,10 @@ static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, co
   if (h->brows == 0) return BSP_OK;
 #define DV_CASE(NT)                                                                             \
   case NT:                                                                                      \
-    bspmm_dvals<NT, BF><<<h->brows, kDvWaves * 64, 0, s>>>(h->browptr, h->bcol, dC, h->M, B,  \
-                                                           h->K, dvals, h->brows);              \
+    in_slices(h->brows, kDvWaves * 64, [&](int64_t f, int64_t c) {                             \
+      bspmm_dvals<NT, BF><<<c, kDvWaves * 64, 0, s>>>(h->browptr, h->bcol, dC, h->M, B, h->K,  \
+                                                      dvals, h->brows, f);                      \
+    });                                                                                         \
     break;
   switch (N / 16) {
     DV_CASE(2)
@@ -367,11 +372,16 @@ static bsp_status paneled(bsp_bspmm_s* h, const uint16_t* src, int64_t R, int64_
     h->work_cap = nslab * 16 * N;
   }
   if (nslab == 0) return BSP_OK;
+  auto go = [&](auto kernel) {
+    in_slices(nslab, kTB, [&](int64_t f, int64_t c) {
+      kernel<<<c, kTB, 0, s>>>(src, R, h->work.get(), f);
+    });
+  };
   switch (N / 16) {
-    case 2: panelize<2><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
-    case 4: panelize<4><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
-    case 8: panelize<8><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
-    case 16: panelize<16><<<nslab, kTB, 0, s>>>(src, R, h->work.get()); break;
+    case 2: go(panelize<2>); break;
+    case 4: go(panelize<4>); break;
+    case 8: go(panelize<8>); break;
+    case 16: go(panelize<16>); break;
     default: return fail(BSP_ERR_UNSUPPORTED, "block SpMM needs N in {32, 64, 128, 256}");
   }
   BSP_HIP(hipGetLastError());
@@ -406,12 +416,14 @@ bsp_status bsp_bspmm_build(bsp_bsr A, bsp_dtype half_type, bsp_stream s, bsp_bsp
   BSP_TRY(h->hv.alloc(nb * 256));
   BSP_TRY(h->hvt.alloc(nb * 256));
   if (nb > 0) {
-    if (half_type == BSP_BF16)
-      block_scale_convert<true><<<cdiv(nb * kWave, kTB), kTB, 0, st>>>(v.val, nb, h->scale.get(),
-                                                                      h->hv.get(), h->hvt.get());
-    else
-      block_scale_convert<false><<<cdiv(nb * kWave, kTB), kTB, 0, st>>>(v.val, nb, h->scale.get(),
-                                                                       h->hv.get(), h->hvt.get());
+    in_slices(nb, kWave, [&](int64_t f, int64_t c) {
+      if (half_type == BSP_BF16)
+        block_scale_convert<true><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+            v.val, nb, h->scale.get(), h->hv.get(), h->hvt.get(), f);
+      else
+        block_scale_convert<false><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+            v.val, nb, h->scale.get(), h->hv.get(), h->hvt.get(), f);
+    });
     BSP_HIP(hipGetLastError());
   }
   // transposed block index: blocks ordered by (block column, block row)

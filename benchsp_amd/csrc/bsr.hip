@@ -126,10 +126,10 @@ __global__ void row_len_sq_dev(const int32_t* __restrict__ rowptr, int64_t brows
 __global__ __launch_bounds__(kTB) void bsr_valu_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
-    int64_t brows, int64_t nrows, int64_t ncols, int tm, int tk) {
+    int64_t brows, int64_t nrows, int64_t ncols, int tm, int tk, int64_t first) {
   __shared__ double part[kTB];
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t br = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t br = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int slots = kWave / tm;
   const int slot = lane / tm, r = lane % tm;
   double acc = 0.0;
@@ -167,9 +167,9 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
-    int64_t brows, int64_t nrows, int64_t ncols) {
+    int64_t brows, int64_t nrows, int64_t ncols, int64_t first) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t br = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t br = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (br >= brows) return;  // whole wave exits together
   const int ai = lane & 15;       // A row / B column
   const int kk = lane >> 4;       // k index (column within a block)
@@ -236,8 +236,8 @@ __global__ __launch_bounds__(kTB) void dbsr_place(const int32_t* __restrict__ ro
                                                   const int32_t* __restrict__ rec,
                                                   const int32_t* __restrict__ blcptr, int64_t nrows,
                                                   int32_t* __restrict__ ocol,
-                                                  double* __restrict__ oval) {
-  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+                                                  double* __restrict__ oval, int64_t first) {
+  const int64_t i = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (i >= nrows) return;
   const int o = rec[i];
@@ -467,9 +467,10 @@ __device__ __forceinline__ double dbsr_group(const int32_t* __restrict__ blcptr,
 __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
     const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
-    double* __restrict__ y, int64_t brows, int64_t nrows, int64_t narrow, int sorted_order) {
+    double* __restrict__ y, int64_t brows, int64_t nrows, int64_t narrow, int sorted_order,
+    int64_t first) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t w = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t npair = (narrow + 1) / 2;
   auto store = [&](int64_t g, int r, double v) {
     const int64_t i = 4 * g + r;
@@ -694,9 +695,11 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
   BSP_HIP(hipMemsetAsync(h->colidx.get(), 0, sizeof(int32_t) * (nb ? nb * 16 : 1), s));
   BSP_HIP(hipMemsetAsync(h->val.get(), 0, sizeof(double) * (nb ? nb * 16 : 1), s));
   if (n > 0) {
-    dbsr_place<<<cdiv(n * kWave, kTB), kTB, 0, s>>>(
-        A.rowptr, A.colidx, static_cast<const double*>(A.val), h->recorder.get(),
-        h->blcptr.get(), n, h->colidx.get(), h->val.get());
+    in_slices(n, kWave, [&](int64_t f, int64_t c) {
+      dbsr_place<<<cdiv(c * kWave, kTB), kTB, 0, s>>>(
+          A.rowptr, A.colidx, static_cast<const double*>(A.val), h->recorder.get(),
+          h->blcptr.get(), n, h->colidx.get(), h->val.get(), f);
+    });
     dbsr_map<<<cdiv(v.brows, kTB), kTB, 0, s>>>(slen.get(), h->blcptr.get(), n, v.brows,
                                                 h->map.get());
     BSP_HIP(hipGetLastError());
@@ -781,13 +784,17 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
   if (v.brows == 0) return BSP_OK;
   if (kernel == BSP_BSR_AUTO) kernel = BSP_BSR_VALU;
   hipStream_t st = (hipStream_t)s;
-  const int64_t grid = cdiv(v.brows * kWave, kTB);
   if (kernel == BSP_BSR_VALU) {
-    bsr_valu_f64<<<grid, kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows, v.nrows, v.ncols,
-                                       v.tm, v.tk);
+    in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
+      bsr_valu_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows,
+                                                         v.nrows, v.ncols, v.tm, v.tk, f);
+    });
   } else if (kernel == BSP_BSR_MFMA) {
     if (v.tm != 4 || v.tk != 4) return fail(BSP_ERR_UNSUPPORTED, "MFMA BSR kernel needs 4x4 blocks");
-    bsr4_mfma_f64<<<grid, kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows, v.nrows, v.ncols);
+    in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
+      bsr4_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
+                                                          v.brows, v.nrows, v.ncols, f);
+    });
   } else {
     return fail(BSP_ERR_INVALID, "unknown BSR kernel");
   }
@@ -821,8 +828,10 @@ bsp_status bsp_dbsr_spmv(bsp_dbsr h, const double* x, double* y, int sorted_orde
   BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
   if (v.brows == 0) return BSP_OK;
   const int64_t waves = (h->narrow + 1) / 2 + (v.brows - h->narrow);
-  dbsr_spmv_f64<<<cdiv(waves * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
-      v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, h->narrow, sorted_order);
+  in_slices(waves, kWave, [&](int64_t f, int64_t c) {
+    dbsr_spmv_f64<<<cdiv(c * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
+        v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, h->narrow, sorted_order, f);
+  });
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

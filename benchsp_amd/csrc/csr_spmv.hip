@@ -342,8 +342,9 @@ static int scatter_stats_grid(int64_t nrows) {
 // the lane at its last entry adds j + 1 (cnt starts at zero).
 __global__ __launch_bounds__(kWG) void seg_count(const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ col, int64_t nrows,
-                                                 int64_t width, int32_t* __restrict__ cnt) {
-  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+                                                 int64_t width, int32_t* __restrict__ cnt,
+                                                 int64_t first) {
+  const int64_t r = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (r >= nrows) return;
   const int j0 = rowptr[r], e = rowptr[r + 1];
@@ -365,8 +366,8 @@ __global__ __launch_bounds__(kWG) void seg_scatter(const int32_t* __restrict__ r
                                                    int64_t width, const int32_t* __restrict__ segrp,
                                                    const int64_t* __restrict__ base,
                                                    int32_t* __restrict__ ocol,
-                                                   double* __restrict__ oval) {
-  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+                                                   double* __restrict__ oval, int64_t first) {
+  const int64_t r = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (r >= nrows) return;
   const int j0 = rowptr[r], e = rowptr[r + 1];
@@ -454,7 +455,9 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     DevBuf<int32_t> cnt;
     BSP_TRY(cnt.alloc(nseg * (n + 1)));
     BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * nseg * (n + 1), s));
-    seg_count<<<cdiv(n * kWave, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, W, cnt.get());
+    in_slices(n, kWave, [&](int64_t f, int64_t c) {
+      seg_count<<<cdiv(c * kWave, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, n, W, cnt.get(), f);
+    });
     BSP_HIP(hipGetLastError());
     DevBuf<int32_t> segrp;
     BSP_TRY(segrp.alloc(nseg * (n + 1)));
@@ -471,9 +474,11 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
                            hipMemcpyHostToDevice, s));
     BSP_TRY(p->seg_col.alloc(A.nnz));
     BSP_TRY(p->seg_val.alloc(A.nnz));
-    seg_scatter<<<cdiv(n * kWave, kWG), kWG, 0, s>>>(
-        A.rowptr, A.colidx, static_cast<const double*>(A.val), n, W, segrp.get(), dbase.get(),
-        p->seg_col.get(), p->seg_val.get());
+    in_slices(n, kWave, [&](int64_t f, int64_t c) {
+      seg_scatter<<<cdiv(c * kWave, kWG), kWG, 0, s>>>(
+          A.rowptr, A.colidx, static_cast<const double*>(A.val), n, W, segrp.get(), dbase.get(),
+          p->seg_col.get(), p->seg_val.get(), f);
+    });
     BSP_HIP(hipGetLastError());
     for (int64_t k = 0; k < nseg; ++k) {
       CsrSegment& g = p->seg[k];

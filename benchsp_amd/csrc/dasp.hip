@@ -206,9 +206,9 @@ __global__ __launch_bounds__(kTB) void block_fill(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cid,
     const double* __restrict__ val, const int32_t* __restrict__ ridA, int32_t nb,
     const int32_t* __restrict__ bptr, const int32_t* __restrict__ irpt, double* __restrict__ rv,
-    int32_t* __restrict__ rc, double* __restrict__ iv, int32_t* __restrict__ ic) {
+    int32_t* __restrict__ rc, double* __restrict__ iv, int32_t* __restrict__ ic, int64_t first) {
   // wave per slot k, lanes striding its entries
-  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t k = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (k >= nb) return;
   const int b = (int)(k / BS), g = (int)(k % BS);
@@ -490,10 +490,11 @@ static bsp_status build_dasp(const bsp_csr& A, double threshold, int32_t block_l
   BSP_TRY(h->irreg_cid.alloc(nirr));
   BSP_HIP(hipMemsetAsync(h->reg_val.get(), 0, sizeof(double) * (freg ? freg : 1), s));
   BSP_HIP(hipMemsetAsync(h->reg_cid.get(), 0, sizeof(int32_t) * (freg ? freg : 1), s));
-  if (nb > 0)
-    block_fill<<<cdiv((int64_t)nb * kWave, kTB), kTB, 0, s>>>(
+  in_slices(nb, kWave, [&](int64_t f, int64_t c) {
+    block_fill<<<cdiv(c * kWave, kTB), kTB, 0, s>>>(
         A.rowptr, A.colidx, aval, h->ridA.get(), nb, h->block_ptr.get(), h->irreg_rpt.get(),
-        h->reg_val.get(), h->reg_cid.get(), h->irreg_val.get(), h->irreg_cid.get());
+        h->reg_val.get(), h->reg_cid.get(), h->irreg_val.get(), h->irreg_cid.get(), f);
+  });
   BSP_HIP(hipGetLastError());
   BSP_HIP(hipStreamSynchronize(s));
   v.order_rid = h->order_rid.get();

@@ -225,8 +225,8 @@ __global__ void fill_items(const int32_t* __restrict__ ptr, int64_t n,
 __global__ __launch_bounds__(kTB) void csc_keys(const int32_t* __restrict__ rowptr,
                                                 const int32_t* __restrict__ col, int64_t n,
                                                 int64_t* __restrict__ key,
-                                                int32_t* __restrict__ idx) {
-  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+                                                int32_t* __restrict__ idx, int64_t first) {
+  const int64_t r = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (r >= n) return;
   for (int j = rowptr[r] + lane, e = rowptr[r + 1]; j < e; j += kWave) {
@@ -299,8 +299,10 @@ static bsp_status build_spmm_plan(bsp_spmm_plan_s* p, hipStream_t s) {
   BSP_TRY(p->rowidx.alloc(nnz));
   BSP_TRY(p->colptr.alloc(A.ncols + 1));
   if (nnz > 0) {
-    csc_keys<<<cdiv(A.nrows * kWave, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, A.nrows, key.get(),
-                                                        idx.get());
+    in_slices(A.nrows, kWave, [&](int64_t f, int64_t c) {
+      csc_keys<<<cdiv(c * kWave, kTB), kTB, 0, s>>>(A.rowptr, A.colidx, A.nrows, key.get(),
+                                                    idx.get(), f);
+    });
     BSP_HIP(hipGetLastError());
     int bits = 1;
     while (bits < 63 && (int64_t(1) << bits) <= A.ncols * A.nrows) ++bits;
