@@ -472,6 +472,35 @@ def copy_probe():
     print("copied 256 MiB x3", flush=True)
 
 
+def measured_peaks() -> dict:
+    """What the box's HBM delivers to plain torch kernels: a 1 GiB device
+    copy (bytes read + written over time) and a 1 GiB FP64 sum (bytes read
+    over time), each the best of 10 event-timed runs after 2 warmups; 1 GiB
+    is four times the MALL, so both stream from HBM."""
+    import torch
+    a = torch.rand(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    nbytes = a.numel() * 8
+
+    def best(fn):
+        t = []
+        for i in range(12):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            if i >= 2:
+                t.append(e0.elapsed_time(e1) * 1e-3)
+        return min(t)
+    copy_s = best(lambda: b.copy_(a))
+    read_s = best(lambda: a.sum())
+    del a, b
+    torch.cuda.empty_cache()
+    return {"copy_GBs": round(2 * nbytes / copy_s / 1e9, 1),
+            "read_GBs": round(nbytes / read_s / 1e9, 1)}
+
+
 SPMV_KERNELS = ("csr_adaptive", "sum_partials")
 
 
@@ -590,6 +619,11 @@ def main():
                    "matrices": sp["per"]},
         "roofline": roofline(sp["per"], args.method),
     }
+    # the nominal 8 TB/s is `peak`; what plain streaming kernels reach on
+    # this box is reported beside it
+    pk = measured_peaks()
+    out["roofline"]["measured_peaks"] = dict(
+        pk, frac_of_read=round(out["roofline"]["achieved"] / pk["read_GBs"], 4))
     if not args.no_formats:
         out["formats"] = formats_leg(args, dev, list(zip(sp["per"].keys(), sp["mats"])), sp["xs"])
         _progress("formats leg done")
