@@ -86,6 +86,17 @@ class BlockInfo(C.Structure):
                 ("avg_packed", f64)]
 
 
+class TuneRecord(C.Structure):
+    """struct bsp_tune_record."""
+    _fields_ = [("format", i32), ("kernel", i32), ("chunk", i32), ("block_longest", i32),
+                ("seg_cols", i64), ("threshold", f64), ("time_us", f64), ("reserved", i64 * 4)]
+
+
+class TuneOptions(C.Structure):
+    """struct bsp_tune_options."""
+    _fields_ = [("formats", i32), ("reps", i32), ("reserved", i64 * 7)]
+
+
 VOID = "void"  # restype marker for the reference's void functions
 
 # name -> (restype, argtypes); restype None means bsp_status (checked).
@@ -142,6 +153,13 @@ _SIGS: dict[str, tuple] = {
     "bsp_dasp_view_get": (None, [vp, C.POINTER(DaspView)]),
     "bsp_dasp_spmv": (None, [vp, vp, vp, C.c_int, vp]),
     "bsp_dasp_destroy": (None, [vp]),
+    "bsp_autotune": (None, [C.POINTER(CsrDesc), C.POINTER(TuneOptions), vp, C.POINTER(vp)]),
+    "bsp_tuned_from_record": (None, [C.POINTER(CsrDesc), C.POINTER(TuneRecord), vp,
+                                     C.POINTER(vp)]),
+    "bsp_tuned_record": (None, [vp, C.POINTER(TuneRecord)]),
+    "bsp_tuned_trials": (None, [vp, C.POINTER(TuneRecord), i32, C.POINTER(i32)]),
+    "bsp_tuned_spmv": (None, [vp, vp, vp, vp]),
+    "bsp_tuned_destroy": (None, [vp]),
     "bsp_copy_to_host": (None, [vp, vp, C.c_size_t]),
     "bsp_bsr_build": (None, [C.POINTER(CsrDesc), C.c_int, C.c_int, vp, C.POINTER(vp)]),
     "bsp_bsr_view_get": (None, [vp, C.POINTER(BsrView)]),

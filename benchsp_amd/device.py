@@ -240,6 +240,76 @@ class Dasp:
             pass
 
 
+TUNE_FORMATS = {"csr": 0, "bsr4": 1, "dbsr": 2, "dasp": 3}
+_TUNE_NAMES = {v: k for k, v in TUNE_FORMATS.items()}
+
+
+def _record_dict(r) -> dict:
+    return {"format": _TUNE_NAMES.get(r.format, r.format), "kernel": r.kernel, "chunk": r.chunk,
+            "seg_cols": r.seg_cols, "threshold": r.threshold,
+            "block_longest": r.block_longest, "time_us": r.time_us}
+
+
+class Tuned:
+    """The fastest SpMV layout of one FP64 matrix (bsp_autotune), or the layout
+    a stored record names (Tuned.from_record, no timing). `record` is a plain
+    dict (JSON-serialisable); y comes back in the original row order."""
+
+    def __init__(self, A: DeviceCsr, formats=None, reps: int = 0, _record=None):
+        from ._lib import TuneOptions, TuneRecord
+        self.A = A
+        d = A.desc()
+        h = C.c_void_p()
+        if _record is None:
+            mask = 0 if formats is None else sum(1 << TUNE_FORMATS[f] for f in formats)
+            lib().bsp_autotune(C.byref(d), C.byref(TuneOptions(mask, int(reps))),
+                               stream_handle(), C.byref(h))
+        else:
+            r = TuneRecord()
+            r.format = TUNE_FORMATS[_record["format"]]
+            r.kernel = int(_record.get("kernel", 0))
+            r.chunk = int(_record.get("chunk", 0))
+            r.seg_cols = int(_record.get("seg_cols", 0))
+            r.threshold = float(_record.get("threshold", 0.0))
+            r.block_longest = int(_record.get("block_longest", 0))
+            lib().bsp_tuned_from_record(C.byref(d), C.byref(r), stream_handle(), C.byref(h))
+        self._h = h
+
+    @classmethod
+    def from_record(cls, A: DeviceCsr, record: dict) -> "Tuned":
+        return cls(A, _record=record)
+
+    @property
+    def record(self) -> dict:
+        from ._lib import TuneRecord
+        r = TuneRecord()
+        lib().bsp_tuned_record(self._h, C.byref(r))
+        return _record_dict(r)
+
+    def trials(self) -> list[dict]:
+        from ._lib import TuneRecord
+        n = C.c_int32()
+        lib().bsp_tuned_trials(self._h, None, 0, C.byref(n))
+        recs = (TuneRecord * max(n.value, 1))()
+        lib().bsp_tuned_trials(self._h, recs, n.value, C.byref(n))
+        return [_record_dict(recs[i]) for i in range(n.value)]
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        assert x.dtype == torch.float64 and x.is_contiguous() and x.numel() == self.A.ncols
+        if y is None:
+            y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
+        lib().bsp_tuned_spmv(self._h, ptr(x), ptr(y), stream_handle())
+        return y
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_tuned_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 class BlockSpmm:
     """Block-sparse SpMM on FP16/BF16 MFMA over a 16x16 Bsr (bsp_bspmm_*)."""
 

@@ -231,6 +231,60 @@ bsp_status bsp_dasp_spmv(bsp_dasp h, const double* x, double* y, int permuted, b
 bsp_status bsp_dasp_destroy(bsp_dasp h);
 
 /* ------------------------------------------------------------------------ */
+/* Per-matrix SpMV autotuner. No counterpart in the reference, where format
+ * and thresholds are fixed per call site (bench/spmvfp64.cpp:135-177; DASP
+ * threshold 0.75 and block_longest 256 at :177; AmgT's stand / avgnz cuts,
+ * src/amgT.cu:1355-1488): the tuner builds each candidate layout of one FP64
+ * matrix on the GPU, times its SpMV and keeps the fastest. Candidates:
+ * adaptive CSR with windows 512 / 1024 / 2048 (and, when the automatic plan
+ * segments columns, the unsegmented plan too); 4x4 BSR on VALU and MFMA when
+ * its blocks hold at least 8 entries on average (bsp_block_info); DBSR; DASP
+ * with threshold 0.5 / 0.75 / 1.0 x block_longest 64 / 256. Timing is
+ * cache-warm (the same matrix and x every launch), so it ranks the layouts'
+ * kernels, not a cold HBM read. The choice is a plain record: store it and
+ * rebuild the same layout later with bsp_tuned_from_record, no timing. */
+typedef enum bsp_tune_format {
+  BSP_TUNE_CSR = 0,  /* adaptive plan: chunk, seg_cols as in bsp_csr_options */
+  BSP_TUNE_BSR4 = 1, /* 4x4 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
+  BSP_TUNE_DBSR = 2,
+  BSP_TUNE_DASP = 3  /* threshold, block_longest as in bsp_dasp_build */
+} bsp_tune_format;
+
+typedef struct bsp_tune_record {
+  int32_t format;        /* bsp_tune_format */
+  int32_t kernel;        /* BSR4 kernel */
+  int32_t chunk;         /* CSR window, 128..2048 */
+  int32_t block_longest; /* DASP */
+  int64_t seg_cols;      /* CSR: 0 = automatic, < 0 = never segment */
+  double threshold;      /* DASP */
+  double time_us;        /* mean SpMV time measured by the tuner (0 when replayed) */
+  int64_t reserved[4];
+} bsp_tune_record;
+
+typedef struct bsp_tune_options {
+  int32_t formats; /* bit mask of 1 << bsp_tune_format to try; 0 = all */
+  int32_t reps;    /* timed SpMVs per candidate after two warmups; 0 = 10 */
+  int64_t reserved[7];
+} bsp_tune_options;
+
+typedef struct bsp_tuned_s* bsp_tuned;
+/* Times every candidate (stream-ordered, synchronises `s`); a candidate
+ * whose build fails (unsorted columns for a segmented plan, memory) is
+ * skipped. BSP_ERR_UNSUPPORTED when no candidate could be built. */
+bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_stream s,
+                        bsp_tuned* out);
+/* The layout `rec` names, built without timing (rec->time_us is ignored). */
+bsp_status bsp_tuned_from_record(const bsp_csr* A, const bsp_tune_record* rec, bsp_stream s,
+                                 bsp_tuned* out);
+bsp_status bsp_tuned_record(bsp_tuned h, bsp_tune_record* rec);
+/* The candidates timed, in the order tried: up to `cap` records; *count is
+ * the number tried (0 for a handle built from a record). */
+bsp_status bsp_tuned_trials(bsp_tuned h, bsp_tune_record* recs, int32_t cap, int32_t* count);
+/* y = A*x in the original row order, with the chosen layout. */
+bsp_status bsp_tuned_spmv(bsp_tuned h, const double* x, double* y, bsp_stream s);
+bsp_status bsp_tuned_destroy(bsp_tuned h);
+
+/* ------------------------------------------------------------------------ */
 /* Differentiable CSR SpMM: Y = A*X with dense row-major X (ncols x F) and Y
  * (nrows x F, FP32), FP32 accumulation. A's values are FP32 or BF16, dense
  * operands FP32 or BF16; F is 64, 128 or 256. No counterpart in the

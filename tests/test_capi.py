@@ -145,14 +145,14 @@ def test_compat_header_driver_builds(tmp_path):
 
 
 @pytest.mark.parametrize("which", ["bsr", "dbsr", "dasp", "spmm", "blockinfo", "rocsparse",
-                                   "rocsparse_spmm", "batch"])
+                                   "rocsparse_spmm", "batch", "autotune", "tuned_from_record"])
 @pytest.mark.parametrize("size", ["nnz", "nrows", "ncols", "negative"])
 def test_builders_reject_bad_sizes_before_any_gpu_work(which, size):
     # every builder validates the descriptor first: sizes past 2^31 - 2^20
     # (int32 indices with room for in-kernel offsets) give BSP_ERR_TOO_LARGE
     # (6), negative sizes BSP_ERR_INVALID (1), and no device call is made
     # (this runs without a GPU)
-    from benchsp_amd._lib import BlockInfo, BspError, CsrDesc, lib
+    from benchsp_amd._lib import BlockInfo, BspError, CsrDesc, TuneRecord, lib
     from benchsp_amd.device import BatchOptions
     big = (1 << 31) - (1 << 20) + 1
     n, m, z = {"nnz": (10, 10, big), "nrows": (big, 10, 5), "ncols": (10, big, 5),
@@ -173,7 +173,27 @@ def test_builders_reject_bad_sizes_before_any_gpu_work(which, size):
                                                                     C.byref(h)),
         "batch": lambda: L.bsp_batch_create(C.byref(d), 1, C.byref(BatchOptions(0, 0)), None,
                                             C.byref(h)),
+        "autotune": lambda: L.bsp_autotune(C.byref(d), None, None, C.byref(h)),
+        "tuned_from_record": lambda: L.bsp_tuned_from_record(C.byref(d), C.byref(TuneRecord()),
+                                                             None, C.byref(h)),
     }
     with pytest.raises(BspError) as e:
         calls[which]()
     assert e.value.status == (1 if size == "negative" else 6), (which, size, str(e.value))
+
+
+@pytest.mark.parametrize("rec", [dict(format=7), dict(format=0, chunk=64),
+                                 dict(format=1, kernel=0), dict(format=3, threshold=0.0,
+                                                                block_longest=256),
+                                 dict(format=3, threshold=0.75, block_longest=4)])
+def test_tune_record_validated_before_any_gpu_work(rec):
+    # an empty matrix and a malformed record: BSP_ERR_INVALID (1), no device call
+    from benchsp_amd._lib import BspError, CsrDesc, TuneRecord, lib
+    d = CsrDesc(0, 0, 0, None, None, None, 0)
+    r = TuneRecord()
+    for k, v in rec.items():
+        setattr(r, k, v)
+    h = C.c_void_p()
+    with pytest.raises(BspError) as e:
+        lib().bsp_tuned_from_record(C.byref(d), C.byref(r), None, C.byref(h))
+    assert e.value.status == 1, str(e.value)
