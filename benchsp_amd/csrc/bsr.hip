@@ -206,6 +206,61 @@ __global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
   }
 }
 
+// 8x8 blocks on the same v_mfma_f64_16x16x4f64, two blocks per product:
+// A rows 8a..8a+7 are rows of block a (a = 0, 1) at four of its columns
+// (4h..4h+3, h = 0, 1: two products per pair), B column j is the matching x
+// segment of block j/8. The two 8x8 diagonal tiles of the 16x16 result then
+// hold A_a*x_a in every column: result row R, column R sits in acc[R/4] of
+// lane 16(R%4) + R (C layout above), and y row r of the block row is the sum
+// of result rows r and r+8, found in lanes L and L+8, L = 16(r%4) + r.
+__global__ __launch_bounds__(kTB) void bsr8_mfma_f64(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
+    int64_t brows, int64_t nrows, int64_t ncols, int64_t first) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t br = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (br >= brows) return;  // whole wave exits together
+  const int i = lane & 15;   // A row / B column
+  const int kk = lane >> 4;  // k index
+  const int a = i >> 3, r = i & 7;
+  const int s0 = rowptr[br], e0 = rowptr[br + 1];
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  // block column indices 64 at a time, then four pairs (eight blocks, eight
+  // products) per step with all their operand loads in flight
+  for (int base = s0; base < e0; base += kWave) {
+    const int nb = min(kWave, e0 - base);
+    const int cid = lane < nb ? colidx[base + lane] : 0;
+    for (int k0 = 0; k0 < nb; k0 += 8) {
+      double av[8], bv[8];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kb = k0 + 2 * u + a;
+        const int c8 = __shfl(cid, min(kb, kWave - 1), 64);
+        const bool ok = kb < nb;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int cb = 4 * h + kk;  // column within the block
+          const int64_t c = (int64_t)c8 * 8 + cb;
+          av[2 * u + h] = ok ? ld_stream(&bval[(int64_t)(base + kb) * 64 + r * 8 + cb]) : 0.0;
+          bv[2 * u + h] = ok && c < ncols ? x[c] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+    }
+  }
+  // lane (J = lane%16, g = lane/16) holds result row g + 4v, column J in
+  // acc[v]; the diagonal entry (row == column) is v = (J - g)/4
+  const int J = lane & 15, g = lane >> 4, d = J - g;
+  double v = 0.0;
+  if (d >= 0 && (d & 3) == 0) v = (d >> 2) == 0 ? acc[0] : (d >> 2) == 1 ? acc[1] : (d >> 2) == 2 ? acc[2] : acc[3];
+  const double t = v + __shfl_down(v, 8, 64);
+  if (J < 8 && g == (J & 3)) {
+    const int64_t row = br * 8 + J;
+    if (row < nrows) y[row] = t;
+  }
+}
+
 // ---- DBSR -------------------------------------------------------------------
 
 __global__ void row_lengths(const int32_t* __restrict__ rowptr, int64_t nrows,
@@ -790,10 +845,15 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
                                                          v.nrows, v.ncols, v.tm, v.tk, f);
     });
   } else if (kernel == BSP_BSR_MFMA) {
-    if (v.tm != 4 || v.tk != 4) return fail(BSP_ERR_UNSUPPORTED, "MFMA BSR kernel needs 4x4 blocks");
+    const bool b4 = v.tm == 4 && v.tk == 4, b8 = v.tm == 8 && v.tk == 8;
+    if (!b4 && !b8) return fail(BSP_ERR_UNSUPPORTED, "MFMA BSR kernel needs 4x4 or 8x8 blocks");
     in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
-      bsr4_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
-                                                          v.brows, v.nrows, v.ncols, f);
+      if (b4)
+        bsr4_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
+                                                            v.brows, v.nrows, v.ncols, f);
+      else
+        bsr8_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
+                                                            v.brows, v.nrows, v.ncols, f);
     });
   } else {
     return fail(BSP_ERR_INVALID, "unknown BSR kernel");

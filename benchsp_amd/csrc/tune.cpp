@@ -28,7 +28,7 @@ struct bsp_tuned_s {
 
 namespace {
 
-constexpr double kMinBlockFill = 8.0;  // 4x4 BSR candidates: entries per block
+constexpr double kMinBlockFill = 0.5;  // BSR candidates: average fraction of a block filled
 constexpr int kWarmups = 2;
 
 bsp_status check_record(const bsp_tune_record& r) {
@@ -37,8 +37,9 @@ bsp_status check_record(const bsp_tune_record& r) {
       BSP_REQUIRE(r.chunk >= 128 && r.chunk <= 2048, "CSR record: chunk must be in 128..2048");
       return BSP_OK;
     case BSP_TUNE_BSR4:
+    case BSP_TUNE_BSR8:
       BSP_REQUIRE(r.kernel == BSP_BSR_VALU || r.kernel == BSP_BSR_MFMA,
-                  "BSR4 record: kernel must be BSP_BSR_VALU or BSP_BSR_MFMA");
+                  "BSR record: kernel must be BSP_BSR_VALU or BSP_BSR_MFMA");
       return BSP_OK;
     case BSP_TUNE_DBSR:
       return BSP_OK;
@@ -63,6 +64,8 @@ bsp_status build(const bsp_csr& A, const bsp_tune_record& r, hipStream_t s, bsp_
     }
     case BSP_TUNE_BSR4:
       return bsp_bsr_build(&A, 4, 4, s, &h->bsr);
+    case BSP_TUNE_BSR8:
+      return bsp_bsr_build(&A, 8, 8, s, &h->bsr);
     case BSP_TUNE_DBSR:
       return bsp_dbsr_build(&A, s, &h->dbsr);
     default:
@@ -73,7 +76,8 @@ bsp_status build(const bsp_csr& A, const bsp_tune_record& r, hipStream_t s, bsp_
 bsp_status run(const bsp_tuned_s* h, const double* x, double* y, hipStream_t s) {
   switch (h->rec.format) {
     case BSP_TUNE_CSR: return bsp_csr_spmv(h->csr, x, y, s);
-    case BSP_TUNE_BSR4: return bsp_bsr_spmv(h->bsr, h->rec.kernel, x, y, s);
+    case BSP_TUNE_BSR4:
+    case BSP_TUNE_BSR8: return bsp_bsr_spmv(h->bsr, h->rec.kernel, x, y, s);
     case BSP_TUNE_DBSR: return bsp_dbsr_spmv(h->dbsr, x, y, 0, s);
     default: return bsp_dasp_spmv(h->dasp, x, y, 0, s);
   }
@@ -118,9 +122,9 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
   BSP_TRY(check_csr_pattern(A));
   BSP_REQUIRE(out, "null output handle");
   BSP_REQUIRE(A->dtype == BSP_F64 && (A->nnz == 0 || A->val), "FP64 values required");
-  const int formats = opt && opt->formats ? opt->formats : 0xf;
+  const int formats = opt && opt->formats ? opt->formats : 0x1f;
   const int reps = opt && opt->reps > 0 ? opt->reps : 10;
-  BSP_REQUIRE((formats & ~0xf) == 0, "unknown format bits");
+  BSP_REQUIRE((formats & ~0x1f) == 0, "unknown format bits");
   BSP_REQUIRE(!opt || opt->reps >= 0, "reps must be >= 0");
   hipStream_t s = (hipStream_t)stream;
 
@@ -138,16 +142,20 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
     if (k > 1)
       for (int c : {1024, 2048}) cands.push_back(csr_rec(c, -1));
   }
-  if ((formats & (1 << BSP_TUNE_BSR4)) && A->nnz > 0) {
+  // block sizes: 4x4 and 8x8 BSR, each only where its blocks are at least
+  // half full on average (sparser blocks move several times the CSR bytes)
+  for (const int b : {4, 8}) {
+    const int fmt = b == 4 ? BSP_TUNE_BSR4 : BSP_TUNE_BSR8;
+    if (!(formats & (1 << fmt)) || A->nnz == 0) continue;
     bsp_blockinfo bi{};
-    BSP_TRY(bsp_block_info(A, 4, stream, &bi));
-    if (bi.avg >= kMinBlockFill)
-      for (int kern : {BSP_BSR_VALU, BSP_BSR_MFMA}) {
-        bsp_tune_record r{};
-        r.format = BSP_TUNE_BSR4;
-        r.kernel = kern;
-        cands.push_back(r);
-      }
+    BSP_TRY(bsp_block_info(A, b, stream, &bi));
+    if (bi.avg < kMinBlockFill * b * b) continue;
+    for (int kern : {BSP_BSR_VALU, BSP_BSR_MFMA}) {
+      bsp_tune_record r{};
+      r.format = fmt;
+      r.kernel = kern;
+      cands.push_back(r);
+    }
   }
   if (formats & (1 << BSP_TUNE_DBSR)) {
     bsp_tune_record r{};

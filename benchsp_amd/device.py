@@ -240,7 +240,7 @@ class Dasp:
             pass
 
 
-TUNE_FORMATS = {"csr": 0, "bsr4": 1, "dbsr": 2, "dasp": 3}
+TUNE_FORMATS = {"csr": 0, "bsr4": 1, "dbsr": 2, "dasp": 3, "bsr8": 4}
 _TUNE_NAMES = {v: k for k, v in TUNE_FORMATS.items()}
 
 

@@ -148,9 +148,10 @@ typedef enum bsp_bsr_kernel {
    * Counterpart of bsr_spmv_cc_fp64 / bcsr_spmv_kernel_column_by_column
    * (src/amgT.cu:276-322, src/bcsr.cu:161-222). */
   BSP_BSR_VALU = 1,
-  /* 4x4 only: four blocks of a block row stacked into the 16x4 A operand of
-   * v_mfma_f64_16x16x4f64, x segments as B columns, the block-diagonal of the
-   * 16x16 result accumulated. Counterpart of bsr_spmv_tc_fp64
+  /* 4x4 or 8x8 blocks on v_mfma_f64_16x16x4f64: four 4x4 blocks (or two
+   * 8x8 blocks, four of their columns per product) of a block row stacked
+   * into the 16x4 A operand, x segments as B columns, the block-diagonal of
+   * the 16x16 result accumulated. Counterpart of bsr_spmv_tc_fp64
    * (src/amgT.cu:210-274, mma.m8n8k4 with 2 blocks). */
   BSP_BSR_MFMA = 2
 } bsp_bsr_kernel;
@@ -237,8 +238,8 @@ bsp_status bsp_dasp_destroy(bsp_dasp h);
  * src/amgT.cu:1355-1488): the tuner builds each candidate layout of one FP64
  * matrix on the GPU, times its SpMV and keeps the fastest. Candidates:
  * adaptive CSR with windows 512 / 1024 / 2048 (and, when the automatic plan
- * segments columns, the unsegmented plan too); 4x4 BSR on VALU and MFMA when
- * its blocks hold at least 8 entries on average (bsp_block_info); DBSR; DASP
+ * segments columns, the unsegmented plan too); 4x4 and 8x8 BSR on VALU and
+ * MFMA when their blocks are at least half full on average (bsp_block_info); DBSR; DASP
  * with threshold 0.5 / 0.75 / 1.0 x block_longest 64 / 256. Timing is
  * cache-warm (the same matrix and x every launch), so it ranks the layouts'
  * kernels, not a cold HBM read. The choice is a plain record: store it and
@@ -247,12 +248,13 @@ typedef enum bsp_tune_format {
   BSP_TUNE_CSR = 0,  /* adaptive plan: chunk, seg_cols as in bsp_csr_options */
   BSP_TUNE_BSR4 = 1, /* 4x4 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
   BSP_TUNE_DBSR = 2,
-  BSP_TUNE_DASP = 3  /* threshold, block_longest as in bsp_dasp_build */
+  BSP_TUNE_DASP = 3, /* threshold, block_longest as in bsp_dasp_build */
+  BSP_TUNE_BSR8 = 4  /* 8x8 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
 } bsp_tune_format;
 
 typedef struct bsp_tune_record {
   int32_t format;        /* bsp_tune_format */
-  int32_t kernel;        /* BSR4 kernel */
+  int32_t kernel;        /* BSR4 / BSR8 kernel */
   int32_t chunk;         /* CSR window, 128..2048 */
   int32_t block_longest; /* DASP */
   int64_t seg_cols;      /* CSR: 0 = automatic, < 0 = never segment */

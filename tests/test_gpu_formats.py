@@ -43,7 +43,7 @@ def _same(a, b, what):
     assert a.shape == b.shape and np.array_equal(a, b), what
 
 
-@pytest.mark.parametrize("tm,tk", [(4, 4), (3, 3), (16, 16), (2, 5)])
+@pytest.mark.parametrize("tm,tk", [(4, 4), (3, 3), (8, 8), (16, 16), (2, 5)])
 def test_bsr_builder_bit_exact(oracle, bsp_host, tm, tk):
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
@@ -66,13 +66,16 @@ def test_bsr_builder_bit_exact(oracle, bsp_host, tm, tk):
 
 
 @pytest.mark.parametrize("kernel", ["valu", "mfma"])
-@pytest.mark.parametrize("tile", [4, 3, 16])
+@pytest.mark.parametrize("tile", [4, 3, 8, 16])
 def test_bsr_spmv(oracle, bsp_host, kernel, tile):
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
-    if kernel == "mfma" and tile != 4:
-        pytest.skip("MFMA kernel is 4x4 only")
-    for name, A in _mats(bsp_host):
+    if kernel == "mfma" and tile not in (4, 8):
+        pytest.skip("MFMA kernels take 4x4 and 8x8 blocks")
+    mats = list(_mats(bsp_host))
+    if tile in (4, 8):  # dense blocks of the kernel's size, odd block-row lengths
+        mats.append((f"blocks-{tile}", bsp_host.blocksparse(300, 257, tile, 25_000, seed=tile)))
+    for name, A in mats:
         if tile == 16 and A.nrows > 200_000:
             continue
         print(f"[bsr-spmv] {name} {tile} {kernel}", flush=True)

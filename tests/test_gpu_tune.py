@@ -18,6 +18,7 @@ def _mats(bsp_host):
     yield "stencil27-64", bsp_host.stencil27_64()
     yield "powerlaw-100k", bsp_host.powerlaw(100_000, 20.0, 1.2, 2000, seed=4)
     yield "blocks-4x4", bsp_host.blocksparse(2000, 2000, 4, 20_000, seed=6)  # BSR4 candidates
+    yield "blocks-8x8", bsp_host.blocksparse(1000, 1000, 8, 8_000, seed=10)  # BSR4 and BSR8
     yield from _degenerate(rng)
 
 
@@ -35,7 +36,8 @@ def test_autotune_parity_and_replay(oracle, bsp_host):
         assert rec in trials and rec["time_us"] == min(t["time_us"] for t in trials)
         formats = {t["format"] for t in trials}
         assert {"csr", "dbsr", "dasp"} <= formats
-        assert ("bsr4" in formats) == (name == "blocks-4x4")
+        assert ("bsr4" in formats) == (name in ("blocks-4x4", "blocks-8x8"))
+        assert ("bsr8" in formats) == (name == "blocks-8x8")
         seen.add(rec["format"])
         y = T(xd)
         torch.cuda.synchronize()
@@ -49,11 +51,11 @@ def test_autotune_parity_and_replay(oracle, bsp_host):
         assert torch.equal(y, y2), name
 
 
-@pytest.mark.parametrize("fmt", ["csr", "dbsr", "dasp", "bsr4"])
+@pytest.mark.parametrize("fmt", ["csr", "dbsr", "dasp", "bsr4", "bsr8"])
 def test_autotune_format_mask(oracle, bsp_host, fmt):
     import torch
     from benchsp_amd.device import DeviceCsr, Tuned
-    A = bsp_host.blocksparse(500, 500, 4, 4000, seed=7)
+    A = bsp_host.blocksparse(250, 250, 8, 2000, seed=7)  # every format qualifies
     dA = DeviceCsr.from_host(A)
     T = Tuned(dA, formats=[fmt], reps=2)
     assert {t["format"] for t in T.trials()} == {fmt} and T.record["format"] == fmt
