@@ -639,15 +639,8 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
     BSP_TRY(Y.alloc(b->row_off[M] * kCols));
     BSP_HIP(hipMemsetAsync(x.get(), 0, sizeof(double) * std::max<int64_t>(1, b->col_off[M]), s));
     BSP_HIP(hipMemsetAsync(X.get(), 0, sizeof(uint16_t) * std::max<int64_t>(1, b->col_off[M] * kCols), s));
-    struct Events {  // destroyed on every exit path
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      ~Events() {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-      }
-    } ev;
-    BSP_HIP(hipEventCreate(&ev.e0));
-    BSP_HIP(hipEventCreate(&ev.e1));
+    EventPair ev;
+    BSP_TRY(ev.create());
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
     bsp_status st = BSP_OK;
     for (size_t m = 0; m < M && st == BSP_OK; ++m) {

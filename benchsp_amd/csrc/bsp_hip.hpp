@@ -92,6 +92,23 @@ bsp_status reduce_sum_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_
 // BSR builder without placing any value.
 bsp_status bsr_count_blocks(const bsp_csr& A, int tm, int tk, hipStream_t s, int64_t* nblocks);
 
+// A pair of timing events, destroyed on every exit path (create() first).
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EventPair() = default;
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  bsp_status create() {
+    BSP_HIP(hipEventCreate(&e0));
+    BSP_HIP(hipEventCreate(&e1));
+    return BSP_OK;
+  }
+};
+
 // Copies one value from device to host (synchronising the stream).
 template <class T>
 bsp_status fetch(const T* dptr, T* host, hipStream_t s) {

@@ -83,17 +83,9 @@ bsp_status run(const bsp_tuned_s* h, const double* x, double* y, hipStream_t s) 
   }
 }
 
-struct Events {  // destroyed on every exit path
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  ~Events() {
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-  }
-};
-
 // Mean time of `reps` SpMVs after the warmups.
 bsp_status time_spmv(const bsp_tuned_s* h, const double* x, double* y, int reps, hipStream_t s,
-                     const Events& ev, double* us) {
+                     const EventPair& ev, double* us) {
   for (int i = 0; i < kWarmups; ++i) BSP_TRY(run(h, x, y, s));
   BSP_HIP(hipEventRecord(ev.e0, s));
   for (int i = 0; i < reps; ++i) BSP_TRY(run(h, x, y, s));
@@ -176,9 +168,8 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
   BSP_TRY(x.alloc(A->ncols));
   BSP_TRY(y.alloc(A->nrows));
   BSP_HIP(hipMemsetAsync(x.get(), 0, sizeof(double) * std::max<int64_t>(1, A->ncols), s));
-  Events ev;
-  BSP_HIP(hipEventCreate(&ev.e0));
-  BSP_HIP(hipEventCreate(&ev.e1));
+  EventPair ev;
+  BSP_TRY(ev.create());
 
   std::unique_ptr<bsp_tuned_s> best;
   std::vector<bsp_tune_record> trials;
