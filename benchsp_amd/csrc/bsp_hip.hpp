@@ -85,7 +85,7 @@ bsp_status sort_pairs_i32(const int32_t* keys_in, int32_t* keys_out, const int32
 // Stable ascending sort of (int64 key, int32 value) pairs on bits [0, end_bit).
 bsp_status sort_pairs_i64key(const int64_t* keys_in, int64_t* keys_out, const int32_t* vals_in,
                              int32_t* vals_out, int64_t n, int end_bit, hipStream_t s);
-// Sum of n int32 into *out (device).
+// Sum of n int64 into *out (device).
 bsp_status reduce_sum_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s);
 
 // Number of occupied tm x tk blocks of a CSR (bsr.hip): the key sort of the
