@@ -562,7 +562,14 @@ def measured_traffic(segmented: bool):
     with open(p) as f:
         d = json.load(f)
     keys = ("csr_adaptive_xcd", "sum_partials") if segmented else ("csr_adaptive_f64<",)
-    return int(sum(v["hbm_bytes"] for k, v in d["launches"].items() if any(q in k for q in keys)))
+    total = 0
+    for q in keys:
+        hits = [v["hbm_bytes"] for k, v in d["launches"].items() if q in k]
+        if len(hits) != 1:  # the measurement must hold the one launch of the SpMV leg
+            raise RuntimeError(f"{p}: {len(hits)} launches match {q!r}; re-run the PMC passes "
+                               "with --no-formats --no-gnn")
+        total += hits[0]
+    return int(total)
 
 
 def roofline(per: dict, method: str) -> dict:
