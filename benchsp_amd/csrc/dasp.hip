@@ -289,8 +289,25 @@ __global__ __launch_bounds__(kTB) void dasp_spmv_f64(DaspArgs a) {
     const int p0 = a.bptr[b], p1 = a.bptr[b + 1];
     // two slabs per step: lane l reads entry l of the 64-entry window,
     // i.e. slab (l/32), row g = (l%32)/4, column t = l%4
-    double acc = 0.0;
-    for (int p = p0 + lane; p < p1; p += kWave) acc += ld_stream(&a.rv[p]) * a.x[ld_stream(&a.rc[p])];
+    // two 64-entry windows per step, both windows' loads in flight (lanes
+    // past the block's end re-read its first entry and add nothing)
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int base = p0; base < p1; base += 2 * kWave) {
+      double v[2], xv[2];
+      int c[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int p = base + u * kWave + lane;
+        const int pc = p < p1 ? p : p0;
+        v[u] = p < p1 ? ld_stream(&a.rv[pc]) : 0.0;
+        c[u] = ld_stream(&a.rc[pc]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) xv[u] = a.x[c[u]];
+      acc0 += v[0] * xv[0];
+      acc1 += v[1] * xv[1];
+    }
+    double acc = acc0 + acc1;
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     acc += __shfl_xor(acc, 32, 64);
