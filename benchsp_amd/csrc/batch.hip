@@ -17,7 +17,10 @@ namespace {
 
 constexpr int kCap = 1024;  // window capacity of adaptive_rows<kCap>
 constexpr int kCols = 8;    // SpMM columns
-constexpr int kKindCsr = 0, kKindBsr = 1;
+constexpr int kKindCsr = 0, kKindBsr4 = 1, kKindBsr8 = 2;
+// A BSR (4x4 or 8x8) is a candidate only where it moves no more bytes than
+// CSR: at least 11/16 of its block entries filled on average.
+constexpr double kBsrFill = 11.0 / 16.0;
 constexpr int kMaxGrid = 2048;  // bound on resident workgroups (8 per CU)
 // Default launch size: this many times the resident workgroups. Short tile
 // ranges let the hardware dispatcher even out what the weighted cuts miss
@@ -36,9 +39,12 @@ struct BMat {
   const int32_t* col;
   const double* val;
   const uint16_t* hval;
-  const int32_t* brp;  // 4x4 BSR (when chosen)
+  const int32_t* brp;  // 4x4 BSR (a candidate or chosen)
   const int32_t* bci;
   const double* bval;
+  const int32_t* brp8;  // 8x8 BSR (a candidate or chosen)
+  const int32_t* bci8;
+  const double* bval8;
   int64_t xoff, yoff;
   int32_t nrows, ncols;
 };
@@ -73,6 +79,40 @@ __device__ __forceinline__ void bsr4_rows(const int32_t* __restrict__ brp,
 #pragma unroll
   for (int o = 4; o < kWave; o <<= 1) acc += __shfl_xor(acc, o, 64);
   const int row = br * 4 + r;
+  if (slot == 0 && row < nrows) y[row] = acc;
+}
+
+// Block rows [br0, br1) (at most four) of an 8x8 BSR, a wave per block row;
+// lane = slot*8 + r, 8 blocks per step, partial sums of equal r added across
+// slots with lane shuffles.
+__device__ __forceinline__ void bsr8_rows(const int32_t* __restrict__ brp,
+                                          const int32_t* __restrict__ bci,
+                                          const double* __restrict__ bval,
+                                          const double* __restrict__ x, double* __restrict__ y,
+                                          int br0, int br1, int nrows, int ncols) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int br = br0 + wave;
+  if (br >= br1) return;
+  const int slot = lane >> 3, r = lane & 7;
+  double acc = 0.0;
+  for (int k = brp[br] + slot, e = brp[br + 1]; k < e; k += 8) {
+    const int c0 = bci[k] * 8;
+    const double* p = bval + (int64_t)k * 64 + r * 8;
+    if (c0 + 8 <= ncols) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 8; c += 2) {
+        s0 += ld_stream(&p[c]) * x[c0 + c];
+        s1 += ld_stream(&p[c + 1]) * x[c0 + c + 1];
+      }
+      acc += s0 + s1;
+    } else {
+      for (int c = 0; c < ncols - c0; ++c) acc += ld_stream(&p[c]) * x[c0 + c];
+    }
+  }
+#pragma unroll
+  for (int o = 8; o < kWave; o <<= 1) acc += __shfl_xor(acc, o, 64);
+  const int row = br * 8 + r;
   if (slot == 0 && row < nrows) y[row] = acc;
 }
 
@@ -192,8 +232,11 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (T.kind == kKindCsr)
       adaptive_rows<kCap, false, false>(d->rowptr, d->col, d->val, x + d->xoff, y + d->yoff,
                                         T.r0, T.p0, T.r1, T.p1);
-    else
+    else if (T.kind == kKindBsr4)
       bsr4_rows(d->brp, d->bci, d->bval, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
+                d->ncols);
+    else
+      bsr8_rows(d->brp8, d->bci8, d->bval8, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
                 d->ncols);
     __syncthreads();
   }
@@ -280,16 +323,17 @@ void csr_tiles(const std::vector<int32_t>& rp, int m, int c, std::vector<BTile>&
   }
 }
 
-void bsr_tiles(const std::vector<int32_t>& brp, int m, std::vector<BTile>& out) {
+// Four block rows (a wave each) per tile; kind kKindBsr4 or kKindBsr8.
+void bsr_tiles(const std::vector<int32_t>& brp, int m, int kind, std::vector<BTile>& out) {
   const int brows = (int)brp.size() - 1;
   for (int b = 0; b < brows; b += kWG / kWave)
-    out.push_back(BTile{m, kKindBsr, b, std::min(brows, b + kWG / kWave), brp[b],
+    out.push_back(BTile{m, kind, b, std::min(brows, b + kWG / kWave), brp[b],
                         brp[std::min(brows, b + kWG / kWave)], 0, 0});
 }
 
 int64_t tile_weight(const BTile& t) {
-  return t.kind == kKindCsr ? (int64_t)(t.r1 - t.r0) + (t.p1 - t.p0) + kTileOverhead
-                            : (int64_t)(t.p1 - t.p0) * 16 + kTileOverhead;
+  if (t.kind == kKindCsr) return (int64_t)(t.r1 - t.r0) + (t.p1 - t.p0) + kTileOverhead;
+  return (int64_t)(t.p1 - t.p0) * (t.kind == kKindBsr4 ? 16 : 64) + kTileOverhead;
 }
 
 // Contiguous per-workgroup tile ranges with balanced weight.
@@ -338,7 +382,7 @@ struct bsp_batch_s {
   std::vector<bsp_csr> A;
   std::vector<int64_t> row_off, col_off;
   std::vector<int32_t> fmt, win_spmv, win_spmm;
-  std::vector<bsp_bsr> bsr;  // owned, per matrix (null unless chosen)
+  std::vector<bsp_bsr> bsr, bsr8;  // owned 4x4 / 8x8 BSR per matrix (null unless chosen)
   DevBuf<uint16_t> hval;     // FP16 copies of every matrix's values
   DevBuf<BMat> mats;
   Schedule spmv, spmm;
@@ -358,6 +402,8 @@ struct bsp_batch_s {
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     for (bsp_bsr h : bsr)
+      if (h) bsp_bsr_destroy(h);
+    for (bsp_bsr h : bsr8)
       if (h) bsp_bsr_destroy(h);
   }
 };
@@ -385,6 +431,13 @@ bsp_status upload_mats(bsp_batch_s* b, hipStream_t s) {
       d.brp = v.rowptr;
       d.bci = v.colidx;
       d.bval = v.val;
+    }
+    if (b->bsr8[m]) {
+      bsp_bsr_view v;
+      bsp_bsr_view_get(b->bsr8[m], &v);
+      d.brp8 = v.rowptr;
+      d.bci8 = v.colidx;
+      d.bval8 = v.val;
     }
     h[m] = d;
     hoff += A.nnz;
@@ -449,6 +502,17 @@ bsp_status time_candidate(const std::vector<BTile>& one, bool spmm, int resident
   return BSP_OK;
 }
 
+// Host copy of a BSR's block-row pointers.
+bsp_status bsr_rowptr(bsp_bsr h, hipStream_t s, std::vector<int32_t>& brp) {
+  bsp_bsr_view v;
+  BSP_TRY(bsp_bsr_view_get(h, &v));
+  brp.resize(v.brows + 1);
+  BSP_HIP(hipMemcpyAsync(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(),
+                         hipMemcpyDeviceToHost, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
 // Tiles of every matrix for one operation: with uniform > 0 every matrix in
 // CSR with that window; otherwise the per-matrix windows (and, for the SpMV,
 // formats).
@@ -457,14 +521,11 @@ bsp_status op_tiles(const bsp_batch_s* b, const std::vector<std::vector<int32_t>
   out.clear();
   for (size_t m = 0; m < b->A.size(); ++m) {
     if (b->A[m].nrows == 0) continue;
-    if (!spmm && uniform == 0 && b->fmt[m] == BSP_BATCH_BSR4) {
-      bsp_bsr_view v;
-      bsp_bsr_view_get(b->bsr[m], &v);
-      std::vector<int32_t> brp(v.brows + 1);
-      BSP_HIP(hipMemcpyAsync(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(),
-                             hipMemcpyDeviceToHost, s));
-      BSP_HIP(hipStreamSynchronize(s));
-      bsr_tiles(brp, (int)m, out);
+    const int f = b->fmt[m];
+    if (!spmm && uniform == 0 && (f == BSP_BATCH_BSR4 || f == BSP_BATCH_BSR8)) {
+      std::vector<int32_t> brp;
+      BSP_TRY(bsr_rowptr(f == BSP_BATCH_BSR4 ? b->bsr[m] : b->bsr8[m], s, brp));
+      bsr_tiles(brp, (int)m, f == BSP_BATCH_BSR4 ? kKindBsr4 : kKindBsr8, out);
     } else {
       const int c = uniform > 0 ? uniform : spmm ? b->win_spmm[m] : b->win_spmv[m];
       csr_tiles(rp[m], (int)m, c, out);
@@ -572,6 +633,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   b->win_spmv.assign(M, kCap);
   b->win_spmm.assign(M, kCapM);
   b->bsr.assign(M, nullptr);
+  b->bsr8.assign(M, nullptr);
   BSP_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   BSP_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   BSP_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
@@ -605,29 +667,32 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   BSP_HIP(hipStreamSynchronize(s));
 
   if (tune == 1 || tune == 2) {
-    // 4x4 BSR is a candidate only where it moves no more bytes than CSR
-    // (>= 11 of 16 block entries filled); the others are freed at once.
+    // block sizes 4 and 8: a BSR is built only where its blocks are filled
+    // enough (kBsrFill); block statistics first (key sort only), so most
+    // matrices are screened out without building anything
     for (size_t m = 0; m < M; ++m) {
       const bsp_csr& A = b->A[m];
       if (A.nnz < 16 * 64) continue;
-      // block statistics first (key sort only): most matrices are screened
-      // out without building their BSR
-      bsp_blockinfo bi;
-      BSP_TRY(bsp_block_info(&A, 4, stream, &bi));
-      if (bi.avg < 11.0) continue;
-      bsp_bsr h = nullptr;
-      BSP_TRY(bsp_bsr_build(&A, 4, 4, stream, &h));
-      bsp_bsr_view v;
-      bsp_bsr_view_get(h, &v);
-      if (v.avg_nnz >= 11.0)
-        b->bsr[m] = h;
-      else
-        bsp_bsr_destroy(h);
+      for (const int bs : {4, 8}) {
+        bsp_blockinfo bi;
+        BSP_TRY(bsp_block_info(&A, bs, stream, &bi));
+        if (bi.avg < kBsrFill * bs * bs) continue;
+        bsp_bsr h = nullptr;
+        BSP_TRY(bsp_bsr_build(&A, bs, bs, stream, &h));
+        (bs == 4 ? b->bsr : b->bsr8)[m] = h;
+      }
     }
   }
-  if (tune == 2)  // every eligible matrix in 4x4 BSR, no timing
-    for (size_t m = 0; m < M; ++m)
-      if (b->bsr[m]) b->fmt[m] = BSP_BATCH_BSR4;
+  if (tune == 2)  // every eligible matrix in BSR (8x8 where eligible), no timing
+    for (size_t m = 0; m < M; ++m) {
+      if (b->bsr8[m]) {
+        b->fmt[m] = BSP_BATCH_BSR8;
+        if (b->bsr[m]) bsp_bsr_destroy(b->bsr[m]);
+        b->bsr[m] = nullptr;
+      } else if (b->bsr[m]) {
+        b->fmt[m] = BSP_BATCH_BSR4;
+      }
+    }
   if (tune == 1) {
     BSP_TRY(upload_mats(b.get(), s));
     DevBuf<double> x, y;
@@ -669,22 +734,26 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       b->win_spmv[m] = best_v < kMargin * def_v ? win_v : kCap;
       b->win_spmm[m] = best_m < kMargin * def_m ? win_m : kCapM;
       best_v = b->win_spmv[m] == kCap ? def_v : best_v;
-      if (st == BSP_OK && b->bsr[m]) {
-        bsp_bsr_view v;
-        bsp_bsr_view_get(b->bsr[m], &v);
-        std::vector<int32_t> brp(v.brows + 1);
-        BSP_HIP(hipMemcpyAsync(brp.data(), v.rowptr, sizeof(int32_t) * brp.size(),
-                               hipMemcpyDeviceToHost, s));
-        BSP_HIP(hipStreamSynchronize(s));
+      // each BSR candidate must beat the best so far by the margin
+      for (const int f : {BSP_BATCH_BSR4, BSP_BATCH_BSR8}) {
+        const bsp_bsr h = f == BSP_BATCH_BSR4 ? b->bsr[m] : b->bsr8[m];
+        if (st != BSP_OK || !h) continue;
+        std::vector<int32_t> brp;
+        st = bsr_rowptr(h, s, brp);
+        if (st != BSP_OK) break;
         std::vector<BTile> one;
-        bsr_tiles(brp, (int)m, one);
+        bsr_tiles(brp, (int)m, f == BSP_BATCH_BSR4 ? kKindBsr4 : kKindBsr8, one);
         float t = 0;
         st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
-        if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = BSP_BATCH_BSR4;
+        if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = f;
       }
       if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR4 && b->bsr[m]) {
         bsp_bsr_destroy(b->bsr[m]);
         b->bsr[m] = nullptr;
+      }
+      if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR8 && b->bsr8[m]) {
+        bsp_bsr_destroy(b->bsr8[m]);
+        b->bsr8[m] = nullptr;
       }
     }
     if (st == BSP_OK) st = upload_mats(b.get(), s);
@@ -703,7 +772,8 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         b->fmt[m] = BSP_BATCH_CSR;
         b->win_spmv[m] = uni_v;
         if (b->bsr[m]) bsp_bsr_destroy(b->bsr[m]);
-        b->bsr[m] = nullptr;
+        if (b->bsr8[m]) bsp_bsr_destroy(b->bsr8[m]);
+        b->bsr[m] = b->bsr8[m] = nullptr;
       }
     }
     if (uni_m > 0) std::fill(b->win_spmm.begin(), b->win_spmm.end(), uni_m);

@@ -1,5 +1,5 @@
 """Batched FP64 SpMV + 8-column FP16 SpMM over many matrices (bsp_batch_*):
-persistent schedule (fixed, autotuned, forced 4x4 BSR), plain schedule and
+persistent schedule (fixed, autotuned, forced BSR), plain schedule and
 graph replays, each matrix against the oracle."""
 import numpy as np
 import pytest
@@ -20,15 +20,16 @@ def _mix(bsp_host):
     rp[1:] = np.cumsum(lens)
     col = np.concatenate([np.sort(rng.choice(2500, L, replace=False)) for L in lens]).astype(np.int32)
     mats.append(Csr(len(lens), 2500, rp, col, rng.uniform(-1, 1, int(rp[-1]))))
-    # dense 4x4 blocks (eligible for the BSR candidate), ragged last block row/col
-    mats.append(bsp_host.blocksparse(40, 37, 4, 600, seed=3))
-    B = mats[-1]
-    keep = B.nrows - 2, B.ncols - 3
-    d = B.to_dense()[: keep[0], : keep[1]]
-    r, c = np.nonzero(d)
-    rp = np.zeros(keep[0] + 1, np.int64)
-    np.add.at(rp, r + 1, 1)
-    mats[-1] = Csr(keep[0], keep[1], np.cumsum(rp), c.astype(np.int32), d[r, c])
+    # dense 8x8 and 4x4 blocks (eligible for the BSR candidates), ragged last
+    # block row and column
+    for bs, nbr, nbc, nb, cut in ((8, 30, 29, 300, (3, 5)), (4, 40, 37, 600, (2, 3))):
+        B = bsp_host.blocksparse(nbr, nbc, bs, nb, seed=bs)
+        keep = B.nrows - cut[0], B.ncols - cut[1]
+        d = B.to_dense()[: keep[0], : keep[1]]
+        r, c = np.nonzero(d)
+        rp = np.zeros(keep[0] + 1, np.int64)
+        np.add.at(rp, r + 1, 1)
+        mats.append(Csr(keep[0], keep[1], np.cumsum(rp), c.astype(np.int32), d[r, c]))
     mats.append(Csr(0, 5, np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0)))
     return mats
 
@@ -74,8 +75,9 @@ def test_batch_persistent(oracle, bsp_host, tune):
     B = Batch(dm, tune=tune)
     ch = B.choice()
     assert set(np.unique(ch["spmv_window"])) <= {128, 256, 512, 1024}
-    if tune == 2:
-        assert ch["format"][-2] == 1  # the dense-block matrix runs as 4x4 BSR
+    if tune == 2:  # the dense-block matrices run as 8x8 and 4x4 BSR
+        assert ch["format"][-3] == 2 and ch["format"][-2] == 1
+        assert np.all(ch["format"][:-3] == 0)
     if tune == 0:
         assert np.all(ch["format"] == 0) and np.all(ch["spmv_window"] == 1024)
     y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
