@@ -11,13 +11,15 @@
 #include <vector>
 
 #include "csr_adaptive.hpp"
+#include "dasp_kernel.hpp"
 
 namespace bsp {
 namespace {
 
 constexpr int kCap = 1024;  // window capacity of adaptive_rows<kCap>
 constexpr int kCols = 8;    // SpMM columns
-constexpr int kKindCsr = 0, kKindBsr4 = 1, kKindBsr8 = 2;
+constexpr int kKindCsr = 0, kKindBsr4 = 1, kKindBsr8 = 2, kKindDasp = 3;
+constexpr int kDaspPerTile = 4;  // workgroups of a matrix's DASP launch per tile
 // A BSR (4x4 or 8x8) is a candidate only where it moves no more bytes than
 // CSR: at least 11/16 of its block entries filled on average.
 constexpr double kBsrFill = 11.0 / 16.0;
@@ -45,6 +47,7 @@ struct BMat {
   const int32_t* brp8;  // 8x8 BSR (a candidate or chosen)
   const int32_t* bci8;
   const double* bval8;
+  const dasp::DaspArgs* dasp;  // DASP layout (a candidate or chosen), x/y unset
   int64_t xoff, yoff;
   int32_t nrows, ncols;
 };
@@ -215,12 +218,26 @@ __device__ __forceinline__ void spmm8_rows(const int32_t* __restrict__ rowptr,
   }
 }
 
+// Workgroups [v0, v1) of a matrix's fused DASP launch. Kept out of line so
+// its register needs (the DASP paths are the batch kernel's largest) stay in
+// this function instead of raising the pressure on the CSR and BSR paths.
+__device__ __attribute__((noinline)) void dasp_tile(const dasp::DaspArgs* a, const double* x,
+                                                    double* y, int v0, int v1) {
+  for (int v = v0; v < v1; ++v) {
+    dasp::dasp_block(*a, x, y, v);
+    __syncthreads();  // the long-row part reuses its LDS
+  }
+}
+
 // Persistent launches: workgroup w (XCD-remapped, so an XCD owns a contiguous
 // run of tiles) runs tiles [wg_start[w], wg_start[w+1]) in order; the cuts
 // balance rows+entries plus a fixed cost per tile. No counters, no atomics:
 // every row of every matrix is written by exactly one tile.
 // (The SpMV kernel is held to 64 VGPRs, eight waves per SIMD; unconstrained
-// it takes 66 and seven.)
+// it takes 66 and seven.) DASP tiles run only in the DASP = true instance:
+// their code would otherwise add call and scratch overhead to every launch
+// (measured: the CSR-only step 0.33 -> 1.23 ms).
+template <bool DASP>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) void batch_spmv_f64(
     const BTile* __restrict__ tiles, const int32_t* __restrict__ wg_start,
     const BMat* __restrict__ mats, const double* __restrict__ x, double* __restrict__ y) {
@@ -235,9 +252,11 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     else if (T.kind == kKindBsr4)
       bsr4_rows(d->brp, d->bci, d->bval, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
                 d->ncols);
-    else
+    else if (T.kind == kKindBsr8)
       bsr8_rows(d->brp8, d->bci8, d->bval8, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
                 d->ncols);
+    else if constexpr (DASP)
+      dasp_tile(d->dasp, x + d->xoff, y + d->yoff, T.r0, T.r1);
     __syncthreads();
   }
 }
@@ -331,8 +350,30 @@ void bsr_tiles(const std::vector<int32_t>& brp, int m, int kind, std::vector<BTi
                         brp[std::min(brows, b + kWG / kWave)], 0, 0});
 }
 
+// Tiles of a DASP layout: its fused launch's workgroups, a long row alone,
+// row blocks and short rows kDaspPerTile workgroups at a time; p0/p1 carry
+// the entries a tile covers (the weight), counted from the long-row and
+// row-block pointers (short rows: at most 4 entries per row).
+void dasp_tiles(const dasp::DaspArgs& a, const std::vector<int32_t>& lrpt,
+                const std::vector<int32_t>& bptr, int m, std::vector<BTile>& out) {
+  for (int i = 0; i < (int)a.wg_long; ++i)
+    out.push_back(BTile{m, kKindDasp, i, i + 1, 0, (lrpt[i + 1] - lrpt[i]) * 64, 0, 0});
+  const int r0 = (int)a.wg_long, r1 = r0 + (int)a.wg_reg, v1 = r1 + (int)a.wg_short;
+  constexpr int kBlocksPerWg = dasp::kTB / kWave;
+  for (int v = r0; v < r1; v += kDaspPerTile) {
+    const int e = std::min(r1, v + kDaspPerTile);
+    const int b0 = (v - r0) * kBlocksPerWg, b1 = std::min(a.blocknum, (e - r0) * kBlocksPerWg);
+    out.push_back(BTile{m, kKindDasp, v, e, 0, bptr[b1] - bptr[b0] + (b1 - b0) * 8, 0, 0});
+  }
+  for (int v = r1; v < v1; v += kDaspPerTile) {
+    const int e = std::min(v1, v + kDaspPerTile);
+    out.push_back(BTile{m, kKindDasp, v, e, 0, (e - v) * dasp::kTB * 4, 0, 0});
+  }
+}
+
 int64_t tile_weight(const BTile& t) {
   if (t.kind == kKindCsr) return (int64_t)(t.r1 - t.r0) + (t.p1 - t.p0) + kTileOverhead;
+  if (t.kind == kKindDasp) return (int64_t)(t.p1 - t.p0) + kTileOverhead;
   return (int64_t)(t.p1 - t.p0) * (t.kind == kKindBsr4 ? 16 : 64) + kTileOverhead;
 }
 
@@ -358,8 +399,10 @@ struct Schedule {
   int grid = 0;
   int resident = kMaxGrid;  // workgroups the device holds at once for the kernel
   int grid_cap = 0;         // workgroups per launch (0: kOversub * resident)
+  bool has_dasp = false;    // some tiles are DASP workgroups
   bsp_status upload(const std::vector<BTile>& t, hipStream_t s) {
     ntiles = (int64_t)t.size();
+    has_dasp = std::any_of(t.begin(), t.end(), [](const BTile& b) { return b.kind == kKindDasp; });
     const int64_t cap = grid_cap > 0 ? grid_cap : (int64_t)kOversub * resident;
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(cap, ntiles));
     std::vector<int32_t> c = wg_cuts(t, grid);
@@ -383,6 +426,8 @@ struct bsp_batch_s {
   std::vector<int64_t> row_off, col_off;
   std::vector<int32_t> fmt, win_spmv, win_spmm;
   std::vector<bsp_bsr> bsr, bsr8;  // owned 4x4 / 8x8 BSR per matrix (null unless chosen)
+  std::vector<bsp_dasp> dasp;      // owned DASP layout per matrix (null unless chosen)
+  DevBuf<dasp::DaspArgs> dargs;    // their SpMV arguments (x, y unset)
   DevBuf<uint16_t> hval;     // FP16 copies of every matrix's values
   DevBuf<BMat> mats;
   Schedule spmv, spmm;
@@ -405,6 +450,8 @@ struct bsp_batch_s {
       if (h) bsp_bsr_destroy(h);
     for (bsp_bsr h : bsr8)
       if (h) bsp_bsr_destroy(h);
+    for (bsp_dasp h : dasp)
+      if (h) bsp_dasp_destroy(h);
   }
 };
 
@@ -413,6 +460,8 @@ namespace {
 bsp_status upload_mats(bsp_batch_s* b, hipStream_t s) {
   const size_t M = b->A.size();
   std::vector<BMat> h(M);
+  std::vector<dasp::DaspArgs> da(M);
+  if (!b->dargs.get()) BSP_TRY(b->dargs.alloc(M));
   int64_t hoff = 0;
   for (size_t m = 0; m < M; ++m) {
     const bsp_csr& A = b->A[m];
@@ -439,18 +488,29 @@ bsp_status upload_mats(bsp_batch_s* b, hipStream_t s) {
       d.bci8 = v.colidx;
       d.bval8 = v.val;
     }
+    if (b->dasp[m]) {
+      bsp_dasp_view v;
+      bsp_dasp_view_get(b->dasp[m], &v);
+      da[m] = dasp::dasp_args(v, nullptr, nullptr, 0);
+      d.dasp = b->dargs.get() + m;
+    }
     h[m] = d;
     hoff += A.nnz;
   }
   BSP_TRY(b->mats.alloc(M));
   BSP_HIP(hipMemcpyAsync(b->mats.get(), h.data(), sizeof(BMat) * M, hipMemcpyHostToDevice, s));
+  BSP_HIP(hipMemcpyAsync(b->dargs.get(), da.data(), sizeof(dasp::DaspArgs) * M,
+                         hipMemcpyHostToDevice, s));
   BSP_HIP(hipStreamSynchronize(s));
   return BSP_OK;
 }
 
 void launch_spmv(const Schedule& S, const BMat* mats, const double* x, double* y, hipStream_t s) {
-  if (S.ntiles)
-    batch_spmv_f64<<<S.grid, kWG, 0, s>>>(S.tiles.get(), S.cuts.get(), mats, x, y);
+  if (S.ntiles == 0) return;
+  if (S.has_dasp)
+    batch_spmv_f64<true><<<S.grid, kWG, 0, s>>>(S.tiles.get(), S.cuts.get(), mats, x, y);
+  else
+    batch_spmv_f64<false><<<S.grid, kWG, 0, s>>>(S.tiles.get(), S.cuts.get(), mats, x, y);
 }
 
 void launch_spmm(const Schedule& S, const BMat* mats, const uint16_t* X, float* Y, hipStream_t s) {
@@ -513,6 +573,21 @@ bsp_status bsr_rowptr(bsp_bsr h, hipStream_t s, std::vector<int32_t>& brp) {
   return BSP_OK;
 }
 
+// DASP tiles of matrix m (host copies of its long-row and row-block pointers).
+bsp_status dasp_matrix_tiles(bsp_dasp h, int m, hipStream_t s, std::vector<BTile>& out) {
+  bsp_dasp_view v;
+  BSP_TRY(bsp_dasp_view_get(h, &v));
+  const dasp::DaspArgs a = dasp::dasp_args(v, nullptr, nullptr, 0);
+  std::vector<int32_t> lrpt(v.row_long + 1), bptr(v.blocknum + 1);
+  BSP_HIP(hipMemcpyAsync(lrpt.data(), v.long_rpt, sizeof(int32_t) * lrpt.size(),
+                         hipMemcpyDeviceToHost, s));
+  BSP_HIP(hipMemcpyAsync(bptr.data(), v.block_ptr, sizeof(int32_t) * bptr.size(),
+                         hipMemcpyDeviceToHost, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  dasp_tiles(a, lrpt, bptr, m, out);
+  return BSP_OK;
+}
+
 // Tiles of every matrix for one operation: with uniform > 0 every matrix in
 // CSR with that window; otherwise the per-matrix windows (and, for the SpMV,
 // formats).
@@ -526,6 +601,8 @@ bsp_status op_tiles(const bsp_batch_s* b, const std::vector<std::vector<int32_t>
       std::vector<int32_t> brp;
       BSP_TRY(bsr_rowptr(f == BSP_BATCH_BSR4 ? b->bsr[m] : b->bsr8[m], s, brp));
       bsr_tiles(brp, (int)m, f == BSP_BATCH_BSR4 ? kKindBsr4 : kKindBsr8, out);
+    } else if (!spmm && uniform == 0 && f == BSP_BATCH_DASP) {
+      BSP_TRY(dasp_matrix_tiles(b->dasp[m], (int)m, s, out));
     } else {
       const int c = uniform > 0 ? uniform : spmm ? b->win_spmm[m] : b->win_spmv[m];
       csr_tiles(rp[m], (int)m, c, out);
@@ -608,7 +685,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   hipStream_t s = (hipStream_t)stream;
   const int tune = opt ? opt->tune : 0;
   const int grid = opt ? opt->grid : 0;
-  BSP_REQUIRE(tune >= 0 && tune <= 2, "tune must be 0, 1 or 2");
+  BSP_REQUIRE(tune >= 0 && tune <= 3, "tune must be 0, 1, 2 or 3");
   BSP_REQUIRE(grid >= 0, "grid must be >= 0");
   auto b = std::make_unique<bsp_batch_s>();
   const size_t M = (size_t)count;
@@ -634,10 +711,11 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   b->win_spmm.assign(M, kCapM);
   b->bsr.assign(M, nullptr);
   b->bsr8.assign(M, nullptr);
+  b->dasp.assign(M, nullptr);
   BSP_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   BSP_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   BSP_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
-  BSP_TRY(resident_grid(batch_spmv_f64, &b->spmv.resident));
+  BSP_TRY(resident_grid(batch_spmv_f64<false>, &b->spmv.resident));
   BSP_TRY(resident_grid(batch_spmm8_f16, &b->spmm.resident));
   b->spmv.grid_cap = b->spmm.grid_cap = grid;
 
@@ -683,6 +761,16 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       }
     }
   }
+  // DASP (threshold 0.75, block_longest 256, the reference's values): a
+  // candidate of every matrix of at least 1024 entries (tune = 1), or the
+  // format of every matrix with rows (tune = 3, no timing)
+  if (tune == 1 || tune == 3)
+    for (size_t m = 0; m < M; ++m) {
+      const bsp_csr& A = b->A[m];
+      if (A.nrows == 0 || (tune == 1 && A.nnz < 16 * 64)) continue;
+      BSP_TRY(bsp_dasp_build(&A, 0.75, 256, stream, &b->dasp[m]));
+      if (tune == 3) b->fmt[m] = BSP_BATCH_DASP;
+    }
   if (tune == 2)  // every eligible matrix in BSR (8x8 where eligible), no timing
     for (size_t m = 0; m < M; ++m) {
       if (b->bsr8[m]) {
@@ -751,9 +839,21 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         bsp_bsr_destroy(b->bsr[m]);
         b->bsr[m] = nullptr;
       }
+      if (st == BSP_OK && b->dasp[m]) {
+        std::vector<BTile> one;
+        st = dasp_matrix_tiles(b->dasp[m], (int)m, s, one);
+        float t = 0;
+        if (st == BSP_OK)
+          st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
+        if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = BSP_BATCH_DASP;
+      }
       if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR8 && b->bsr8[m]) {
         bsp_bsr_destroy(b->bsr8[m]);
         b->bsr8[m] = nullptr;
+      }
+      if (st == BSP_OK && b->fmt[m] != BSP_BATCH_DASP && b->dasp[m]) {
+        bsp_dasp_destroy(b->dasp[m]);
+        b->dasp[m] = nullptr;
       }
     }
     if (st == BSP_OK) st = upload_mats(b.get(), s);
@@ -773,7 +873,9 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         b->win_spmv[m] = uni_v;
         if (b->bsr[m]) bsp_bsr_destroy(b->bsr[m]);
         if (b->bsr8[m]) bsp_bsr_destroy(b->bsr8[m]);
+        if (b->dasp[m]) bsp_dasp_destroy(b->dasp[m]);
         b->bsr[m] = b->bsr8[m] = nullptr;
+        b->dasp[m] = nullptr;
       }
     }
     if (uni_m > 0) std::fill(b->win_spmm.begin(), b->win_spmm.end(), uni_m);

@@ -1,5 +1,5 @@
 """Batched FP64 SpMV + 8-column FP16 SpMM over many matrices (bsp_batch_*):
-persistent schedule (fixed, autotuned, forced BSR), plain schedule and
+persistent schedule (fixed, autotuned, forced BSR, forced DASP), plain schedule and
 graph replays, each matrix against the oracle."""
 import numpy as np
 import pytest
@@ -65,7 +65,7 @@ def _device(mats, xs, Xs):
     return dm, x, X
 
 
-@pytest.mark.parametrize("tune", [0, 1, 2])
+@pytest.mark.parametrize("tune", [0, 1, 2, 3])
 def test_batch_persistent(oracle, bsp_host, tune):
     import torch
     from benchsp_amd.device import Batch
@@ -78,6 +78,8 @@ def test_batch_persistent(oracle, bsp_host, tune):
     if tune == 2:  # the dense-block matrices run as 8x8 and 4x4 BSR
         assert ch["format"][-3] == 2 and ch["format"][-2] == 1
         assert np.all(ch["format"][:-3] == 0)
+    if tune == 3:  # every matrix with rows as DASP tiles
+        assert np.all(ch["format"][:-1] == 3)
     if tune == 0:
         assert np.all(ch["format"] == 0) and np.all(ch["spmv_window"] == 1024)
     y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
