@@ -458,7 +458,8 @@ def batch_leg(args, dev):
            "tuned_over_fixed": round(res["persistent_fixed_graph_ms"] / best, 3),
            "grid": {"fixed": fixed.info()["grid"], "tuned": tuned.info()["grid"]},
            "tuner_choice": {
-               "bsr4": int((ch["format"] == 1).sum()),
+               "formats": {name: int((ch["format"] == code).sum()) for code, name in
+                           enumerate(("csr", "bsr4", "bsr8", "dasp"))},
                "spmv_windows": {int(k): int(v) for k, v in
                                 zip(*np.unique(ch["spmv_window"], return_counts=True))},
                "spmm_windows": {int(k): int(v) for k, v in
