@@ -20,7 +20,6 @@ struct DaspArgs {
   const double* lv;
   const int32_t* lc;
   const int32_t* lrpt;
-  const int32_t* lrow;  // original index of long row i: order[i]
   int32_t nl;
   // row blocks
   const double* rv;
