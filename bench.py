@@ -207,6 +207,7 @@ def spmm_leg(args, dev):
         return e0.elapsed_time(e1) / args.steps  # ms per step
 
     dA = DeviceCsr.from_host(A, dev)
+    A64 = None
     bsr_bytes = A.nnz * 2 * 2  # hv + hvt per copy
     for half in ("float16", "bfloat16"):
         td = getattr(torch, half)
@@ -223,7 +224,19 @@ def spmm_leg(args, dev):
         ms = timed(step, 3)
         out[f"block_{half}_GFLOPs"] = round(3 * flop / (ms * 1e-3) / 1e9, 1)
         out[f"block_{half}_ms_per_step"] = round(ms, 4)
-        del ops, bsrs
+        # forward error against FP64 on the unquantized A (dense FP64 product
+        # on the GPU) with the same rounded B
+        if A64 is None:
+            A64 = torch.zeros(A.nrows, A.ncols, dtype=torch.float64, device=dev)
+            rows = torch.from_numpy(np.repeat(np.arange(A.nrows), np.diff(A.rowptr))).to(dev)
+            A64[rows, torch.from_numpy(A.col.astype(np.int64)).to(dev)] = \
+                torch.from_numpy(A.val).to(dev)
+        ref = A64 @ B.double()
+        C = ops[0].fwd(B).double()
+        out[f"block_{half}_fwd_rel_err_vs_fp64"] = float((C - ref).norm() / ref.norm())
+        del ops, bsrs, ref, C
+    del A64
+    torch.cuda.empty_cache()
     # plain FP32 CSR SpMM on the same matrix and operands
     ncopy = _copies(A.nnz * 8, args.warm)
     pats = [SparsePattern.from_host(A, dev) for _ in range(ncopy)]
