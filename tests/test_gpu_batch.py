@@ -107,15 +107,17 @@ def test_batch_plain_schedule_matches(oracle, bsp_host):
                               oracle.spmv_f64(A.rowptr, A.col, A.val, xs[i]))
 
 
-def test_batch_graph_replays(oracle, bsp_host):
+@pytest.mark.parametrize("tune", [1, 2, 3])
+def test_batch_graph_replays(oracle, bsp_host, tune):
     # the graph reads x and X at replay time: change them between replays,
     # with eager work and synchronisation in between, and check every replay
+    # (tune 2 and 3 put BSR and DASP tiles in the captured launch)
     import torch
     from benchsp_amd.device import Batch
     mats = _mix(bsp_host)
     xs, Xs = _vectors(bsp_host, mats, 41)
     dm, x, X = _device(mats, xs, Xs)
-    B = Batch(dm, tune=1)
+    B = Batch(dm, tune=tune)
     y = torch.zeros(B.nrows, dtype=torch.float64, device="cuda")
     Y = torch.zeros(B.nrows, COLS, dtype=torch.float32, device="cuda")
     B.capture(x, y, X, Y)
