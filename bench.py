@@ -11,13 +11,17 @@ synthetic data from the library's deterministic generators, resident in HBM.
 Each matrix is replicated so the copies together exceed 512 MB and every
 launch reads a copy that is not in the 256 MB Infinity Cache (cold HBM).
 value = sum of CSR-equivalent algorithmic bytes (12*nnz + 4*(n+1) + 8*ncols
-+ 8*n per SpMV) over the timed steps / timed wall time, in GB/s.
++ 8*n per SpMV) over the timed steps / timed wall time, in GB/s. `roofline`
+holds the dominant launch against the nominal 8 TB/s, its measured HBM
+traffic, and (`measured_peaks`) what a plain 1 GiB copy and read reach on
+this box.
 
 SpMM leg: A = 16384^2 with 52,429 dense 16x16 blocks (5%), B and dC
 16384x128 N(0,1). One step = forward C = A*B + backward (dB = A^T*dC and
 dvals = dC*B^T on A's blocks) = 3 * 2*nnz*N FLOP, on FP16 (and BF16) MFMA
 with FP32 accumulation, against the same three products with the library's
-CSR SpMM in FP32.
+CSR SpMM in FP32; each step is replayed from a captured graph, and the
+forward's error against a dense FP64 product is reported.
 
 Formats leg: the same two matrices through every SpMV format and method of
 the library (scalar / balanced / split / adaptive CSR, 4x4 BSR on MFMA and
