@@ -89,6 +89,54 @@ def test_spmv_f16_matches_numpy_emulation(oracle):
     assert np.array_equal(y, expect)
 
 
+def _f32_row_sums(A, v32, x32, mode):
+    """numpy emulation of a row loop over FP32 inputs. mode "ref" is
+    ref.cpp:14-24 (product rounded to FP32, then added, left to right);
+    "fma" contracts each product into the add; "f64" accumulates in FP64 and
+    rounds once. The last two are the plausible mistakes an oracle could make."""
+    y = np.zeros(A.nrows, np.float32)
+    for i in range(A.nrows):
+        s32, s64 = np.float32(0), 0.0
+        for j in range(A.rowptr[i], A.rowptr[i + 1]):
+            p = float(v32[j]) * float(x32[A.col[j]])  # exact: 24+24 bits < 53
+            if mode == "ref":
+                s32 = np.float32(s32 + np.float32(p))
+            elif mode == "fma":
+                s32 = np.float32(float(s32) + p)  # one rounding of s + v*x
+            else:
+                s64 += p
+        y[i] = s64 if mode == "f64" else s32
+    return y
+
+
+def test_spmv_f32_matches_numpy_emulation(oracle):
+    rng = np.random.default_rng(5)
+    A = _rand_csr(rng, rng.integers(0, 30, 200), 90)
+    x = rng.normal(0, 1, 90).astype(np.float32)
+    v32 = A.val.astype(np.float32)
+    y = oracle.spmv_f32(A.rowptr, A.col, v32, x)
+    assert y.dtype == np.float32
+    assert np.array_equal(y, _f32_row_sums(A, v32, x, "ref"))
+
+
+def test_spmv_f32_rounds_each_product_and_each_add(oracle):
+    # row 0: -(1+2^-11)*1 + (1+2^-12)^2. The product is 1 + 2^-11 + 2^-24,
+    # a tie that FP32 rounds to 1 + 2^-11, so the ref loop gives 0; an
+    # FMA-contracted or FP64-accumulated loop gives 2^-24.
+    # row 1: 2^24 + 1 - 2^24 is 0 in FP32 left to right, 1 in FP64.
+    a = np.float32(1 + 2.0**-12)
+    A = Csr(2, 4, np.array([0, 2, 5]), np.array([0, 1, 2, 3, 2], np.int32),
+            np.array([-(1 + 2.0**-11), float(a), 2.0**24, 1.0, -(2.0**24)]))
+    x = np.array([1.0, a, 1.0, 1.0], np.float32)
+    v32 = A.val.astype(np.float32)
+    y = oracle.spmv_f32(A.rowptr, A.col, v32, x)
+    assert np.array_equal(y, np.array([0.0, 0.0], np.float32))
+    assert np.array_equal(_f32_row_sums(A, v32, x, "ref"), y)
+    # the crafted rows tell the two mistaken loops apart from the ref loop
+    assert _f32_row_sums(A, v32, x, "fma")[0] == np.float32(2.0**-24)
+    assert np.array_equal(_f32_row_sums(A, v32, x, "f64"), np.array([2.0**-24, 1.0], np.float32))
+
+
 # ---------------------------------------------------------------- split ----
 
 def test_split_maps(oracle):
