@@ -51,4 +51,28 @@ inline bsp_status check_csr_pattern(const bsp_csr* A) {
   return BSP_OK;
 }
 
+// Full O(rows + nnz) check of a caller's host CSR before anything is
+// uploaded, so that a malformed pattern is an error status instead of an
+// out-of-bounds access on the GPU: sizes non-negative and within kMaxIndex,
+// rowptr[0] == 0, rowptr non-decreasing, rowptr[rows] == nnz, and every
+// column index in [0, cols). Works for int32 and int64 index arrays (an
+// int64 value that passes fits int32).
+template <class RP, class CI>
+bsp_status check_host_csr(const RP* rp, const CI* ci, int64_t rows, int64_t cols, int64_t nnz) {
+  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "negative CSR size");
+  if (!sizes_fit(rows, cols, nnz))
+    return fail(BSP_ERR_TOO_LARGE, "CSR sizes must fit int32 indices (at most 2^31 - 2^20)");
+  BSP_REQUIRE(rp, "null rowptr");
+  BSP_REQUIRE(nnz == 0 || ci, "null column index array");
+  BSP_REQUIRE(rp[0] == 0, "rowptr[0] must be 0");
+  for (int64_t i = 0; i < rows; ++i)
+    if (rp[i + 1] < rp[i])
+      return fail(BSP_ERR_INVALID, "rowptr decreases at row " + std::to_string(i));
+  BSP_REQUIRE((int64_t)rp[rows] == nnz, "rowptr[rows] must equal nnz");
+  for (int64_t j = 0; j < nnz; ++j)
+    if (ci[j] < 0 || (int64_t)ci[j] >= cols)
+      return fail(BSP_ERR_INVALID, "column index out of range at entry " + std::to_string(j));
+  return BSP_OK;
+}
+
 }  // namespace bsp

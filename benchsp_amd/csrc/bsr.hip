@@ -909,6 +909,29 @@ bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out)
   BSP_REQUIRE(m->blcPtr && (m->row == 0 || m->recorderMap) &&
                   (m->blocknnz == 0 || (m->colIdx && m->blcVal && m->blcMap)),
               "null dbsrMat array");
+  // The kernel gathers x through every stored column index (padding too)
+  // and writes y through recorderMap, so both are checked on the host
+  // before anything is uploaded.
+  BSP_REQUIRE(m->col >= 0 && m->nnz >= 0, "negative dbsrMat size");
+  if (!sizes_fit(m->row, m->col, m->nnz) || !sizes_fit(0, 0, m->blocknnz))
+    return fail(BSP_ERR_TOO_LARGE, "dbsrMat sizes must fit int32 indices (at most 2^31 - 2^20)");
+  BSP_REQUIRE(m->blcPtr[0] == 0, "blcPtr[0] must be 0");
+  for (int64_t g = 0; g < m->blc_row; ++g)
+    if (m->blcPtr[g + 1] < m->blcPtr[g])
+      return fail(BSP_ERR_INVALID, "blcPtr decreases at block row " + std::to_string(g));
+  BSP_REQUIRE(m->blcPtr[m->blc_row] == m->blocknnz / 16, "blcPtr[blc_row] must equal blocknnz/16");
+  for (int64_t k = 0; k < m->blocknnz; ++k)
+    if (m->colIdx[k] < 0 || m->colIdx[k] >= m->col)
+      return fail(BSP_ERR_INVALID, "colIdx out of range at slot " + std::to_string(k));
+  {
+    std::vector<char> seen(m->row, 0);
+    for (int64_t i = 0; i < m->row; ++i) {
+      const int32_t r = m->recorderMap[i];
+      if (r < 0 || r >= m->row || seen[r])
+        return fail(BSP_ERR_INVALID, "recorderMap is not a permutation of the rows");
+      seen[r] = 1;
+    }
+  }
   hipStream_t st = (hipStream_t)s;
   auto h = std::make_unique<bsp_dbsr_s>();
   bsp_dbsr_view& v = h->v;

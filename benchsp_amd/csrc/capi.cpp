@@ -46,8 +46,8 @@ struct HostRun {
 
 bsp_status upload_csr(HostRun& h, const int32_t* rowPtr, const int32_t* colIdx, int64_t rows,
                       int64_t cols, int64_t nnz, const double* values, const double* x) {
-  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "negative size");
-  BSP_REQUIRE(rowPtr && (nnz == 0 || (colIdx && values)) && (cols == 0 || x), "null host pointer");
+  BSP_TRY(check_host_csr(rowPtr, colIdx, rows, cols, nnz));
+  BSP_REQUIRE((nnz == 0 || values) && (cols == 0 || x), "null host pointer");
   BSP_TRY(h.rp.alloc(rows + 1));
   BSP_TRY(h.ci.alloc(nnz));
   BSP_TRY(h.v.alloc(nnz));

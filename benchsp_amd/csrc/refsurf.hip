@@ -109,9 +109,8 @@ struct DevCsr {
 
 bsp_status upload_i32(DevCsr& c, const int32_t* rp, const int32_t* ci, const double* v,
                       int64_t rows, int64_t cols, int64_t nnz) {
-  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "bad sizes");
-  if (!sizes_fit(rows, cols, nnz)) return fail(BSP_ERR_TOO_LARGE, "sizes must fit 32-bit indices");
-  BSP_REQUIRE(rp && (nnz == 0 || (ci && v)), "null host pointer");
+  BSP_TRY(check_host_csr(rp, ci, rows, cols, nnz));
+  BSP_REQUIRE(nnz == 0 || v, "null host pointer");
   BSP_TRY(up(c.rp, rp, rows + 1));
   BSP_TRY(up(c.ci, ci, nnz));
   BSP_TRY(up(c.v, v, nnz));
@@ -121,9 +120,9 @@ bsp_status upload_i32(DevCsr& c, const int32_t* rp, const int32_t* ci, const dou
 
 bsp_status upload_i64(DevCsr& c, const int64_t* rp, const int64_t* ci, const double* v,
                       int64_t rows, int64_t cols, int64_t nnz) {
-  BSP_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "bad sizes");
-  if (!sizes_fit(rows, cols, nnz)) return fail(BSP_ERR_TOO_LARGE, "sizes must fit 32-bit indices");
-  BSP_REQUIRE(rp && (nnz == 0 || (ci && v)), "null host pointer");
+  // a pattern that passes has every index below 2^31, so narrowing is exact
+  BSP_TRY(check_host_csr(rp, ci, rows, cols, nnz));
+  BSP_REQUIRE(nnz == 0 || v, "null host pointer");
   DevBuf<int64_t> t;
   BSP_TRY(up(t, rp, rows + 1));
   BSP_TRY(c.rp.alloc(rows + 1));
@@ -471,7 +470,7 @@ void bsp_MySpmvWithPrefp64(int64_t* rowPtr, int64_t* colIdx, int64_t rows, int64
 
 void bsp_MySpmvWithPrefp16(int32_t* rowPtr, int32_t* colIdx, int32_t rows, int32_t cols,
                            int32_t nnz, float* values, float* x, float* outvalues) {
-  if (!rowPtr || rows < 0 || cols < 0 || nnz < 0 || (nnz && (!colIdx || !values)) || !x ||
+  if (check_host_csr(rowPtr, colIdx, rows, cols, nnz) != BSP_OK || (nnz && !values) || !x ||
       !outvalues)
     return;
   DevBuf<int32_t> rp, ci;
@@ -497,8 +496,8 @@ void bsp_MySpmvWithPrefp16(int32_t* rowPtr, int32_t* colIdx, int32_t rows, int32
 
 void bsp_smatSpmmfp16(int* rowPtr, int* colIdx, int row, int col, int n, int nnz, float* values,
                       float* outvalues) {
-  if (!rowPtr || row < 0 || col < 0 || nnz < 0 || n <= 0 || n % 32 != 0 || !outvalues ||
-      (nnz && (!colIdx || !values)))
+  if (check_host_csr(rowPtr, colIdx, row, col, nnz) != BSP_OK || n <= 0 || n % 32 != 0 ||
+      !outvalues || (nnz && !values))
     return;
   DevCsr c;
   DevBuf<float> v32;
@@ -536,9 +535,8 @@ void bsp_smatSpmmfp16(int* rowPtr, int* colIdx, int row, int col, int n, int nnz
 
 int bsp_cusparse_spmv(int* hA_csrOffsets, int64_t* hA_columns, float* hA_values, float* hX,
                       float* hY, int64_t A_num_rows, int64_t A_num_cols, int64_t A_nnz) {
-  if (!hA_csrOffsets || A_num_rows < 0 || A_num_cols < 0 || A_nnz < 0 ||
-      !sizes_fit(A_num_rows, A_num_cols, A_nnz) ||
-      (A_nnz && (!hA_columns || !hA_values)) || !hX || !hY)
+  if (check_host_csr(hA_csrOffsets, hA_columns, A_num_rows, A_num_cols, A_nnz) != BSP_OK ||
+      (A_nnz && !hA_values) || !hX || !hY)
     return 1;
   DevBuf<int32_t> rp, ci;
   DevBuf<int64_t> c64;
@@ -560,8 +558,8 @@ int bsp_cusparse_spmv(int* hA_csrOffsets, int64_t* hA_columns, float* hA_values,
 int bsp_cusparse_spmv_fp16(int* hA_csrOffsets, int32_t* hA_columns, const uint16_t* hA_values,
                            const uint16_t* hX, uint16_t* hY, int32_t A_num_rows,
                            int32_t A_num_cols, int32_t A_nnz) {
-  if (!hA_csrOffsets || A_num_rows < 0 || A_num_cols < 0 || A_nnz < 0 ||
-      (A_nnz && (!hA_columns || !hA_values)) || !hX || !hY)
+  if (check_host_csr(hA_csrOffsets, hA_columns, A_num_rows, A_num_cols, A_nnz) != BSP_OK ||
+      (A_nnz && !hA_values) || !hX || !hY)
     return 1;
   DevBuf<int32_t> rp, ci;
   DevBuf<uint16_t> v, x, yh;
@@ -583,7 +581,8 @@ int bsp_cusparse_spmv_fp16(int* hA_csrOffsets, int32_t* hA_columns, const uint16
 void bsp_cusparse_spmv_fp16_warpper(int* hA_csrOffsets, int32_t* hA_columns, float* hA_values32,
                                     float* hX32, float* hY32, int32_t A_num_rows,
                                     int32_t A_num_cols, int32_t A_nnz) {
-  if (A_nnz < 0 || A_num_cols < 0 || A_num_rows < 0 || !hX32 || !hY32 || (A_nnz && !hA_values32))
+  if (check_host_csr(hA_csrOffsets, hA_columns, A_num_rows, A_num_cols, A_nnz) != BSP_OK ||
+      !hX32 || !hY32 || (A_nnz && !hA_values32))
     return;
   std::vector<uint16_t> hv(A_nnz), hx(A_num_cols), hy(A_num_rows);
   DevBuf<float> t32;
@@ -628,7 +627,7 @@ static void ref_rows_f32(bool half, int* rowptr, int32_t* cols, float* vals, flo
                          int32_t row, int32_t col) {
   if (!rowptr || row < 0 || col < 0 || !x || !y) return;
   const int64_t nnz = rowptr[row];
-  if (nnz < 0 || (nnz && (!cols || !vals))) return;
+  if (check_host_csr(rowptr, cols, row, col, nnz) != BSP_OK || (nnz && !vals)) return;
   DevBuf<int32_t> rp, ci;
   DevBuf<float> v, xd, yd;
   if (up(rp, rowptr, (int64_t)row + 1) != BSP_OK || up(ci, cols, nnz) != BSP_OK ||

@@ -74,6 +74,113 @@ def test_reference_wrapper_rejects_null_without_exiting():
     assert lib().bsp_spmv_fp64_warpper(None, None, 4, 4, 4, None, None, None, 1, C.byref(t)) == 1
 
 
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# Malformed host patterns: (rowptr, col, ncols, message fragment). nrows is 3
+# and nnz 4 in every case; each must be refused before any upload.
+_BAD_CSR = [
+    ([1, 2, 3, 4], [0, 1, 2, 3], 4, "rowptr[0]"),
+    ([0, 3, 2, 4], [0, 1, 2, 3], 4, "decreases"),
+    ([0, 1, 2, 3], [0, 1, 2, 3], 4, "equal nnz"),
+    ([0, 1, 2, 4], [0, 1, 4, 3], 4, "out of range"),
+    ([0, 1, 2, 4], [0, -1, 2, 3], 4, "out of range"),
+]
+
+
+def _last_error():
+    from benchsp_amd._lib import lib
+    return lib().bsp_last_error().decode()
+
+
+@pytest.mark.parametrize("rp,col,ncols,msg", _BAD_CSR)
+def test_host_csr_rejected_before_upload_int32(rp, col, ncols, msg):
+    from benchsp_amd._lib import lib
+    L = lib()
+    rp32, c32 = np.array(rp, np.int32), np.array(col, np.int32)
+    v, x, y = np.ones(4), np.ones(ncols), np.zeros(3)
+    t = C.c_double(0)
+    for fn in (L.bsp_spmv_fp64_warpper, L.bsp_spmv_fp64_balance_warpper):
+        assert fn(_p(rp32), _p(c32), 3, ncols, 4, _p(v), _p(x), _p(y), 1, C.byref(t)) == 1
+        assert msg in _last_error()
+    assert L.bsp_amgT_spmv_fp64(_p(rp32), _p(c32), _p(v), _p(x), _p(y), 3, ncols, 4, 1,
+                                C.byref(t)) == 1
+    assert msg in _last_error()
+    v32, x32, y32 = np.ones(4, np.float32), np.ones(ncols, np.float32), np.zeros(3, np.float32)
+    L.bsp_ref_spmv_fp32(_p(rp32), _p(c32), _p(v32), _p(x32), _p(y32), 3, ncols)
+    assert not y32.any()
+    L.bsp_MySpmvWithPrefp16(_p(rp32), _p(c32), 3, ncols, 4, _p(v32), _p(x32), _p(y32))
+    assert not y32.any() and msg in _last_error()
+    from benchsp_amd._lib import CSRFormatF64, DbsrMat
+    csr = CSRFormatF64(_p(rp32), _p(c32), _p(v), 3, ncols, 4)
+    out = DbsrMat()
+    assert L.bsp_csrTodbsrWithRecorder(C.byref(csr), C.byref(out)) == 1
+    assert msg in _last_error()
+
+
+@pytest.mark.parametrize("rp,col,ncols,msg", _BAD_CSR)
+def test_host_csr_rejected_before_upload_int64(rp, col, ncols, msg):
+    from benchsp_amd._lib import lib
+    L = lib()
+    rp64, c64 = np.array(rp, np.int64), np.array(col, np.int64)
+    v, x, y = np.ones(4), np.ones(ncols), np.zeros(3)
+    t = C.c_double(0)
+    assert L.bsp_cusparse_spmv_fp64(_p(rp64), _p(c64), _p(v), _p(x), _p(y), 3, ncols, 4, 1,
+                                    C.byref(t)) == 1
+    assert msg in _last_error()
+    L.bsp_MySpmvWithPrefp64(_p(rp64), _p(c64), 3, ncols, 4, _p(v), _p(x), _p(y))
+    assert not y.any() and msg in _last_error()
+    L.bsp_bcsr_spmv_fp64(_p(rp64), _p(c64), _p(v), _p(x), _p(y), 3, ncols, 4, 1)
+    assert not y.any() and msg in _last_error()
+
+
+def test_int64_index_that_does_not_fit_int32_is_rejected():
+    from benchsp_amd._lib import lib
+    L = lib()
+    rp = np.array([0, 1, 2], np.int64)
+    col = np.array([0, (1 << 32) + 1], np.int64)  # would narrow to 1
+    v, x, y = np.ones(2), np.ones(4), np.zeros(2)
+    t = C.c_double(0)
+    assert L.bsp_cusparse_spmv_fp64(_p(rp), _p(col), _p(v), _p(x), _p(y), 2, 4, 2, 1,
+                                    C.byref(t)) == 1
+    assert "out of range" in _last_error()
+
+
+def _dbsr_host(recorder, blcptr, colidx, row=5, col=6):
+    from benchsp_amd._lib import DbsrMat
+    keep = [np.array(recorder, np.int32), np.array(blcptr, np.int32),
+            np.array(colidx, np.int32)]
+    nb = len(colidx) // 16
+    keep += [np.zeros(16 * nb), np.zeros(nb, np.uint16), np.zeros(2, np.int32)]
+    m = DbsrMat()
+    m.row, m.col, m.nnz, m.blc_row, m.blc_num = row, col, 0, (row + 3) // 4, nb
+    m.blocknnz, m.warpnum = 16 * nb, 1
+    m.recorderMap, m.blcPtr, m.colIdx, m.blcVal, m.blcMap, m.prefix = map(_p, keep)
+    return m, keep
+
+
+@pytest.mark.parametrize("recorder,blcptr,colidx,msg", [
+    ([0, 1, 2, 3, 3], [0, 1, 2], [0] * 32, "permutation"),
+    ([0, 1, 2, 3, 5], [0, 1, 2], [0] * 32, "permutation"),
+    ([4, 3, 2, 1, 0], [0, 2, 1], [0] * 32, "decreases"),
+    ([4, 3, 2, 1, 0], [0, 1, 1], [0] * 32, "blocknnz"),
+    ([4, 3, 2, 1, 0], [0, 1, 2], [0] * 31 + [6], "colIdx"),
+    ([4, 3, 2, 1, 0], [1, 1, 2], [0] * 32, "blcPtr[0]"),
+])
+def test_host_dbsr_rejected_before_upload(recorder, blcptr, colidx, msg):
+    from benchsp_amd._lib import BspError, lib
+    L = lib()
+    m, keep = _dbsr_host(recorder, blcptr, colidx)
+    h = C.c_void_p()
+    with pytest.raises(BspError) as e:
+        L.bsp_dbsr_from_host(C.byref(m), None, C.byref(h))
+    assert e.value.status == 1 and msg in str(e.value)
+    x, y, t = np.ones(6), np.zeros(5), C.c_double(0)
+    assert L.bsp_amgT_spmv_fp64_dbsr(C.byref(m), _p(x), _p(y), 1, 1, C.byref(t)) == 1
+    assert msg in _last_error()
+
+
 def test_generator_sizes(bsp_host):
     A = bsp_host.poisson5_256()
     assert (A.nrows, A.nnz) == (65536, 326656)
