@@ -835,10 +835,6 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = f;
       }
-      if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR4 && b->bsr[m]) {
-        bsp_bsr_destroy(b->bsr[m]);
-        b->bsr[m] = nullptr;
-      }
       if (st == BSP_OK && b->dasp[m]) {
         std::vector<BTile> one;
         st = dasp_matrix_tiles(b->dasp[m], (int)m, s, one);
@@ -846,6 +842,11 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         if (st == BSP_OK)
           st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = BSP_BATCH_DASP;
+      }
+      // only the chosen format's layout is kept (decided after DASP's turn)
+      if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR4 && b->bsr[m]) {
+        bsp_bsr_destroy(b->bsr[m]);
+        b->bsr[m] = nullptr;
       }
       if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR8 && b->bsr8[m]) {
         bsp_bsr_destroy(b->bsr8[m]);

@@ -67,6 +67,8 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // work-items: `launch(first, count)` per slice, one slice at every size that
 // fits (a wave per row stays one dispatch up to 2^25 rows).
 constexpr int64_t kSliceThreads = int64_t(1) << 31;
+// Work-items one dispatch may hold (a launch issued whole must stay below).
+constexpr int64_t kDispatchLimit = int64_t(1) << 32;
 template <class Launch>
 inline void in_slices(int64_t items, int64_t threads_per_item, Launch&& launch) {
   const int64_t per = kSliceThreads / threads_per_item;

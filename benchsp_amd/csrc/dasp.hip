@@ -343,6 +343,10 @@ static bsp_status build_dasp(const bsp_csr& A, double threshold, int32_t block_l
   BSP_HIP(hipMemsetAsync(h->long_val.get(), 0, sizeof(double) * (v.fill0_nnz_long ? v.fill0_nnz_long : 1), s));
   BSP_HIP(hipMemsetAsync(h->long_cid.get(), 0, sizeof(int32_t) * (v.fill0_nnz_long ? v.fill0_nnz_long : 1), s));
   BSP_HIP(hipMemsetAsync(h->rid_by_warp.get(), 0, sizeof(int32_t) * (v.warp_number ? v.warp_number : 1), s));
+  // long_fill and the fused SpMV are single dispatches: each must hold
+  // fewer than 2^32 work-items
+  if ((int64_t)nl * kWave >= kDispatchLimit)
+    return fail(BSP_ERR_TOO_LARGE, "DASP: too many long rows for one dispatch");
   if (nl > 0)
     long_fill<<<cdiv((int64_t)nl * kWave, kTB), kTB, 0, s>>>(
         A.rowptr, A.colidx, aval, lists[CLONG].get(), nl, h->long_rpt.get(), h->long_val.get(),
@@ -405,6 +409,11 @@ static bsp_status build_dasp(const bsp_csr& A, double threshold, int32_t block_l
   v.reg_cid = h->reg_cid.get();
   v.irreg_val = h->irreg_val.get();
   v.irreg_cid = h->irreg_cid.get();
+  {
+    const dasp::DaspArgs a = dasp::dasp_args(v, nullptr, nullptr, 0);
+    if ((a.wg_long + a.wg_reg + a.wg_short) * kTB >= kDispatchLimit)
+      return fail(BSP_ERR_TOO_LARGE, "DASP: the fused SpMV launch would exceed one dispatch");
+  }
   return BSP_OK;
 }
 
