@@ -87,6 +87,31 @@ def test_bsr_spmv(oracle, bsp_host, kernel, tile):
                           abs_spmv(oracle, A, x), what=f"{name} bsr{tile} {kernel}")
 
 
+def test_bsr4_builder_bit_exact_powerlaw_1m(oracle, bsp_host):
+    # the 4x4 BSR of the power-law 2^20 matrix (about one entry per block)
+    from benchsp_amd.device import Bsr, DeviceCsr
+    A = bsp_host.powerlaw_1m()
+    g = Bsr(DeviceCsr.from_host(A), 4, 4).arrays()
+    o = oracle.bcsr(A.nrows, A.ncols, A.rowptr, A.col, A.val, 4, 4)
+    assert g["nblocks"] == o["nblocks"]
+    for k in ("rowptr", "colidx", "val", "map", "rowptr_by_warp", "rowidx_by_warp"):
+        _same(g[k], o[k], f"powerlaw-1M {k}")
+    assert g["warpnum"] == o["warpnum"] and g["avg_nnz"] == o["avg_nnz"]
+
+
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+def test_bsr4_spmv_powerlaw_1m(oracle, bsp_host, kernel):
+    import torch
+    from benchsp_amd.device import Bsr, DeviceCsr
+    A = bsp_host.powerlaw_1m()
+    x = bsp_host.uniform(A.ncols, 22)
+    B = Bsr(DeviceCsr.from_host(A), 4, 4)
+    y = B(torch.from_numpy(x).cuda(), kernel=Bsr.VALU if kernel == "valu" else Bsr.MFMA)
+    torch.cuda.synchronize()
+    assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                      abs_spmv(oracle, A, x), what=f"powerlaw-1M bsr4 {kernel}")
+
+
 def _dbsr_inputs(bsp_host):
     for p in sorted(glob.glob(os.path.join(GOLDEN, "dbsr_*.npz"))):
         g = np.load(p)

@@ -88,6 +88,56 @@ def test_autograd_function_matches_fp64_and_fd(oracle, bsp_host):
         assert abs(fd - gv[j]) <= 1e-4 * (1 + abs(fd))
 
 
+# BF16 inputs, FP32 accumulation, gradients returned in BF16: one rounding
+# of the output to BF16 (at most 2^-9 relative, stated with a factor-2
+# margin) plus the FP32 accumulation of 64 products (64 * 2^-24 of the sum
+# of their magnitudes, stated as 1e-5).
+BF16_FD_REL = 2.0**-8
+BF16_FD_ABS = 1e-5
+
+
+def test_autograd_bf16_matches_fd(oracle, bsp_host):
+    import torch
+    from benchsp_amd.autograd import spmm
+    A = bsp_host.graph(2000, 25.0, 2.1, False, seed=3)
+    P, tv, tX, tdY, _, _ = _setup(A, 64, torch.bfloat16, seed=6)
+    tv.requires_grad_(True)
+    tX.requires_grad_(True)
+    Y = spmm(tv, tX, P)
+    (Y * tdY.float()).sum().backward()
+    assert tv.grad.dtype == torch.bfloat16 and tX.grad.dtype == torch.bfloat16
+    gv = tv.grad.float().cpu().numpy().astype(np.float64)
+    gX = tX.grad.float().cpu().numpy().astype(np.float64)
+    # the operands the kernels saw, exactly, in FP64
+    v64 = tv.detach().float().cpu().numpy().astype(np.float64)
+    X64 = tX.detach().float().cpu().numpy().astype(np.float64)
+    dY64 = tdY.float().cpu().numpy().astype(np.float64)
+    assert rel_fro(Y.detach().cpu().numpy(), oracle.csr_spmm_f64(A.rowptr, A.col, v64, X64)) <= 1e-5
+    eps = 1e-3
+    rng = np.random.default_rng(2)
+    rows = np.repeat(np.arange(A.nrows), A.row_lengths())
+    # dvals: L = <dY, A X> restricted to row r, the only row val_j enters
+    for j in rng.choice(A.nnz, 64, replace=False):
+        r = rows[j]
+        s, e = A.rowptr[r], A.rowptr[r + 1]
+        def row_loss(v):
+            return float(dY64[r] @ (v[s:e] @ X64[A.col[s:e]]))
+        vp, vm = v64.copy(), v64.copy()
+        vp[j] += eps
+        vm[j] -= eps
+        fd = (row_loss(vp) - row_loss(vm)) / (2 * eps)
+        mag = float(np.abs(dY64[r]) @ np.abs(X64[A.col[j]]))
+        assert abs(fd - gv[j]) <= BF16_FD_REL * abs(fd) + BF16_FD_ABS * mag, (j, fd, gv[j])
+    # dX: L restricted to the entries of column c, the only ones X[c, f] enters
+    for c, f in zip(rng.choice(A.ncols, 32, replace=False), rng.integers(0, 64, 32)):
+        js = np.nonzero(A.col == c)[0]
+        def col_loss(xcf):
+            return float(np.sum(v64[js] * dY64[rows[js], f] * xcf))
+        fd = (col_loss(X64[c, f] + eps) - col_loss(X64[c, f] - eps)) / (2 * eps)
+        mag = float(np.sum(np.abs(v64[js] * dY64[rows[js], f])))
+        assert abs(fd - gX[c, f]) <= BF16_FD_REL * abs(fd) + BF16_FD_ABS * mag, (c, f, fd)
+
+
 @pytest.mark.parametrize("dtype_name", ["float32", "bfloat16"])
 def test_spmm_gnn_200k(oracle, bsp_host, dtype_name):
     import torch
