@@ -321,8 +321,11 @@ static int cmp_len_idx(const void* a, const void* b) {
   return x[1] < y[1] ? -1 : (x[1] > y[1]);
 }
 
-orc_dbsr* orc_dbsr_build(int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* col,
-                         const double* val) {
+/* sort_rows = 1: csrTodbsrWithRecorder (dbsr.h:208-253), rows sorted first.
+ * sort_rows = 0: csrTodbsr (dbsr.h:10-93) on the CSR as given, then
+ * balanceDbsr (:94-169); recorder is then the identity. */
+orc_dbsr* orc_dbsr_build_ex(int64_t nrows, int64_t ncols, const int64_t* rowptr,
+                            const int32_t* col, const double* val, int sort_rows) {
   orc_dbsr* d = (orc_dbsr*)calloc(1, sizeof(orc_dbsr));
   d->nrows = nrows;
   d->ncols = ncols;
@@ -330,10 +333,10 @@ orc_dbsr* orc_dbsr_build(int64_t nrows, int64_t ncols, const int64_t* rowptr, co
   /* dbsr.h:213-226: rows sorted by (len, index) ascending; recorder[new] = old */
   int64_t* key = (int64_t*)malloc(sizeof(int64_t) * 2 * (nrows ? nrows : 1));
   for (int64_t i = 0; i < nrows; i++) {
-    key[2 * i] = rowptr[i + 1] - rowptr[i];
+    key[2 * i] = sort_rows ? rowptr[i + 1] - rowptr[i] : 0;
     key[2 * i + 1] = i;
   }
-  qsort(key, (size_t)nrows, 2 * sizeof(int64_t), cmp_len_idx);
+  if (sort_rows) qsort(key, (size_t)nrows, 2 * sizeof(int64_t), cmp_len_idx);
   d->recorder = (int32_t*)malloc(sizeof(int32_t) * (nrows ? nrows : 1));
   for (int64_t i = 0; i < nrows; i++) d->recorder[i] = (int32_t)key[2 * i + 1];
   free(key);
@@ -414,6 +417,11 @@ orc_dbsr* orc_dbsr_build(int64_t nrows, int64_t ncols, const int64_t* rowptr, co
   for (int w = 0; w < warpcount; w++) d->prefix[w + 1] = d->prefix[w] + cnt[w];
   free(cnt);
   return d;
+}
+
+orc_dbsr* orc_dbsr_build(int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* col,
+                         const double* val) {
+  return orc_dbsr_build_ex(nrows, ncols, rowptr, col, val, 1);
 }
 
 void orc_dbsr_free(orc_dbsr* d) {

@@ -51,6 +51,8 @@ def lib():
         L.orc_bcsr_free.argtypes = [C.POINTER(_Bcsr)]
         L.orc_dbsr_build.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _f64p]
         L.orc_dbsr_build.restype = C.POINTER(_Dbsr)
+        L.orc_dbsr_build_ex.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _f64p, C.c_int]
+        L.orc_dbsr_build_ex.restype = C.POINTER(_Dbsr)
         L.orc_dbsr_free.argtypes = [C.POINTER(_Dbsr)]
         L.orc_dasp_build.argtypes = [C.c_int32, _i32p, _i32p, _f64p, C.c_double, C.c_int32]
         L.orc_dasp_build.restype = C.POINTER(_Dasp)
@@ -196,9 +198,11 @@ def bcsr(nrows, ncols, rowptr, col, val, tm: int, tk: int):
     return out
 
 
-def dbsr(nrows, ncols, rowptr, col, val):
-    """DBSR with recorder and balance (include/dbsr.h:10-253)."""
-    p = lib().orc_dbsr_build(int(nrows), int(ncols), _c64(rowptr), _c32(col), _cf64(val))
+def dbsr(nrows, ncols, rowptr, col, val, sort_rows: bool = True):
+    """DBSR with recorder and balance (include/dbsr.h:10-253). sort_rows=False
+    is csrTodbsr + balanceDbsr on the rows as given (recorder = identity)."""
+    p = lib().orc_dbsr_build_ex(int(nrows), int(ncols), _c64(rowptr), _c32(col), _cf64(val),
+                                int(bool(sort_rows)))
     d = p.contents
     nb = d.nblocks
     out = dict(brows=d.brows, nblocks=nb,
@@ -272,25 +276,50 @@ def ref_available() -> bool:
     return os.path.exists(_REF)
 
 
-def ref_dbsr(nrows, ncols, rowptr, col, val):
-    """csrTodbsrWithRecorder from the reference header (small inputs only:
-    the arrays it allocates are not freed)."""
+def _ref_lib():
     global _ref
     if _ref is None:
         _ref = C.CDLL(_REF)
-        _ref.ref_csr_to_dbsr.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p, _f64p,
-                                         C.POINTER(_RefDbsr)]
+        for name in ("ref_csr_to_dbsr", "ref_csr_to_dbsr_plain"):
+            getattr(_ref, name).argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p, _f64p,
+                                            C.POINTER(_RefDbsr)]
+        pi, pd = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        _ref.ref_csr_recorder_self.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p,
+                                               _f64p, C.POINTER(pi), C.POINTER(pi),
+                                               C.POINTER(pd)]
+    return _ref
+
+
+def ref_dbsr(nrows, ncols, rowptr, col, val, sort_rows: bool = True):
+    """csrTodbsrWithRecorder from the reference header, or (sort_rows=False)
+    its csrTodbsr followed by balanceDbsr (small inputs only: the arrays it
+    allocates are not freed)."""
     rp, ci, v = _c32(rowptr), _c32(col), _cf64(val)
     o = _RefDbsr()
-    _ref.ref_csr_to_dbsr(int(nrows), int(ncols), int(rp[-1]), rp, ci, v, C.byref(o))
+    fn = _ref_lib().ref_csr_to_dbsr if sort_rows else _ref_lib().ref_csr_to_dbsr_plain
+    fn(int(nrows), int(ncols), int(rp[-1]), rp, ci, v, C.byref(o))
     nb = o.nblocks
+    rec = (_arr(o.recorderMap, nrows, np.int32) if sort_rows
+           else np.arange(nrows, dtype=np.int32))
     return dict(brows=o.brows, nblocks=nb, warpnum=o.warpnum,
-                recorder=_arr(o.recorderMap, nrows, np.int32),
+                recorder=rec,
                 blcptr=_arr(o.blcPtr, o.brows + 1, np.int32),
                 colidx=_arr(o.colIdx, nb * 16, np.int32),
                 val=_arr(o.blcVal, nb * 16, np.float64),
                 map=_arr(o.blcMap, nb, np.uint16),
                 prefix=_arr(o.prefix, o.warpnum + 1, np.int32))
+
+
+def ref_recorder_self(nrows, ncols, rowptr, col, val):
+    """csrRecorderSelf from the reference header: (rowptr, col, val) of the
+    CSR with rows sorted by (length, index)."""
+    rp, ci, v = _c32(rowptr), _c32(col), _cf64(val)
+    orp, oci = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+    ov = C.POINTER(C.c_double)()
+    _ref_lib().ref_csr_recorder_self(int(nrows), int(ncols), int(rp[-1]), rp, ci, v,
+                                     C.byref(orp), C.byref(oci), C.byref(ov))
+    nnz = int(rp[-1])
+    return (_arr(orp, nrows + 1, np.int32), _arr(oci, nnz, np.int32), _arr(ov, nnz, np.float64))
 
 
 # --------------------------------------------------------------------------

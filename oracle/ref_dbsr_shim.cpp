@@ -46,3 +46,57 @@ int ref_csr_to_dbsr(int32_t row, int32_t col, int32_t nnz, int32_t* rowPtr, int3
   return 0;
 }
 }
+
+extern "C" {
+// Runs csrTodbsr (dbsr.h:10-93) and then balanceDbsr (:94-169) on the CSR as
+// given, without the row sort of csrTodbsrWithRecorder.
+int ref_csr_to_dbsr_plain(int32_t row, int32_t col, int32_t nnz, int32_t* rowPtr,
+                          int32_t* colIdx, double* values, ref_dbsr_out* out) {
+  CSRFormat<double, int32_t> csr;
+  csr.row = row;
+  csr.col = col;
+  csr.nnz = nnz;
+  csr.rowPtr = rowPtr;
+  csr.colIdx = colIdx;
+  csr.values = values;
+  dbsrMat d;
+  d.recorderMap = nullptr;
+  fflush(stdout);
+  FILE* saved = stdout;
+  FILE* devnull = fopen("/dev/null", "w");
+  if (devnull) stdout = devnull;
+  csrTodbsr(&csr, &d);
+  balanceDbsr(&d);
+  if (devnull) { fflush(stdout); stdout = saved; fclose(devnull); }
+  out->brows = d.blc_row;
+  out->blocknnz = d.blocknnz;
+  out->nblocks = d.blocknnz / 16;
+  out->warpnum = d.warpnum;
+  out->blcPtr = d.blcPtr;
+  out->colIdx = d.colIdx;
+  out->blcVal = d.blcVal;
+  out->blcMap = d.blcMap;
+  out->recorderMap = nullptr;
+  out->prefix = d.prefix;
+  return 0;
+}
+
+// Runs csrRecorderSelf (dbsr.h:170-206): the CSR with rows sorted by
+// (length, index); arrays handed back as the reference allocates them.
+int ref_csr_recorder_self(int32_t row, int32_t col, int32_t nnz, int32_t* rowPtr,
+                          int32_t* colIdx, double* values, int32_t** outRowPtr,
+                          int32_t** outColIdx, double** outValues) {
+  CSRFormat<double, int32_t> csr, rec;
+  csr.row = row;
+  csr.col = col;
+  csr.nnz = nnz;
+  csr.rowPtr = rowPtr;
+  csr.colIdx = colIdx;
+  csr.values = values;
+  csrRecorderSelf(&csr, &rec);
+  *outRowPtr = rec.rowPtr;
+  *outColIdx = rec.colIdx;
+  *outValues = rec.values;
+  return 0;
+}
+}

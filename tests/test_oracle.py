@@ -245,6 +245,45 @@ def test_dbsr_matches_compiled_reference(oracle):
     assert checked >= 20
 
 
+def test_dbsr_unsorted_matches_compiled_reference(oracle):
+    # csrTodbsr + balanceDbsr on the rows as given (no recorder sort)
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(12)
+    checked = 0
+    for trial in range(40):
+        n = int(rng.integers(1, 120))
+        lens = rng.integers(0, int(rng.choice([6, 30, 300])), n)
+        A = _rand_csr(rng, lens, 700)
+        d = oracle.dbsr(A.nrows, A.ncols, A.rowptr, A.col, A.val, sort_rows=False)
+        assert np.array_equal(d["recorder"], np.arange(A.nrows))
+        if d["overflow"]:
+            continue  # undefined in the reference
+        r = oracle.ref_dbsr(A.nrows, A.ncols, A.rowptr, A.col, A.val, sort_rows=False)
+        _check_dbsr(d, {"warpnum": r["warpnum"], "recorder": r["recorder"],
+                        "blcptr": r["blcptr"], "colidx": r["colidx"], "blcval": r["val"],
+                        "map": r["map"], "prefix": r["prefix"]})
+        checked += 1
+    assert checked >= 20
+
+
+def test_recorder_self_matches_compiled_reference(oracle):
+    # csrRecorderSelf: the CSR permuted by the oracle's recorder order
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(13)
+    for trial in range(10):
+        lens = rng.integers(0, 40, int(rng.integers(1, 200)))
+        A = _rand_csr(rng, lens, 300)
+        rp, col, val = oracle.ref_recorder_self(A.nrows, A.ncols, A.rowptr, A.col, A.val)
+        rec = oracle.dbsr(A.nrows, A.ncols, A.rowptr, A.col, A.val)["recorder"]
+        assert np.array_equal(np.diff(rp), lens[rec])
+        for i, o in enumerate(rec):
+            s, e = A.rowptr[o], A.rowptr[o + 1]
+            assert np.array_equal(col[rp[i]:rp[i + 1]], A.col[s:e])
+            assert np.array_equal(val[rp[i]:rp[i + 1]], A.val[s:e])
+
+
 def test_dbsr_balance_overflow_is_defined(oracle):
     # block-row widths [1,1,4,17,18]: pass 1 packs all 41 blocks in one warp,
     # pass 2 starts from now=41 and opens a second warp at the last block row
