@@ -129,6 +129,12 @@ _SIGS: dict[str, tuple] = {
     "bsp_dbsr_from_host": (None, [C.POINTER(DbsrMat), vp, C.POINTER(vp)]),
     "bsp_csrTodbsrWithRecorder": (C.c_int, [C.POINTER(CSRFormatF64), C.POINTER(DbsrMat)]),
     "bsp_dbsrMat_free": (VOID, [C.POINTER(DbsrMat)]),
+    "bsp_csrTodbsr": (C.c_int, [C.POINTER(CSRFormatF64), C.POINTER(DbsrMat)]),
+    "bsp_balanceDbsr": (C.c_int, [C.POINTER(DbsrMat)]),
+    "bsp_csrRecorderSelf": (C.c_int, [C.POINTER(CSRFormatF64), C.POINTER(CSRFormatF64)]),
+    "bsp_csrToBcsr": (C.c_int, [vp, vp, vp, i64, i64, C.c_int, C.c_int, i64p, C.POINTER(vp),
+                                C.POINTER(vp), C.POINTER(vp)]),
+    "bsp_free": (VOID, [vp]),
     "bsp_amgT_spmv_fp64_dbsr": (C.c_int, [C.POINTER(DbsrMat), vp, vp, i32, i32, f64p]),
     "bsp_amgT_spmv_fp64_dbsr_balance": (C.c_int, [C.POINTER(DbsrMat), vp, vp, i32, i32, f64p]),
     "bsp_amgT_spmv_fp64": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, C.c_int, f64p]),

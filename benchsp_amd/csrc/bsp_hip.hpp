@@ -94,6 +94,17 @@ bsp_status reduce_sum_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_
 // BSR builder without placing any value.
 bsp_status bsr_count_blocks(const bsp_csr& A, int tm, int tk, hipStream_t s, int64_t* nblocks);
 
+// DBSR of an FP64 CSR, rows sorted by (length, index) first (sort_rows) or
+// kept in their own order (bsr.hip).
+bsp_status dbsr_build_ordered(const bsp_csr& A, bool sort_rows, hipStream_t s, bsp_dbsr* out);
+// balanceDbsr on a host block-row pointer: prefix malloc'ed (warpnum + 1).
+bsp_status dbsr_balance_host(const int32_t* blcptr, int64_t brows, int32_t** prefix,
+                             int32_t* warpnum);
+// The CSR with rows sorted by (length, index) into device arrays of n + 1,
+// nnz and nnz entries.
+bsp_status csr_sort_rows(const bsp_csr& A, hipStream_t s, int32_t* rowptr, int32_t* col,
+                         double* val);
+
 // A pair of timing events, destroyed on every exit path (create() first).
 struct EventPair {
   hipEvent_t e0 = nullptr, e1 = nullptr;

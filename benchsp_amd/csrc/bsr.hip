@@ -571,6 +571,23 @@ __global__ void dbsr_narrow_flags(const int32_t* __restrict__ blcptr, int64_t br
   if (g < brows) f[g] = blcptr[g + 1] - blcptr[g] <= 8;
 }
 
+// Rows of a CSR in recorder order: new row i is old row rec[i] (wave per row).
+__global__ void csr_permute_rows(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                 const double* __restrict__ val, const int32_t* __restrict__ rec,
+                                 const int32_t* __restrict__ nrp, int64_t n,
+                                 int32_t* __restrict__ ocol, double* __restrict__ oval,
+                                 int64_t first) {
+  const int64_t i = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (i >= n) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int o = rec[i];
+  const int s = rowptr[o], len = rowptr[o + 1] - s, d = nrp[i];
+  for (int k = lane; k < len; k += kWave) {
+    ocol[d + k] = col[s + k];
+    oval[d + k] = val[s + k];
+  }
+}
+
 }  // namespace
 
 // Groups of a DBSR handle that take the two-groups-per-wave path.
@@ -717,7 +734,42 @@ static bsp_status build_bsr(const bsp_csr& A, int tm, int tk, hipStream_t s, bsp
   return BSP_OK;
 }
 
-static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
+// balanceDbsr (dbsr.h:94-169) on a device block-row pointer: prefix
+// (warps + 1 entries, allocated here), the warp count, and whether the
+// reference's second pass would run past its warp list.
+static bsp_status dbsr_balance(const int32_t* blcptr, int64_t brows, hipStream_t s,
+                               DevBuf<int32_t>& prefix, int32_t* warpnum, int32_t* overflow) {
+  const int64_t nchunks = cdiv(brows, kDbChunk);
+  DevBuf<int32_t> counts, tab;
+  DevBuf<int2> entry;
+  BSP_TRY(counts.alloc(2));
+  BSP_TRY(tab.alloc(nchunks * 4 * kDbStates));
+  BSP_TRY(entry.alloc(nchunks));
+  if (nchunks > 0) dbsr_chunk_tables<<<nchunks, kDbTB, 0, s>>>(blcptr, brows, tab.get());
+  dbsr_chunk_walk<<<1, kWave, 0, s>>>(tab.get(), nchunks, brows, counts.get(), entry.get());
+  BSP_HIP(hipGetLastError());
+  int32_t c[2] = {0, 0};
+  BSP_HIP(hipMemcpyAsync(c, counts.get(), sizeof(c), hipMemcpyDeviceToHost, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  *overflow = c[1] > c[0] ? 1 : 0;
+  const int32_t warps = c[1] > c[0] ? c[1] : c[0];
+  *warpnum = warps;
+  DevBuf<int32_t> cnt;
+  BSP_TRY(cnt.alloc(warps + 1));
+  BSP_TRY(prefix.alloc(warps + 1));
+  BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * (warps + 1), s));
+  if (nchunks > 0) dbsr_chunk_fill<<<nchunks, kDbTB, 0, s>>>(blcptr, brows, entry.get(), cnt.get());
+  BSP_HIP(hipGetLastError());
+  // prefix[i] = block rows of warps 0..i-1, i = 0..warps (cnt[warps] is 0)
+  BSP_TRY(exclusive_scan_i32(cnt.get(), prefix.get(), (int64_t)warps + 1, s));
+  BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
+// sort_rows: csrTodbsrWithRecorder (rows sorted by (length, index) first);
+// otherwise csrTodbsr on the rows as given (recorder = identity). Both end
+// with balanceDbsr.
+static bsp_status build_dbsr(const bsp_csr& A, bool sort_rows, hipStream_t s, bsp_dbsr_s* h) {
   const int64_t n = A.nrows;
   bsp_dbsr_view& v = h->v;
   v.nrows = n;
@@ -732,9 +784,15 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
   if (n > 0) {
     row_lengths<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, n, len.get(), idx.get());
     BSP_HIP(hipGetLastError());
-    // stable radix sort by length keeps equal lengths in index order: the
-    // (length, index) order of dbsr.h:213-216
-    BSP_TRY(sort_pairs_i32(len.get(), slen.get(), idx.get(), h->recorder.get(), n, false, 32, s));
+    if (sort_rows) {
+      // stable radix sort by length keeps equal lengths in index order: the
+      // (length, index) order of dbsr.h:213-216
+      BSP_TRY(sort_pairs_i32(len.get(), slen.get(), idx.get(), h->recorder.get(), n, false, 32, s));
+    } else {
+      BSP_HIP(hipMemcpyAsync(slen.get(), len.get(), sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+      BSP_HIP(hipMemcpyAsync(h->recorder.get(), idx.get(), sizeof(int32_t) * n,
+                             hipMemcpyDeviceToDevice, s));
+    }
   }
   BSP_TRY(w.alloc(v.brows + 1));
   BSP_TRY(h->blcptr.alloc(v.brows + 1));
@@ -759,32 +817,7 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
                                                 h->map.get());
     BSP_HIP(hipGetLastError());
   }
-  const int64_t nchunks = cdiv(v.brows, kDbChunk);
-  DevBuf<int32_t> counts, tab;
-  DevBuf<int2> entry;
-  BSP_TRY(counts.alloc(2));
-  BSP_TRY(tab.alloc(nchunks * 4 * kDbStates));
-  BSP_TRY(entry.alloc(nchunks));
-  if (nchunks > 0)
-    dbsr_chunk_tables<<<nchunks, kDbTB, 0, s>>>(h->blcptr.get(), v.brows, tab.get());
-  dbsr_chunk_walk<<<1, kWave, 0, s>>>(tab.get(), nchunks, v.brows, counts.get(), entry.get());
-  BSP_HIP(hipGetLastError());
-  int32_t c[2] = {0, 0};
-  BSP_HIP(hipMemcpyAsync(c, counts.get(), sizeof(c), hipMemcpyDeviceToHost, s));
-  BSP_HIP(hipStreamSynchronize(s));
-  v.overflow = c[1] > c[0] ? 1 : 0;
-  const int32_t warps = c[1] > c[0] ? c[1] : c[0];
-  v.warpnum = warps;
-  DevBuf<int32_t> cnt;
-  BSP_TRY(cnt.alloc(warps + 1));
-  BSP_TRY(h->prefix.alloc(warps + 1));
-  BSP_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int32_t) * (warps + 1), s));
-  if (nchunks > 0)
-    dbsr_chunk_fill<<<nchunks, kDbTB, 0, s>>>(h->blcptr.get(), v.brows, entry.get(), cnt.get());
-  BSP_HIP(hipGetLastError());
-  // prefix[i] = block rows of warps 0..i-1, i = 0..warps (cnt[warps] is 0)
-  BSP_TRY(exclusive_scan_i32(cnt.get(), h->prefix.get(), (int64_t)warps + 1, s));
-  BSP_HIP(hipStreamSynchronize(s));
+  BSP_TRY(dbsr_balance(h->blcptr.get(), v.brows, s, h->prefix, &v.warpnum, &v.overflow));
   v.recorder = h->recorder.get();
   v.blcptr = h->blcptr.get();
   v.colidx = h->colidx.get();
@@ -792,6 +825,60 @@ static bsp_status build_dbsr(const bsp_csr& A, hipStream_t s, bsp_dbsr_s* h) {
   v.map = h->map.get();
   v.prefix = h->prefix.get();
   BSP_TRY(dbsr_count_narrow(h, s));
+  return BSP_OK;
+}
+
+bsp_status dbsr_build_ordered(const bsp_csr& A, bool sort_rows, hipStream_t s, bsp_dbsr* out) {
+  auto h = std::make_unique<bsp_dbsr_s>();
+  BSP_TRY(build_dbsr(A, sort_rows, s, h.get()));
+  *out = h.release();
+  return BSP_OK;
+}
+
+bsp_status dbsr_balance_host(const int32_t* blcptr, int64_t brows, int32_t** prefix,
+                             int32_t* warpnum) {
+  DevBuf<int32_t> bp, pre;
+  BSP_TRY(bp.alloc(brows + 1));
+  BSP_HIP(hipMemcpy(bp.get(), blcptr, sizeof(int32_t) * (brows + 1), hipMemcpyHostToDevice));
+  int32_t wn = 0, overflow = 0;
+  BSP_TRY(dbsr_balance(bp.get(), brows, nullptr, pre, &wn, &overflow));
+  int32_t* h = (int32_t*)std::malloc(sizeof(int32_t) * ((size_t)wn + 1));
+  if (!h) return fail(BSP_ERR_ALLOC, "host allocation failed");
+  if (hipMemcpy(h, pre.get(), sizeof(int32_t) * ((size_t)wn + 1), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    std::free(h);
+    return fail(BSP_ERR_HIP, "prefix download failed");
+  }
+  *prefix = h;
+  *warpnum = wn;
+  return BSP_OK;
+}
+
+bsp_status csr_sort_rows(const bsp_csr& A, hipStream_t s, int32_t* nrp, int32_t* ncol,
+                         double* nval) {
+  const int64_t n = A.nrows;
+  if (n == 0) {
+    BSP_HIP(hipMemsetAsync(nrp, 0, sizeof(int32_t), s));
+    BSP_HIP(hipStreamSynchronize(s));
+    return BSP_OK;
+  }
+  DevBuf<int32_t> len, idx, slen, rec;
+  BSP_TRY(len.alloc(n));
+  BSP_TRY(idx.alloc(n));
+  BSP_TRY(slen.alloc(n + 1));
+  BSP_TRY(rec.alloc(n));
+  row_lengths<<<cdiv(n, kTB), kTB, 0, s>>>(A.rowptr, n, len.get(), idx.get());
+  BSP_HIP(hipGetLastError());
+  // stable by length: the (length, index) order of dbsr.h:176-179
+  BSP_TRY(sort_pairs_i32(len.get(), slen.get(), idx.get(), rec.get(), n, false, 32, s));
+  BSP_HIP(hipMemsetAsync(slen.get() + n, 0, sizeof(int32_t), s));
+  BSP_TRY(exclusive_scan_i32(slen.get(), nrp, n + 1, s));
+  in_slices(n, kWave, [&](int64_t f, int64_t c) {
+    csr_permute_rows<<<cdiv(c * kWave, kTB), kTB, 0, s>>>(
+        A.rowptr, A.colidx, static_cast<const double*>(A.val), rec.get(), nrp, n, ncol, nval, f);
+  });
+  BSP_HIP(hipGetLastError());
+  BSP_HIP(hipStreamSynchronize(s));
   return BSP_OK;
 }
 
@@ -871,7 +958,7 @@ bsp_status bsp_dbsr_build(const bsp_csr* A, bsp_stream s, bsp_dbsr* out) {
   BSP_TRY(check_csr_f64(A));
   BSP_REQUIRE(out, "null output handle");
   auto h = std::make_unique<bsp_dbsr_s>();
-  BSP_TRY(build_dbsr(*A, (hipStream_t)s, h.get()));
+  BSP_TRY(build_dbsr(*A, true, (hipStream_t)s, h.get()));
   *out = h.release();
   return BSP_OK;
 }
@@ -906,7 +993,7 @@ bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out)
   BSP_REQUIRE(m->row >= 0 && m->blc_row == (m->row + 3) / 4 && m->blocknnz >= 0 &&
                   m->blocknnz % 16 == 0 && m->warpnum >= 0,
               "inconsistent dbsrMat sizes");
-  BSP_REQUIRE(m->blcPtr && (m->row == 0 || m->recorderMap) &&
+  BSP_REQUIRE(m->blcPtr &&
                   (m->blocknnz == 0 || (m->colIdx && m->blcVal && m->blcMap)),
               "null dbsrMat array");
   // The kernel gathers x through every stored column index (padding too)
@@ -923,7 +1010,7 @@ bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out)
   for (int64_t k = 0; k < m->blocknnz; ++k)
     if (m->colIdx[k] < 0 || m->colIdx[k] >= m->col)
       return fail(BSP_ERR_INVALID, "colIdx out of range at slot " + std::to_string(k));
-  {
+  if (m->recorderMap) {  // null: rows in their own order (csrTodbsr leaves it unset)
     std::vector<char> seen(m->row, 0);
     for (int64_t i = 0; i < m->row; ++i) {
       const int32_t r = m->recorderMap[i];
@@ -947,9 +1034,15 @@ bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out)
   BSP_TRY(h->val.alloc(v.nblocks * 16));
   BSP_TRY(h->map.alloc(v.nblocks));
   BSP_TRY(h->prefix.alloc(v.warpnum + 1));
-  if (v.nrows)
+  if (v.nrows && m->recorderMap) {
     BSP_HIP(hipMemcpyAsync(h->recorder.get(), m->recorderMap, sizeof(int32_t) * v.nrows,
                            hipMemcpyHostToDevice, st));
+  } else if (v.nrows) {
+    std::vector<int32_t> id(v.nrows);
+    for (int64_t i = 0; i < v.nrows; ++i) id[i] = (int32_t)i;
+    BSP_HIP(hipMemcpy(h->recorder.get(), id.data(), sizeof(int32_t) * v.nrows,
+                      hipMemcpyHostToDevice));
+  }
   BSP_HIP(hipMemcpyAsync(h->blcptr.get(), m->blcPtr, sizeof(int32_t) * (v.brows + 1),
                          hipMemcpyHostToDevice, st));
   if (v.nblocks) {

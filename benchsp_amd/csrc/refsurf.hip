@@ -367,6 +367,121 @@ int bsp_csrTodbsrWithRecorder(const bsp_CSRFormat_f64* csr, bsp_dbsrMat* out) {
   return 0;
 }
 
+int bsp_csrTodbsr(const bsp_CSRFormat_f64* csr, bsp_dbsrMat* out) {
+  if (!csr || !out) return 1;
+  DevCsr c;
+  if (upload_i32(c, csr->rowPtr, csr->colIdx, csr->values, csr->row, csr->col, csr->nnz) != BSP_OK)
+    return 1;
+  bsp_dbsr h = nullptr;
+  if (dbsr_build_ordered(c.A, false, nullptr, &h) != BSP_OK) return 1;
+  std::unique_ptr<bsp_dbsr_s, bsp_status (*)(bsp_dbsr)> own(h, bsp_dbsr_destroy);
+  bsp_dbsr_view v;
+  bsp_dbsr_view_get(h, &v);
+  std::memset(out, 0, sizeof(*out));  // recorderMap and prefix stay null
+  out->row = (int)v.nrows;
+  out->col = (int)v.ncols;
+  out->nnz = (int)v.nnz;
+  out->blc_row = (int)v.brows;
+  out->blc_num = (int)v.nblocks;
+  out->avg_nnz = v.nblocks ? (double)v.nnz / (double)v.nblocks : 0.0;
+  out->blocknnz = (int32_t)(v.nblocks * 16);
+  out->blcPtr = host_copy(v.blcptr, v.brows + 1);
+  out->colIdx = host_copy(v.colidx, v.nblocks * 16);
+  out->blcVal = host_copy(v.val, v.nblocks * 16);
+  out->blcMap = host_copy(v.map, v.nblocks);
+  if (!out->blcPtr || !out->colIdx || !out->blcVal || !out->blcMap) {
+    bsp_dbsrMat_free(out);
+    fail(BSP_ERR_ALLOC, "host allocation failed");
+    return 1;
+  }
+  return 0;
+}
+
+int bsp_balanceDbsr(bsp_dbsrMat* m) {
+  if (!m) return 1;
+  if (m->blc_row < 0 || !m->blcPtr || m->blcPtr[0] != 0) {
+    fail(BSP_ERR_INVALID, "balanceDbsr: blcPtr missing or not starting at 0");
+    return 1;
+  }
+  for (int64_t g = 0; g < m->blc_row; ++g)
+    if (m->blcPtr[g + 1] < m->blcPtr[g]) {
+      fail(BSP_ERR_INVALID, "balanceDbsr: blcPtr decreases at block row " + std::to_string(g));
+      return 1;
+    }
+  int32_t* prefix = nullptr;
+  int32_t wn = 0;
+  if (dbsr_balance_host(m->blcPtr, m->blc_row, &prefix, &wn) != BSP_OK) return 1;
+  std::free(m->prefix);
+  m->prefix = prefix;
+  m->warpnum = wn;
+  return 0;
+}
+
+int bsp_csrRecorderSelf(const bsp_CSRFormat_f64* csr, bsp_CSRFormat_f64* out) {
+  if (!csr || !out) return 1;
+  DevCsr c;
+  if (upload_i32(c, csr->rowPtr, csr->colIdx, csr->values, csr->row, csr->col, csr->nnz) != BSP_OK)
+    return 1;
+  DevBuf<int32_t> rp, ci;
+  DevBuf<double> v;
+  if (rp.alloc((int64_t)csr->row + 1) != BSP_OK || ci.alloc(csr->nnz) != BSP_OK ||
+      v.alloc(csr->nnz) != BSP_OK)
+    return 1;
+  if (csr_sort_rows(c.A, nullptr, rp.get(), ci.get(), v.get()) != BSP_OK) return 1;
+  int32_t* hrp = host_copy(rp.get(), (int64_t)csr->row + 1);
+  int32_t* hci = host_copy(ci.get(), csr->nnz);
+  double* hv = host_copy(v.get(), csr->nnz);
+  if (!hrp || !hci || !hv) {
+    std::free(hrp);
+    std::free(hci);
+    std::free(hv);
+    fail(BSP_ERR_ALLOC, "host allocation failed");
+    return 1;
+  }
+  out->row = csr->row;
+  out->col = csr->col;
+  out->nnz = csr->nnz;
+  out->rowPtr = hrp;
+  out->colIdx = hci;
+  out->values = hv;
+  return 0;
+}
+
+int bsp_csrToBcsr(const int64_t* rowPtr, const int64_t* colIdx, const double* values, int64_t row,
+                  int64_t col, int tile_m, int tile_k, int64_t* nonzeroBlocks,
+                  int64_t** bcsrRowPtr, int64_t** bcsrColIdx, double** bcsrVal) {
+  if (!nonzeroBlocks || !bcsrRowPtr || !bcsrColIdx || !bcsrVal) return 1;
+  const int64_t nnz = rowPtr && row >= 0 ? rowPtr[row] : -1;
+  DevCsr c;
+  if (upload_i64(c, rowPtr, colIdx, values, row, col, nnz) != BSP_OK) return 1;
+  bsp_bsr b = nullptr;
+  if (bsp_bsr_build(&c.A, tile_m, tile_k, nullptr, &b) != BSP_OK) return 1;
+  std::unique_ptr<bsp_bsr_s, bsp_status (*)(bsp_bsr)> own(b, bsp_bsr_destroy);
+  bsp_bsr_view v;
+  bsp_bsr_view_get(b, &v);
+  std::vector<int32_t> rp32(v.brows + 1), ci32(v.nblocks);
+  const int64_t nval = v.nblocks * (int64_t)tile_m * tile_k;
+  int64_t* orp = (int64_t*)std::malloc(sizeof(int64_t) * (v.brows + 1));
+  int64_t* oci = (int64_t*)std::malloc(sizeof(int64_t) * (v.nblocks ? v.nblocks : 1));
+  double* ov = (double*)std::malloc(sizeof(double) * (nval ? nval : 1));
+  if (!orp || !oci || !ov || down(rp32.data(), v.rowptr, v.brows + 1) != BSP_OK ||
+      down(ci32.data(), v.colidx, v.nblocks) != BSP_OK || down(ov, v.val, nval) != BSP_OK) {
+    std::free(orp);
+    std::free(oci);
+    std::free(ov);
+    return 1;
+  }
+  for (int64_t i = 0; i <= v.brows; ++i) orp[i] = rp32[i];
+  for (int64_t i = 0; i < v.nblocks; ++i) oci[i] = ci32[i];
+  *nonzeroBlocks = v.nblocks;
+  *bcsrRowPtr = orp;
+  *bcsrColIdx = oci;
+  *bcsrVal = ov;
+  return 0;
+}
+
+void bsp_free(void* p) { std::free(p); }
+
 void bsp_dbsrMat_free(bsp_dbsrMat* m) {
   if (!m) return;
   std::free(m->blcPtr);

@@ -1,7 +1,8 @@
 // C++ drop-in for the reference's public headers: the same free-function
 // names and signatures as include/csrspmv.h, amgT.h, bcsr.h, dasp.h, spmv.h,
 // smat.h, cusp.h, ref.h, and the types of include/matrixFormat.hpp (CSRFormat,
-// BCSRFormat, dbsrMat) and include/dbsr.h (csrTodbsrWithRecorder), each an
+// BCSRFormat, dbsrMat), include/dbsr.h (csrTodbsr, balanceDbsr, csrRecorderSelf,
+// csrTodbsrWithRecorder) and utils/utils.hpp (csrToBcsr), each an
 // inline forward to the C-ABI in bsp.h. A driver written against the
 // reference headers includes this header instead and links libbsp.so.
 //
@@ -19,8 +20,11 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "bsp.h"
 
@@ -211,6 +215,83 @@ inline void ref_spmv_fp32(int* A_csrOffsets, int32_t* A_columns, float* A_values
 inline void ref_spmv_fp64(int64_t* A_csrOffsets, int64_t* A_columns, double* A_values, double* x,
                           double* y, int64_t row, int64_t col, double& avg_time) {
   bsp_ref_spmv_fp64(A_csrOffsets, A_columns, A_values, x, y, row, col, &avg_time);
+}
+
+// include/dbsr.h:10-93: arrays are malloc'ed; bsp_dbsrMat_free releases them
+inline void csrTodbsr(CSRFormat<double, int32_t>* csr, dbsrMat* bsr) {
+  if (bsp_csrTodbsr(reinterpret_cast<const bsp_CSRFormat_f64*>(csr), bsp_compat_detail::c(bsr)) != 0)
+    throw std::runtime_error(std::string("csrTodbsr: ") + bsp_last_error());
+}
+
+// include/dbsr.h:94-169
+inline void balanceDbsr(dbsrMat* dbsr) {
+  if (bsp_balanceDbsr(bsp_compat_detail::c(dbsr)) != 0)
+    throw std::runtime_error(std::string("balanceDbsr: ") + bsp_last_error());
+}
+
+// include/dbsr.h:170-206: recsr's arrays are malloc'ed (bsp_free)
+inline void csrRecorderSelf(CSRFormat<double, int32_t>* csr, CSRFormat<double, int32_t>* recsr) {
+  if (bsp_csrRecorderSelf(reinterpret_cast<const bsp_CSRFormat_f64*>(csr),
+                          reinterpret_cast<bsp_CSRFormat_f64*>(recsr)) != 0)
+    throw std::runtime_error(std::string("csrRecorderSelf: ") + bsp_last_error());
+}
+
+// utils/utils.hpp:9-124. Reads TILE_M and TILE_K from bcsr; fills blockNum,
+// colRegions, rowRegions, row, col, bcsrRowPtr, bcsrcolIdx, values,
+// relativeBlockIndexMapping (blockNum entries, -1 for an empty block, as the
+// reference's dense block map), nonzeroBlocks and elemCount. Every array is
+// malloc'ed. Values pass through FP64 unchanged (half, float and double are
+// exact there).
+template <typename T, typename Y>
+void csrToBcsr(CSRFormat<T, Y>* csr, BCSRFormat<T, Y>* bcsr) {
+  const int64_t rows = csr->row, cols = csr->col, nnz = rows > 0 ? (int64_t)csr->rowPtr[rows] : 0;
+  const int tm = bcsr->TILE_M, tk = bcsr->TILE_K;
+  if (tm < 1 || tk < 1) throw std::runtime_error("csrToBcsr: TILE_M and TILE_K must be >= 1");
+  auto to_f64 = [](T v) { return static_cast<double>(static_cast<float>(v)); };
+  std::vector<int64_t> rp(rows + 1), ci(nnz);
+  std::vector<double> v(nnz);
+  for (int64_t i = 0; i <= rows; ++i) rp[i] = csr->rowPtr[i];
+  for (int64_t j = 0; j < nnz; ++j) {
+    ci[j] = csr->colIdx[j];
+    if constexpr (std::is_same<T, double>::value)
+      v[j] = csr->values[j];
+    else
+      v[j] = to_f64(csr->values[j]);
+  }
+  int64_t nb = 0, *brp = nullptr, *bci = nullptr;
+  double* bv = nullptr;
+  if (bsp_csrToBcsr(rp.data(), ci.data(), v.data(), rows, cols, tm, tk, &nb, &brp, &bci, &bv) != 0)
+    throw std::runtime_error(std::string("csrToBcsr: ") + bsp_last_error());
+  const int64_t rreg = (rows + tm - 1) / tm, creg = (cols + tk - 1) / tk;
+  bcsr->row = (Y)rows;
+  bcsr->col = (Y)cols;
+  bcsr->rowRegions = (Y)rreg;
+  bcsr->colRegions = (Y)creg;
+  bcsr->blockNum = (Y)(rreg * creg);
+  bcsr->nonzeroBlocks = (Y)nb;
+  bcsr->elemCount = (Y)(nb * tm * tk);
+  bcsr->bcsrRowPtr = (Y*)std::malloc(sizeof(Y) * (rreg + 1));
+  bcsr->bcsrcolIdx = (Y*)std::malloc(sizeof(Y) * (nb ? nb : 1));
+  bcsr->values = (T*)std::malloc(sizeof(T) * (nb ? nb * tm * tk : 1));
+  bcsr->relativeBlockIndexMapping = (Y*)std::malloc(sizeof(Y) * (rreg * creg ? rreg * creg : 1));
+  if (!bcsr->bcsrRowPtr || !bcsr->bcsrcolIdx || !bcsr->values || !bcsr->relativeBlockIndexMapping)
+    throw std::runtime_error("csrToBcsr: host allocation failed");
+  for (int64_t i = 0; i <= rreg; ++i) bcsr->bcsrRowPtr[i] = (Y)brp[i];
+  for (int64_t i = 0; i < rreg * creg; ++i) bcsr->relativeBlockIndexMapping[i] = Y(-1);
+  for (int64_t br = 0; br < rreg; ++br)
+    for (int64_t k = brp[br]; k < brp[br + 1]; ++k) {
+      bcsr->bcsrcolIdx[k] = (Y)bci[k];
+      bcsr->relativeBlockIndexMapping[br * creg + bci[k]] = (Y)k;
+    }
+  for (int64_t i = 0; i < nb * tm * tk; ++i) {
+    if constexpr (std::is_same<T, double>::value)
+      bcsr->values[i] = bv[i];
+    else
+      bcsr->values[i] = T(static_cast<float>(bv[i]));
+  }
+  bsp_free(brp);
+  bsp_free(bci);
+  bsp_free(bv);
 }
 
 // include/dbsr.h:208-253: arrays are malloc'ed; bsp_dbsrMat_free releases them

@@ -414,6 +414,35 @@ typedef struct bsp_dbsrMat {
  * Returns 0 on success. */
 int bsp_csrTodbsrWithRecorder(const bsp_CSRFormat_f64* csr, bsp_dbsrMat* out);
 void bsp_dbsrMat_free(bsp_dbsrMat* m);
+/* Replaces csrTodbsr (include/dbsr.h:10-93): DBSR of the CSR with rows in
+ * their own order, built on the GPU. Sets the fields the reference sets
+ * (row, col, nnz, blc_row, blocknnz, blcPtr, colIdx, blcVal, blcMap) plus
+ * blc_num and avg_nnz; the struct is cleared first, so recorderMap and
+ * prefix are null (the reference leaves them unset) until balanceDbsr.
+ * Returns 0 on success. */
+int bsp_csrTodbsr(const bsp_CSRFormat_f64* csr, bsp_dbsrMat* out);
+/* Replaces balanceDbsr (include/dbsr.h:94-169): packs the block rows of
+ * m->blcPtr into 64-block warps on the GPU and sets m->prefix (malloc'ed,
+ * warpnum + 1 entries; a previous prefix is freed) and m->warpnum. Where the
+ * reference's second pass would run past its warp list (undefined there),
+ * the list is extended by one warp. Returns 0 on success. */
+int bsp_balanceDbsr(bsp_dbsrMat* m);
+/* Replaces csrRecorderSelf (include/dbsr.h:170-206): `out` receives the CSR
+ * with rows sorted by (length, index), built on the GPU; its arrays are
+ * malloc'ed (free with bsp_free). Returns 0 on success. */
+int bsp_csrRecorderSelf(const bsp_CSRFormat_f64* csr, bsp_CSRFormat_f64* out);
+/* The arithmetic of csrToBcsr<T,Y> (utils/utils.hpp:9-124) with int64
+ * indices and FP64 values: tile_m x tile_k blocks (1..64), block columns
+ * ascending, row-major zero-padded values, built on the GPU. The three
+ * arrays are malloc'ed (free with bsp_free): bcsrRowPtr ceil(row/tile_m)+1,
+ * bcsrColIdx and bcsrVal (tile_m*tile_k per block) of *nonzeroBlocks
+ * blocks. include/benchsp_compat.hpp wraps it as the reference template.
+ * Returns 0 on success. */
+int bsp_csrToBcsr(const int64_t* rowPtr, const int64_t* colIdx, const double* values, int64_t row,
+                  int64_t col, int tile_m, int tile_k, int64_t* nonzeroBlocks,
+                  int64_t** bcsrRowPtr, int64_t** bcsrColIdx, double** bcsrVal);
+/* Frees memory the library malloc'ed for the caller. */
+void bsp_free(void* p);
 /* Uploads a host DBSR (as filled above) into a device handle. */
 bsp_status bsp_dbsr_from_host(const bsp_dbsrMat* m, bsp_stream s, bsp_dbsr* out);
 

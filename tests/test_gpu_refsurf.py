@@ -295,6 +295,52 @@ def test_compat_driver_runs():
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
 
 
+def test_compat_header_builders_match_oracle(oracle, tmp_path):
+    # csrTodbsr + balanceDbsr, csrRecorderSelf and csrToBcsr (double 4x4,
+    # half 16x16) called through include/benchsp_compat.hpp by the C++
+    # driver, each array against the oracle's restatement
+    import subprocess
+    from benchsp_amd import build as b
+    r = subprocess.run([b.COMPAT_BIN, "32", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
+    got = {}
+    for line in open(tmp_path / "manifest.txt"):
+        name, dt, n = line.split()
+        if dt == "int":
+            got[name] = int(n)
+        else:
+            got[name] = np.fromfile(tmp_path / f"{name}.bin", dtype=dt, count=int(n))
+    rp, col, val = got["in_rowptr"].astype(np.int64), got["in_col"], got["in_val"]
+    n, ncols = rp.size - 1, got["ncols"]
+    assert np.ptp(np.diff(rp)) >= 30  # ragged rows
+    o = oracle.dbsr(n, ncols, rp, col, val, sort_rows=False)
+    for k in ("blcptr", "colidx", "val", "map", "prefix"):
+        assert np.array_equal(got[f"dbsr_{k}"], o[k]), k
+    rec = oracle.dbsr(n, ncols, rp, col, val)["recorder"]
+    lens = np.diff(rp)
+    assert np.array_equal(np.diff(got["rec_rowptr"]), lens[rec])
+    perm = np.concatenate([np.arange(rp[i], rp[i + 1]) for i in rec])
+    assert np.array_equal(got["rec_col"], col[perm]) and np.array_equal(got["rec_val"], val[perm])
+    b4 = oracle.bcsr(n, ncols, rp, col, val, 4, 4)
+    assert np.array_equal(got["bcsr4_rowptr"], b4["rowptr"])
+    assert np.array_equal(got["bcsr4_col"], b4["colidx"])
+    assert np.array_equal(got["bcsr4_val"], b4["val"])
+    # the dense block map: block index per (block row, block column), -1 if empty
+    creg = -(-ncols // 4)
+    dense = np.full((b4["brows"], creg), -1, np.int64)
+    for br in range(b4["brows"]):
+        for k in range(b4["rowptr"][br], b4["rowptr"][br + 1]):
+            dense[br, b4["colidx"][k]] = k
+    assert np.array_equal(got["bcsr4_map"], dense.ravel())
+    vh = val.astype(np.float16)
+    b16 = oracle.bcsr(n, ncols, rp, col, vh.astype(np.float64), 16, 16)
+    assert np.array_equal(got["bcsr16_rowptr"], b16["rowptr"])
+    assert np.array_equal(got["bcsr16_col"], b16["colidx"])
+    assert np.array_equal(got["bcsr16_val"].astype(np.float64), b16["val"])
+
+
 @pytest.mark.parametrize("dtype_name", ["float32", "float16"])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_rocsparse_spmm_comparator(oracle, bsp_host, dtype_name, transpose):
