@@ -334,7 +334,7 @@ def test_compat_header_builders_match_oracle(oracle, tmp_path):
         for k in range(b4["rowptr"][br], b4["rowptr"][br + 1]):
             dense[br, b4["colidx"][k]] = k
     assert np.array_equal(got["bcsr4_map"], dense.ravel())
-    vh = val.astype(np.float16)
+    vh = val.astype(np.float32).astype(np.float16)  # the driver's __float2half((float)v)
     b16 = oracle.bcsr(n, ncols, rp, col, vh.astype(np.float64), 16, 16)
     assert np.array_equal(got["bcsr16_rowptr"], b16["rowptr"])
     assert np.array_equal(got["bcsr16_col"], b16["colidx"])
