@@ -185,6 +185,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_csr_spmv": (None, [vp, vp, vp, vp]),
     "bsp_csr_plan_info": (None, [vp, i64p, i64p]),
     "bsp_plan_destroy": (None, [vp]),
+    "bsp_csr_spmv_noalloc": (None, [C.POINTER(CsrDesc), vp, vp, vp]),
     "bsp_gen_uniform": (None, [i64, u64, vp]),
     "bsp_gen_normal": (None, [i64, u64, vp]),
     "bsp_gen_stencil5": (None, [i64, i64, C.POINTER(vp)]),

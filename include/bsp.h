@@ -116,6 +116,17 @@ bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream 
 /* Number of workgroups one bsp_csr_spmv launch uses (its main kernel). */
 bsp_status bsp_csr_plan_info(bsp_spmv_plan p, int64_t* nblocks, int64_t* nlong);
 bsp_status bsp_plan_destroy(bsp_spmv_plan p);
+/* y = A*x for an FP64 CSR that allocates nothing: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * adaptive schedule with 2048-unit windows is derived on every call into a
+ * fixed 4 MB device array per GPU (held until the process ends), so y is
+ * bit-identical to bsp_csr_spmv on a plan from bsp_csr_plan_create_ex with
+ * chunk 2048 and seg_cols < 0. Limit: nrows + nnz <= 2^31
+ * (BSP_ERR_TOO_LARGE beyond). Calls that overlap on one GPU (two streams,
+ * or two host threads) share the array and race; serialise them.
+ * Stream-ordered; no host synchronisation. */
+bsp_status bsp_csr_spmv_noalloc(const bsp_csr* A, const double* x, double* y, bsp_stream s);
 
 /* ------------------------------------------------------------------------ */
 /* Block CSR (BSR) with tm x tk dense blocks, built on the GPU.              */

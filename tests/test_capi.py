@@ -304,3 +304,17 @@ def test_tune_record_validated_before_any_gpu_work(rec):
     with pytest.raises(BspError) as e:
         lib().bsp_tuned_from_record(C.byref(d), C.byref(r), None, C.byref(h))
     assert e.value.status == 1, str(e.value)
+
+
+def test_noalloc_spmv_rejects_sizes_beyond_its_schedule_array():
+    # rows + entries above 2^31 do not fit the fixed 4 MB window table; the
+    # check comes before any device access (the pointers are never read)
+    from benchsp_amd._lib import BspError, CsrDesc, lib
+    d = CsrDesc(1_500_000_000, 10, 1_000_000_000, 8, 8, 8, 0)
+    with pytest.raises(BspError) as e:
+        lib().bsp_csr_spmv_noalloc(C.byref(d), 8, 8, None)
+    assert e.value.status == 6 and "2^31" in str(e.value)
+    d = CsrDesc(10, 10, 5, 8, 8, 8, 1)  # FP32 values: FP64 only
+    with pytest.raises(BspError) as e:
+        lib().bsp_csr_spmv_noalloc(C.byref(d), 8, 8, None)
+    assert e.value.status == 1
