@@ -383,6 +383,8 @@ def formats_leg(args, dev, mats, xs):
         if A.nnz / max(1, A.nrows) >= 16:  # 4x4 blocks only where rows are dense
             cands["bsr4_mfma"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=2)))(Bsr(d))
             cands["bsr4_valu"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=1)))(Bsr(d))
+            cands["bsr4_mfma_bal"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=3)))(Bsr(d))
+            cands["bsr4_valu_bal"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=4)))(Bsr(d))
         res = {}
         for key, make in cands.items():
             ops, build = [], []

@@ -10,6 +10,13 @@ struct bsp_bsr_s {
   bsp::DevBuf<int32_t> rowptr, colidx, rpw, riw;
   bsp::DevBuf<double> val;
   bsp::DevBuf<uint16_t> map;
+  // balanced 4x4 kernels: block rows whose chunk count is not one (empty
+  // ones, and those split over several 64-block chunks), and per-chunk
+  // partial sums (4 per chunk) that the combine pass adds in chunk order
+  bsp::DevBuf<int32_t> multi;
+  int64_t nmulti = 0;
+  bsp::DevBuf<double> part;
+  bsp::DevBuf<int4> ctab;  // per chunk {block row, first block, blocks, single}
 };
 
 struct bsp_dbsr_s {
@@ -204,6 +211,157 @@ __global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
     const int64_t row = br * 4 + (lane >> 4);
     if (row < nrows) y[row] = t;
   }
+}
+
+// ---- balanced 4x4 kernels (src/amgT.cu:160-209, 464-521) -------------------
+// Work unit: chunk w of the block-row balance arrays, blocks
+// [rowptr[br] + 64k, min(rowptr[br] + 64(k+1), rowptr[br+1])) of block row
+// br = rowidx_by_warp[w], k = w - rowptr_by_warp[br], described by one
+// 16-byte entry of the chunk table {br, first block, blocks, single} built
+// with the layout. A block row of one chunk ("single") gets its y rows
+// written directly; otherwise the chunk's four row sums go to
+// part[4w..4w+3] and bsr4_bal_combine adds them in chunk order (no atomics:
+// the same bits every run).
+__global__ void bal_chunk_table(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ rpw,
+                                const int32_t* __restrict__ riw, int64_t nwarps,
+                                int4* __restrict__ ctab) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwarps) return;
+  const int br = riw[w];
+  const int w0 = rpw[br], w1 = rpw[br + 1];
+  const int s = rowptr[br] + 64 * (int)(w - w0);
+  ctab[w] = make_int4(br, s, min(64, rowptr[br + 1] - s), w1 - w0 == 1);
+}
+
+__device__ __forceinline__ void bal_store(const int4& d, int64_t w, int r, double t,
+                                          int64_t nrows, double* __restrict__ y,
+                                          double* __restrict__ part) {
+  if (d.w) {
+    const int64_t row = (int64_t)d.x * 4 + r;
+    if (row < nrows) y[row] = t;
+  } else {
+    part[w * 4 + r] = t;
+  }
+}
+
+// MFMA: the chunk's blocks four at a time as the 16x4 A operand of
+// v_mfma_f64_16x16x4f64 (layouts as in bsr4_mfma_f64), 32 blocks' operand
+// loads in flight per step.
+__global__ __launch_bounds__(kTB) void bsr4_bal_mfma_f64(
+    const int4* __restrict__ ctab, int64_t nwarps, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
+    double* __restrict__ part, int64_t nrows, int64_t ncols, int64_t first) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t w = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (w >= nwarps) return;  // whole wave exits together
+  const int4 d = ctab[w];
+  const int ai = lane & 15, kk = lane >> 4, a = ai >> 2, r = ai & 3;
+  const int cid = lane < d.z ? colidx[d.y + lane] : 0;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < d.z; k0 += 32) {
+    double av[8], bv[8];
+    // A operands first: they do not wait for the column indices
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kb = k0 + 4 * u + a;
+      av[u] = kb < d.z ? ld_stream(&bval[(int64_t)(d.y + kb) * 16 + r * 4 + kk]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kb = k0 + 4 * u + a;
+      const int c4 = __shfl(cid, min(kb, kWave - 1), 64);
+      const int64_t col = (int64_t)c4 * 4 + kk;
+      bv[u] = kb < d.z && col < ncols ? x[col] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + 4 * u < d.z) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+  }
+  double t = a == 0 ? acc[0] : a == 1 ? acc[1] : a == 2 ? acc[2] : acc[3];
+  t += __shfl_xor(t, 4, 64);
+  t += __shfl_xor(t, 8, 64);
+  if ((lane & 15) == 0) bal_store(d, w, lane >> 4, t, nrows, y, part);
+}
+
+// VALU with the occupancy bitmap, four chunks per wave: 16 lanes per chunk
+// in four groups of four, group g takes blocks g, g+4, g+8, ... of the
+// chunk, lane r of a group row r of its block; only entries whose map bit
+// r*4+c is set are multiplied (a non-finite x entry in a padded column does
+// not reach y). Row sums over the groups by lane shuffles.
+__global__ __launch_bounds__(kTB) void bsr4_bal_valu_f64(
+    const int4* __restrict__ ctab, int64_t nwarps, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const uint16_t* __restrict__ map,
+    const double* __restrict__ x, double* __restrict__ y, double* __restrict__ part,
+    int64_t nrows, int64_t ncols, int64_t first) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t q = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (4 * q >= nwarps) return;
+  const int slot = lane >> 4, g = (lane >> 2) & 3, r = lane & 3;
+  const int64_t w = 4 * q + slot;
+  const int4 d = w < nwarps ? ctab[w] : make_int4(0, 0, 0, 1);
+  double t = 0.0;
+  for (int i0 = 0; i0 < d.z; i0 += 16) {  // 16 blocks of the chunk per step
+    unsigned bits[4];
+    int cb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = i0 + g + 4 * i;
+      const bool ok = k < d.z;
+      bits[i] = ok ? ((unsigned)map[d.y + k] >> (4 * r)) & 15u : 0u;
+      cb[i] = ok ? colidx[d.y + k] : 0;
+    }
+    dbl4 v[4];
+    double xv[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t b = (int64_t)(d.y + i0 + g + 4 * i) * 16 + r * 4;
+      v[i] = i0 + g + 4 * i < d.z ? ld_stream(reinterpret_cast<const dbl4*>(bval + b))
+                                  : dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int64_t col = (int64_t)cb[i] * 4 + c;
+        xv[i][c] = ((bits[i] >> c) & 1u) && col < ncols ? x[col] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (bits[i] & 1u) t += v[i].x * xv[i][0];
+      if (bits[i] & 2u) t += v[i].y * xv[i][1];
+      if (bits[i] & 4u) t += v[i].z * xv[i][2];
+      if (bits[i] & 8u) t += v[i].w * xv[i][3];
+    }
+  }
+  t += __shfl_xor(t, 4, 64);
+  t += __shfl_xor(t, 8, 64);
+  if ((lane & 12) == 0 && w < nwarps) bal_store(d, w, r, t, nrows, y, part);
+}
+
+// y rows of the block rows listed in `multi`: the chunk partials in chunk
+// order (zero for a block row without blocks).
+__global__ void bsr4_bal_combine(const int32_t* __restrict__ multi, int64_t nmulti,
+                                 const int32_t* __restrict__ rpw, const double* __restrict__ part,
+                                 int64_t nrows, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmulti * 4) return;
+  const int64_t br = multi[i >> 2];
+  const int r = (int)(i & 3);
+  double s = 0.0;
+  for (int w = rpw[br]; w < rpw[br + 1]; ++w) s += part[(int64_t)w * 4 + r];
+  const int64_t row = br * 4 + r;
+  if (row < nrows) y[row] = s;
+}
+
+__global__ void bal_multi_flags(const int32_t* __restrict__ rpw, int64_t brows,
+                                int32_t* __restrict__ flag) {
+  const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (br > brows) return;
+  flag[br] = br < brows && rpw[br + 1] - rpw[br] != 1;
+}
+
+__global__ void bal_multi_scatter(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                                  int64_t brows, int32_t* __restrict__ multi) {
+  const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (br < brows && flag[br]) multi[pos[br]] = (int32_t)br;
 }
 
 // 8x8 blocks on the same v_mfma_f64_16x16x4f64, two blocks per product:
@@ -721,6 +879,31 @@ static bsp_status build_bsr(const bsp_csr& A, int tm, int tk, hipStream_t s, bsp
     BSP_HIP(hipGetLastError());
     v.rowptr_by_warp = h->rpw.get();
     v.rowidx_by_warp = h->riw.get();
+    // balanced kernels' workspace: the list of block rows not covered by
+    // exactly one chunk, and four partial sums per chunk
+    {
+      DevBuf<int32_t> mf, mp;
+      BSP_TRY(mf.alloc(v.brows + 1));
+      BSP_TRY(mp.alloc(v.brows + 1));
+      bal_multi_flags<<<cdiv(v.brows + 1, kTB), kTB, 0, s>>>(h->rpw.get(), v.brows, mf.get());
+      BSP_HIP(hipGetLastError());
+      BSP_TRY(exclusive_scan_i32(mf.get(), mp.get(), v.brows + 1, s));
+      int32_t nm = 0;
+      BSP_TRY(fetch(mp.get() + v.brows, &nm, s));
+      h->nmulti = nm;
+      BSP_TRY(h->multi.alloc(nm));
+      if (v.brows > 0)
+        bal_multi_scatter<<<cdiv(v.brows, kTB), kTB, 0, s>>>(mf.get(), mp.get(), v.brows,
+                                                             h->multi.get());
+      BSP_HIP(hipGetLastError());
+      BSP_TRY(h->part.alloc((int64_t)wn * 4));
+      BSP_TRY(h->ctab.alloc(wn));
+      if (wn > 0)
+        bal_chunk_table<<<cdiv(wn, kTB), kTB, 0, s>>>(h->rowptr.get(), h->rpw.get(),
+                                                      h->riw.get(), wn, h->ctab.get());
+      BSP_HIP(hipGetLastError());
+      BSP_HIP(hipStreamSynchronize(s));
+    }
     DevBuf<double> sq;
     BSP_TRY(sq.alloc(1));
     const double avg_len = v.brows ? (double)nblocks / (double)v.brows : 0.0;
@@ -942,6 +1125,23 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
         bsr8_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
                                                             v.brows, v.nrows, v.ncols, f);
     });
+  } else if (kernel == BSP_BSR_MFMA_BALANCED || kernel == BSP_BSR_VALU_BALANCED) {
+    if (v.tm != 4 || v.tk != 4)
+      return fail(BSP_ERR_UNSUPPORTED, "balanced BSR kernels need 4x4 blocks");
+    if (kernel == BSP_BSR_MFMA_BALANCED)  // a wave per chunk
+      in_slices(v.warpnum, kWave, [&](int64_t f, int64_t c) {
+        bsr4_bal_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+            h->ctab.get(), v.warpnum, v.colidx, v.val, x, y, h->part.get(), v.nrows, v.ncols, f);
+      });
+    else  // a wave per four chunks
+      in_slices(cdiv(v.warpnum, 4), kWave, [&](int64_t f, int64_t c) {
+        bsr4_bal_valu_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+            h->ctab.get(), v.warpnum, v.colidx, v.val, v.map, x, y, h->part.get(), v.nrows,
+            v.ncols, f);
+      });
+    if (h->nmulti > 0)
+      bsr4_bal_combine<<<cdiv(h->nmulti * 4, kTB), kTB, 0, st>>>(
+          h->multi.get(), h->nmulti, v.rowptr_by_warp, h->part.get(), v.nrows, y);
   } else {
     return fail(BSP_ERR_INVALID, "unknown BSR kernel");
   }

@@ -517,7 +517,10 @@ int bsp_amgT_spmv_fp64(int32_t* hA_csrOffsets, int32_t* hA_columns, double* hA_v
   std::unique_ptr<bsp_bsr_s, bsp_status (*)(bsp_bsr)> own(b, bsp_bsr_destroy);
   bsp_bsr_view v;
   bsp_bsr_view_get(b, &v);
-  const int kernel = v.avg_nnz >= 10.0 ? BSP_BSR_MFMA : BSP_BSR_VALU;
+  // BSR_BALANCED_PREPROCESS_GPU sets stand = 12 (src/amgT.cu:1176), so the
+  // reference always takes its balanced branch (:1394-1407): MMA when
+  // avgnz >= 10, CUDA cores with the bitmap otherwise
+  const int kernel = v.avg_nnz >= 10.0 ? BSP_BSR_MFMA_BALANCED : BSP_BSR_VALU_BALANCED;
   DevBuf<double> x, y;
   if (up(x, hX, A_num_cols) != BSP_OK || y.alloc(A_num_rows) != BSP_OK) return 1;
   if (timed(5, repeat, avg_time, [&] { return bsp_bsr_spmv(b, kernel, x.get(), y.get(), nullptr); }) !=

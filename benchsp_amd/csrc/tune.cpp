@@ -37,9 +37,12 @@ bsp_status check_record(const bsp_tune_record& r) {
       BSP_REQUIRE(r.chunk >= 128 && r.chunk <= 2048, "CSR record: chunk must be in 128..2048");
       return BSP_OK;
     case BSP_TUNE_BSR4:
+      BSP_REQUIRE(r.kernel >= BSP_BSR_VALU && r.kernel <= BSP_BSR_VALU_BALANCED,
+                  "BSR4 record: kernel must be a bsp_bsr_kernel other than AUTO");
+      return BSP_OK;
     case BSP_TUNE_BSR8:
       BSP_REQUIRE(r.kernel == BSP_BSR_VALU || r.kernel == BSP_BSR_MFMA,
-                  "BSR record: kernel must be BSP_BSR_VALU or BSP_BSR_MFMA");
+                  "BSR8 record: kernel must be BSP_BSR_VALU or BSP_BSR_MFMA");
       return BSP_OK;
     case BSP_TUNE_DBSR:
       return BSP_OK;
@@ -142,7 +145,8 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
     bsp_blockinfo bi{};
     BSP_TRY(bsp_block_info(A, b, stream, &bi));
     if (bi.avg < kMinBlockFill * b * b) continue;
-    for (int kern : {BSP_BSR_VALU, BSP_BSR_MFMA}) {
+    for (int kern : {BSP_BSR_VALU, BSP_BSR_MFMA, BSP_BSR_MFMA_BALANCED, BSP_BSR_VALU_BALANCED}) {
+      if (b == 8 && kern > BSP_BSR_MFMA) continue;  // the balanced kernels take 4x4 blocks
       bsp_tune_record r{};
       r.format = fmt;
       r.kernel = kern;

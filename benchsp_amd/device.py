@@ -106,7 +106,7 @@ def _to_host(ptr_value, n, dtype) -> np.ndarray:
 class Bsr:
     """Block CSR with tm x tk blocks, built on the GPU (bsp_bsr_build)."""
 
-    VALU, MFMA = 1, 2
+    VALU, MFMA, MFMA_BALANCED, VALU_BALANCED = 1, 2, 3, 4
 
     def __init__(self, A: DeviceCsr, tm: int = 4, tk: int = 4):
         from ._lib import BsrView

@@ -164,7 +164,16 @@ typedef enum bsp_bsr_kernel {
    * into the 16x4 A operand, x segments as B columns, the block-diagonal of
    * the 16x16 result accumulated. Counterpart of bsr_spmv_tc_fp64
    * (src/amgT.cu:210-274, mma.m8n8k4 with 2 blocks). */
-  BSP_BSR_MFMA = 2
+  BSP_BSR_MFMA = 2,
+  /* 4x4 blocks, load-balanced as the reference's balanced kernels
+   * (src/amgT.cu:464-521 on MMA, :160-209 on CUDA cores with the bitmap):
+   * one wave per 64-block chunk of the rowptr_by_warp / rowidx_by_warp
+   * arrays, so a long block row is spread over several waves. Chunk sums of
+   * a block row are added in chunk order by a second pass (no atomics, the
+   * same bits every run). MFMA: the chunk's blocks four per product, as
+   * BSP_BSR_MFMA. VALU: only entries whose map bit is set are multiplied. */
+  BSP_BSR_MFMA_BALANCED = 3,
+  BSP_BSR_VALU_BALANCED = 4
 } bsp_bsr_kernel;
 
 bsp_status bsp_bsr_build(const bsp_csr* A, int tm, int tk, bsp_stream s, bsp_bsr* out);
@@ -468,8 +477,9 @@ int bsp_amgT_spmv_fp64_dbsr(bsp_dbsrMat* dbsr, double* hX, double* hY, int32_t w
 int bsp_amgT_spmv_fp64_dbsr_balance(bsp_dbsrMat* dbsr, double* hX, double* hY, int32_t warmup,
                                     int32_t repeat, double* avg_time);
 /* Replaces int amgT_spmv_fp64(...) (include/amgT.h:3, src/amgT.cu:1355-1488):
- * CSR -> 4x4 BSR on the GPU, then the MFMA kernel when nnz/blocks >= 10 and
- * the VALU kernel otherwise (the reference's avgnz cut, amgT.cu:1394-1424),
+ * CSR -> 4x4 BSR on the GPU, then the balanced kernels the reference runs
+ * (its stand is forced to 12, amgT.cu:1176): BSP_BSR_MFMA_BALANCED when
+ * nnz/blocks >= 10, BSP_BSR_VALU_BALANCED otherwise (amgT.cu:1394-1407),
  * 5 warmups, `repeat` timed launches. */
 int bsp_amgT_spmv_fp64(int32_t* hA_csrOffsets, int32_t* hA_columns, double* hA_values, double* hX,
                        double* hY, int32_t A_num_rows, int32_t A_num_cols, int32_t A_nnz,

@@ -65,13 +65,18 @@ def test_bsr_builder_bit_exact(oracle, bsp_host, tm, tk):
         torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+_KERNELS = {"valu": 1, "mfma": 2, "mfma_bal": 3, "valu_bal": 4}
+
+
+@pytest.mark.parametrize("kernel", ["valu", "mfma", "mfma_bal", "valu_bal"])
 @pytest.mark.parametrize("tile", [4, 3, 8, 16])
 def test_bsr_spmv(oracle, bsp_host, kernel, tile):
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
     if kernel == "mfma" and tile not in (4, 8):
         pytest.skip("MFMA kernels take 4x4 and 8x8 blocks")
+    if kernel.endswith("_bal") and tile != 4:
+        pytest.skip("the balanced kernels take 4x4 blocks")
     mats = list(_mats(bsp_host))
     if tile in (4, 8):  # dense blocks of the kernel's size, odd block-row lengths
         mats.append((f"blocks-{tile}", bsp_host.blocksparse(300, 257, tile, 25_000, seed=tile)))
@@ -81,10 +86,13 @@ def test_bsr_spmv(oracle, bsp_host, kernel, tile):
         print(f"[bsr-spmv] {name} {tile} {kernel}", flush=True)
         x = bsp_host.uniform(A.ncols, 21)
         B = Bsr(DeviceCsr.from_host(A), tile, tile)
-        y = B(torch.from_numpy(x).cuda(), kernel=Bsr.VALU if kernel == "valu" else Bsr.MFMA)
+        y = B(torch.from_numpy(x).cuda(), kernel=_KERNELS[kernel])
         torch.cuda.synchronize()
         assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
                           abs_spmv(oracle, A, x), what=f"{name} bsr{tile} {kernel}")
+        if kernel.endswith("_bal"):  # deterministic: a second run gives the same bits
+            y2 = B(torch.from_numpy(x).cuda(), kernel=_KERNELS[kernel])
+            assert torch.equal(y, y2), name
 
 
 def test_bsr4_builder_bit_exact_powerlaw_1m(oracle, bsp_host):
@@ -99,14 +107,14 @@ def test_bsr4_builder_bit_exact_powerlaw_1m(oracle, bsp_host):
     assert g["warpnum"] == o["warpnum"] and g["avg_nnz"] == o["avg_nnz"]
 
 
-@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+@pytest.mark.parametrize("kernel", ["valu", "mfma", "mfma_bal", "valu_bal"])
 def test_bsr4_spmv_powerlaw_1m(oracle, bsp_host, kernel):
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
     A = bsp_host.powerlaw_1m()
     x = bsp_host.uniform(A.ncols, 22)
     B = Bsr(DeviceCsr.from_host(A), 4, 4)
-    y = B(torch.from_numpy(x).cuda(), kernel=Bsr.VALU if kernel == "valu" else Bsr.MFMA)
+    y = B(torch.from_numpy(x).cuda(), kernel=_KERNELS[kernel])
     torch.cuda.synchronize()
     assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
                       abs_spmv(oracle, A, x), what=f"powerlaw-1M bsr4 {kernel}")
