@@ -370,7 +370,9 @@ def formats_leg(args, dev, mats, xs):
             "csr_balanced": lambda d: CsrSpmv(d, "balanced"),
             "csr_split": lambda d: CsrSpmv(d, "split"),
             "dasp": lambda d: Dasp(d),
+            "dasp_mfma": lambda d: (lambda h: (lambda xx, yy: h(xx, yy, kernel=1)))(Dasp(d)),
             "dbsr": lambda d: Dbsr(d),
+            "dbsr_mfma": lambda d: (lambda h: (lambda xx, yy: h(xx, yy, kernel=1)))(Dbsr(d)),
             "rocsparse": lambda d: RocsparseSpmv(d, 0),
         }
         # the autotuner picks a layout once (cache-warm timing of every

@@ -158,6 +158,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_dasp_build": (None, [C.POINTER(CsrDesc), f64, i32, vp, C.POINTER(vp)]),
     "bsp_dasp_view_get": (None, [vp, C.POINTER(DaspView)]),
     "bsp_dasp_spmv": (None, [vp, vp, vp, C.c_int, vp]),
+    "bsp_dasp_spmv_ex": (None, [vp, C.c_int, vp, vp, C.c_int, vp]),
     "bsp_dasp_destroy": (None, [vp]),
     "bsp_autotune": (None, [C.POINTER(CsrDesc), C.POINTER(TuneOptions), vp, C.POINTER(vp)]),
     "bsp_tuned_from_record": (None, [C.POINTER(CsrDesc), C.POINTER(TuneRecord), vp,
@@ -174,6 +175,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_dbsr_build": (None, [C.POINTER(CsrDesc), vp, C.POINTER(vp)]),
     "bsp_dbsr_view_get": (None, [vp, C.POINTER(DbsrView)]),
     "bsp_dbsr_spmv": (None, [vp, vp, vp, C.c_int, vp]),
+    "bsp_dbsr_spmv_ex": (None, [vp, C.c_int, vp, vp, C.c_int, vp]),
     "bsp_dbsr_destroy": (None, [vp]),
     "bsp_status_string": (C.c_char_p, [C.c_int]),
     "bsp_last_error": (C.c_char_p, []),

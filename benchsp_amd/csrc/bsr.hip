@@ -722,6 +722,52 @@ __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
   }
 }
 
+// DBSR on v_mfma_f64_16x16x4f64 (the reference's DBSR kernels use
+// mma.m8n8k4 with a diagonal pick, src/amgT.cu:324-387): a wave per 4-row
+// group, four of its blocks per product. Lane (i, t) = (l%16, l/16) loads
+// slot t of row i%4 of block i/4 of the step (value and its column's x), as
+// A[i][t] and B[t][i], so the result's diagonal C[i][i] is block i/4's row
+// i%4 over its four slots (C layout as in bsr4_mfma_f64); y row r of the
+// group is the sum of C[4a+r][4a+r] over a.
+__global__ __launch_bounds__(kTB) void dbsr_spmv_f64_mfma(
+    const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
+    const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
+    double* __restrict__ y, int64_t brows, int64_t nrows, int sorted_order, int64_t first) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t g = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (g >= brows) return;  // whole wave exits together
+  const int i = lane & 15, t = lane >> 4;
+  const int off = (i >> 2) * 16 + (i & 3) * 4 + t;  // block i/4 of the step, row i%4, slot t
+  const int s0 = blcptr[g], e0 = blcptr[g + 1];
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = s0; k0 < e0; k0 += 16) {  // 16 blocks' loads in flight per step
+    double av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int blk = k0 + 4 * u + (i >> 2);
+      const int64_t o = (int64_t)(k0 + 4 * u) * 16 + off;
+      const bool ok = blk < e0;
+      av[u] = ok ? ld_stream(&bval[o]) : 0.0;
+      bv[u] = ok ? x[ld_stream(&colidx[o])] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + 4 * u < e0) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+  }
+  // C[R][R] sits in acc[R/4] of lane 16(R%4) + R; row r: lanes 16r + 4a + r
+  const int R = lane & 15;
+  double d = 0.0;
+  if ((lane >> 4) == (R & 3))
+    d = (R >> 2) == 0 ? acc[0] : (R >> 2) == 1 ? acc[1] : (R >> 2) == 2 ? acc[2] : acc[3];
+  d += __shfl_xor(d, 4, 64);
+  d += __shfl_xor(d, 8, 64);
+  const int r = lane >> 4;
+  if ((lane & 15) == r) {
+    const int64_t row = 4 * g + r;
+    if (row < nrows) y[sorted_order ? row : rec[row]] = d;
+  }
+}
+
 // f[g] = 1 when 4-row group g is at most 8 blocks wide
 __global__ void dbsr_narrow_flags(const int32_t* __restrict__ blcptr, int64_t brows,
                                   int64_t* __restrict__ f) {
@@ -1170,10 +1216,24 @@ bsp_status bsp_dbsr_view_get(bsp_dbsr h, bsp_dbsr_view* v) {
 }
 
 bsp_status bsp_dbsr_spmv(bsp_dbsr h, const double* x, double* y, int sorted_order, bsp_stream s) {
+  return bsp_dbsr_spmv_ex(h, BSP_DBSR_VALU, x, y, sorted_order, s);
+}
+
+bsp_status bsp_dbsr_spmv_ex(bsp_dbsr h, int kernel, const double* x, double* y, int sorted_order,
+                            bsp_stream s) {
   BSP_REQUIRE(h, "null handle");
+  BSP_REQUIRE(kernel == BSP_DBSR_VALU || kernel == BSP_DBSR_MFMA, "unknown DBSR kernel");
   const bsp_dbsr_view& v = h->v;
   BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
   if (v.brows == 0) return BSP_OK;
+  if (kernel == BSP_DBSR_MFMA) {
+    in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
+      dbsr_spmv_f64_mfma<<<cdiv(c * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
+          v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, sorted_order, f);
+    });
+    BSP_HIP(hipGetLastError());
+    return BSP_OK;
+  }
   const int64_t waves = (h->narrow + 1) / 2 + (v.brows - h->narrow);
   in_slices(waves, kWave, [&](int64_t f, int64_t c) {
     dbsr_spmv_f64<<<cdiv(c * kWave, kTB), kTB, 0, (hipStream_t)s>>>(

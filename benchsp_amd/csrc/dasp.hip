@@ -230,7 +230,11 @@ __global__ __launch_bounds__(kTB) void block_fill(
 
 // ---- SpMV -------------------------------------------------------------------
 __global__ __launch_bounds__(kTB) void dasp_spmv_f64(dasp::DaspArgs a) {
-  dasp::dasp_block(a, a.x, a.y, blockIdx.x);
+  dasp::dasp_block<false>(a, a.x, a.y, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kTB) void dasp_spmv_f64_mfma(dasp::DaspArgs a) {
+  dasp::dasp_block<true>(a, a.x, a.y, blockIdx.x);
 }
 
 }  // namespace
@@ -448,14 +452,23 @@ bsp_status bsp_dasp_view_get(bsp_dasp h, bsp_dasp_view* v) {
 }
 
 bsp_status bsp_dasp_spmv(bsp_dasp h, const double* x, double* y, int permuted, bsp_stream s) {
+  return bsp_dasp_spmv_ex(h, BSP_DASP_VALU, x, y, permuted, s);
+}
+
+bsp_status bsp_dasp_spmv_ex(bsp_dasp h, int kernel, const double* x, double* y, int permuted,
+                            bsp_stream s) {
   BSP_REQUIRE(h, "null handle");
+  BSP_REQUIRE(kernel == BSP_DASP_VALU || kernel == BSP_DASP_MFMA, "unknown DASP kernel");
   const bsp_dasp_view& v = h->v;
   BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
   if (v.nrows == 0) return BSP_OK;
   const dasp::DaspArgs a = dasp::dasp_args(v, x, y, permuted);
   const int64_t grid = a.wg_long + a.wg_reg + a.wg_short;
   if (grid == 0) return BSP_OK;
-  dasp_spmv_f64<<<grid, kTB, 0, (hipStream_t)s>>>(a);
+  if (kernel == BSP_DASP_MFMA)
+    dasp_spmv_f64_mfma<<<grid, kTB, 0, (hipStream_t)s>>>(a);
+  else
+    dasp_spmv_f64<<<grid, kTB, 0, (hipStream_t)s>>>(a);
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

@@ -41,16 +41,63 @@ __device__ __forceinline__ void dasp_store(const DaspArgs& a, double* y, int64_t
   y[a.permuted ? k : a.order[k]] = v;
 }
 
+typedef double dbl4m __attribute__((ext_vector_type(4)));
+
+// Diagonal of the 16x16 result of v_mfma_f64_16x16x4f64 (lane l holds
+// C[l/16 + 4v][l%16] in acc[v]): C[R][R] sits in acc[R/4] of lane
+// 16(R%4) + R; every other lane returns 0.
+__device__ __forceinline__ double mfma_diag(const dbl4m& acc, int lane) {
+  const int R = lane & 15;
+  if ((lane >> 4) != (R & 3)) return 0.0;
+  const int v = R >> 2;
+  return v == 0 ? acc[0] : v == 1 ? acc[1] : v == 2 ? acc[2] : acc[3];
+}
+
 // Workgroup `bid` of the fused launch: [long rows: a workgroup each][row
 // blocks: a wave each][short and empty rows: a thread each]. Output position
 // k is the row's place in order_rid. x and y are passed apart from the
 // layout's arguments, so the batched step reads those from global memory
 // as they are and supplies its own vectors.
+//
+// MF: long rows and row blocks on FP64 MFMA, as the reference's fused
+// kernel does on mma.m8n8k4 (src/dasp.cu:405-812): lane l loads one entry
+// (value as A[l%16][l/16], its gathered x as B[l/16][l%16]), so
+// v_mfma_f64_16x16x4f64's diagonal C[g][g] is the sum over the four entries
+// of A row g. A row block's 64-entry step stacks its two 8x4 slabs as A
+// rows 0..7 and 8..15 (row g of the block is C[g][g] + C[g+8][g+8]); a long
+// row's 64-entry window is 16 pieces of four entries (the row sum is the
+// trace). Short rows stay on VALU in both.
+template <bool MF = false>
 __device__ __forceinline__ void dasp_block(const DaspArgs& a, const double* __restrict__ x,
                                            double* __restrict__ y, int64_t bid) {
   __shared__ double red[kTB / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
-  if (bid < a.wg_long) {
+  if (MF && bid < a.wg_long) {
+    const int i = (int)bid;
+    const int64_t base = (int64_t)a.lrpt[i] * 64;
+    const int64_t nwin = a.lrpt[i + 1] - a.lrpt[i];  // 64-entry windows, zero padded
+    const int off = 4 * (lane & 15) + (lane >> 4);  // entry of this lane in a window
+    dbl4m acc = {0.0, 0.0, 0.0, 0.0};
+    int64_t wdw = tid / kWave;
+    for (; wdw + 4 < nwin; wdw += 8) {  // two windows' loads in flight
+      const int64_t p0 = base + wdw * 64 + off, p1 = p0 + 4 * 64;
+      const double v0 = ld_stream(&a.lv[p0]), v1 = ld_stream(&a.lv[p1]);
+      const double x0 = x[ld_stream(&a.lc[p0])], x1 = x[ld_stream(&a.lc[p1])];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, x0, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, x1, acc, 0, 0, 0);
+    }
+    if (wdw < nwin) {
+      const int64_t p = base + wdw * 64 + off;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ld_stream(&a.lv[p]), x[ld_stream(&a.lc[p])],
+                                                 acc, 0, 0, 0);
+    }
+    const double v = group_sum<kWave>(mfma_diag(acc, lane));
+    if (lane == 0) red[tid / kWave] = v;
+    __syncthreads();
+    if (tid == 0) dasp_store(a, y, i, (red[0] + red[1]) + (red[2] + red[3]));
+    return;
+  }
+  if (!MF && bid < a.wg_long) {
     const int i = (int)bid;
     const int64_t base = (int64_t)a.lrpt[i] * 64;
     const int64_t len = (int64_t)(a.lrpt[i + 1] - a.lrpt[i]) * 64;  // zero padded
@@ -67,7 +114,41 @@ __device__ __forceinline__ void dasp_block(const DaspArgs& a, const double* __re
     if (tid == 0) dasp_store(a, y, i, (red[0] + red[1]) + (red[2] + red[3]));
     return;
   }
-  if (bid < a.wg_long + a.wg_reg) {
+  if (MF && bid < a.wg_long + a.wg_reg) {
+    const int b = (int)((bid - a.wg_long) * (kTB / kWave) + tid / kWave);
+    if (b >= a.blocknum) return;
+    const int p0 = a.bptr[b], p1 = a.bptr[b + 1];
+    const int g = lane & 15;
+    const int off = (g >> 3) * 32 + (g & 7) * 4 + (lane >> 4);  // slab, row, column
+    dbl4m acc = {0.0, 0.0, 0.0, 0.0};
+    for (int base = p0; base < p1; base += 2 * kWave) {  // two steps' loads in flight
+      double v[2], xv[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int p = base + u * kWave + off;
+        v[u] = p < p1 ? ld_stream(&a.rv[p]) : 0.0;
+        xv[u] = p < p1 ? x[ld_stream(&a.rc[p])] : 0.0;
+      }
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v[0], xv[0], acc, 0, 0, 0);
+      if (base + kWave < p1) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v[1], xv[1], acc, 0, 0, 0);
+    }
+    // row g (0..7): C[g][g] on lane 16(g%4) + g plus C[g+8][g+8] eight lanes up
+    const double d = mfma_diag(acc, lane);
+    const double rs = d + __shfl_down(d, 8, 64);
+    const int gq = lane >> 3, q = lane & 7;  // lanes gq*8 + q: row gq's irregular tail
+    const double reg = __shfl(rs, 16 * (gq & 3) + gq, 64);
+    const int k = b * BS + gq;
+    double tail = 0.0;
+    if (k < a.nb)
+      for (int t = a.irpt[k] + q, e = a.irpt[k + 1]; t < e; t += 8)
+        tail += ld_stream(&a.iv[t]) * x[ld_stream(&a.ic[t])];
+    tail += __shfl_xor(tail, 1, 64);
+    tail += __shfl_xor(tail, 2, 64);
+    tail += __shfl_xor(tail, 4, 64);
+    if (q == 0 && k < a.nb) dasp_store(a, y, (int64_t)a.nl + k, reg + tail);
+    return;
+  }
+  if (!MF && bid < a.wg_long + a.wg_reg) {
     const int b = (int)((bid - a.wg_long) * (kTB / kWave) + tid / kWave);
     if (b >= a.blocknum) return;
     const int p0 = a.bptr[b], p1 = a.bptr[b + 1];

@@ -170,10 +170,14 @@ class Dbsr:
                     map=_to_host(v.map, v.nblocks, np.uint16),
                     prefix=_to_host(v.prefix, v.warpnum + 1, np.int32))
 
-    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, sorted_order=False):
+    VALU, MFMA = 0, 1
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, sorted_order=False,
+                 kernel: int = 0):
         if y is None:
             y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
-        lib().bsp_dbsr_spmv(self._h, ptr(x), ptr(y), int(sorted_order), stream_handle())
+        lib().bsp_dbsr_spmv_ex(self._h, int(kernel), ptr(x), ptr(y), int(sorted_order),
+                               stream_handle())
         return y
 
     def __del__(self):
@@ -225,10 +229,14 @@ class Dasp:
             irreg_cid=_to_host(v.irreg_cid, v.nnz_irreg, np.int32))
         return out
 
-    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, permuted=False):
+    VALU, MFMA = 0, 1
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None, permuted=False,
+                 kernel: int = 0):
         if y is None:
             y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
-        lib().bsp_dasp_spmv(self._h, ptr(x), ptr(y), int(permuted), stream_handle())
+        lib().bsp_dasp_spmv_ex(self._h, int(kernel), ptr(x), ptr(y), int(permuted),
+                               stream_handle())
         return y
 
     def __del__(self):

@@ -209,6 +209,20 @@ bsp_status bsp_dbsr_view_get(bsp_dbsr h, bsp_dbsr_view* v);
  * order, un-permuted by the caller through recorderMap); otherwise y is in
  * the original row order. */
 bsp_status bsp_dbsr_spmv(bsp_dbsr h, const double* x, double* y, int sorted_order, bsp_stream s);
+typedef enum bsp_dbsr_kernel {
+  /* groups at most 8 blocks wide two per wave, wider ones a wave each; per
+   * lane one slot's value and column per block (VALU FP64) */
+  BSP_DBSR_VALU = 0,
+  /* a wave per 4-row group on v_mfma_f64_16x16x4f64, four blocks per
+   * product, each element's value and gathered x as A and B operands and
+   * the row sums read off the result's diagonal: the counterpart of the
+   * reference's mma.m8n8k4 DBSR kernels (src/amgT.cu:324-461), with x used
+   * (the reference overrides it with 1.0, :356, :425) */
+  BSP_DBSR_MFMA = 1
+} bsp_dbsr_kernel;
+/* bsp_dbsr_spmv with a choice of kernel (bsp_dbsr_spmv = BSP_DBSR_VALU). */
+bsp_status bsp_dbsr_spmv_ex(bsp_dbsr h, int kernel, const double* x, double* y, int sorted_order,
+                            bsp_stream s);
 bsp_status bsp_dbsr_destroy(bsp_dbsr h);
 
 /* ------------------------------------------------------------------------ */
@@ -249,6 +263,18 @@ bsp_status bsp_dasp_view_get(bsp_dasp h, bsp_dasp_view* v);
 /* y = A*x. permuted != 0: y[k] is row order_rid[k] (the reference's Y_val);
  * otherwise y is in the original row order. */
 bsp_status bsp_dasp_spmv(bsp_dasp h, const double* x, double* y, int permuted, bsp_stream s);
+typedef enum bsp_dasp_kernel {
+  BSP_DASP_VALU = 0, /* one FP64 FMA per entry, wave-level row sums */
+  /* long rows and row blocks on v_mfma_f64_16x16x4f64, as the reference's
+   * dasp_spmv2 on mma.m8n8k4 (src/dasp.cu:405-812): one entry per lane, the
+   * value as the A operand and its gathered x as B, row sums read off the
+   * diagonal of the result (an 8-row block's two 4-column slabs stacked as
+   * A rows 0..7 and 8..15); short rows on VALU */
+  BSP_DASP_MFMA = 1
+} bsp_dasp_kernel;
+/* bsp_dasp_spmv with a choice of kernel (bsp_dasp_spmv = BSP_DASP_VALU). */
+bsp_status bsp_dasp_spmv_ex(bsp_dasp h, int kernel, const double* x, double* y, int permuted,
+                            bsp_stream s);
 bsp_status bsp_dasp_destroy(bsp_dasp h);
 
 /* ------------------------------------------------------------------------ */

@@ -150,15 +150,17 @@ def test_dbsr_builder_bit_exact(oracle, bsp_host):
             _same(g[k], o[k], f"{name} {k}")
 
 
-def test_dbsr_spmv(oracle, bsp_host):
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+def test_dbsr_spmv(oracle, bsp_host, kernel):
     import torch
     from benchsp_amd.device import DeviceCsr, Dbsr
+    kern = Dbsr.MFMA if kernel == "mfma" else Dbsr.VALU
     for name, A in _dbsr_inputs(bsp_host):
         x = bsp_host.uniform(A.ncols, 5)
         D = Dbsr(DeviceCsr.from_host(A))
         xd = torch.from_numpy(x).cuda()
-        y = D(xd).cpu().numpy()
-        ys = D(xd, sorted_order=True).cpu().numpy()
+        y = D(xd, kernel=kern).cpu().numpy()
+        ys = D(xd, sorted_order=True, kernel=kern).cpu().numpy()
         y_ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
         bound = abs_spmv(oracle, A, x)
         assert_spmv_close(y, y_ref, bound, what=f"{name} dbsr")
@@ -197,16 +199,18 @@ def test_dasp_builder_bit_exact(oracle, bsp_host):
             _same(g[k], o[k], f"{name} {k}")
 
 
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
 @pytest.mark.parametrize("threshold,longest", [(0.75, 256), (0.5, 64), (1.0, 1000)])
-def test_dasp_spmv(oracle, bsp_host, threshold, longest):
+def test_dasp_spmv(oracle, bsp_host, threshold, longest, kernel):
     import torch
     from benchsp_amd.device import Dasp, DeviceCsr
+    kern = Dasp.MFMA if kernel == "mfma" else Dasp.VALU
     for name, A in _dasp_inputs(bsp_host):
         x = bsp_host.uniform(A.ncols, 8)
         D = Dasp(DeviceCsr.from_host(A), threshold, longest)
         xd = torch.from_numpy(x).cuda()
-        y = D(xd).cpu().numpy()
-        yp = D(xd, permuted=True).cpu().numpy()
+        y = D(xd, kernel=kern).cpu().numpy()
+        yp = D(xd, permuted=True, kernel=kern).cpu().numpy()
         y_ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
         assert_spmv_close(y, y_ref, abs_spmv(oracle, A, x), what=f"{name} dasp")
         order = D.arrays()["order_rid"]
