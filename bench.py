@@ -366,6 +366,7 @@ def formats_leg(args, dev, mats, xs):
         dAs = [DeviceCsr.from_host(A, dev) for _ in range(ncopy)]
         cands = {
             "csr_adaptive": lambda d: CsrSpmv(d, "adaptive"),
+            "csr_adaptive_narrow": lambda d: CsrSpmv(d, "adaptive", narrow_cols=1),
             "csr_scalar": lambda d: CsrSpmv(d, "scalar"),
             "csr_balanced": lambda d: CsrSpmv(d, "balanced"),
             "csr_split": lambda d: CsrSpmv(d, "split"),

@@ -26,7 +26,8 @@ class BspError(RuntimeError):
 
 class CsrOptions(C.Structure):
     """struct bsp_csr_options (include/bsp.h)."""
-    _fields_ = [("seg_cols", i64), ("chunk", i32), ("seg_mode", i32), ("reserved", i64 * 6)]
+    _fields_ = [("seg_cols", i64), ("chunk", i32), ("seg_mode", i32), ("narrow_cols", i32),
+                ("reserved32", i32), ("reserved", i64 * 5)]
 
 
 class CsrDesc(C.Structure):
@@ -184,6 +185,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_csr_plan_create_ex": (None, [C.POINTER(CsrDesc), C.c_int, C.c_void_p, vp,
                                        C.POINTER(vp)]),
     "bsp_csr_plan_segments": (None, [vp, i64p, i64p]),
+    "bsp_csr_plan_narrow": (None, [vp, C.POINTER(i32)]),
     "bsp_csr_spmv": (None, [vp, vp, vp, vp]),
     "bsp_csr_plan_info": (None, [vp, i64p, i64p]),
     "bsp_plan_destroy": (None, [vp]),

@@ -155,6 +155,12 @@ bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* ns
   return BSP_OK;
 }
 
+bsp_status bsp_csr_plan_narrow(bsp_spmv_plan p, int32_t* narrow) {
+  BSP_REQUIRE(p && narrow, "null argument");
+  *narrow = p->narrow ? 1 : 0;
+  return BSP_OK;
+}
+
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s) {
   BSP_REQUIRE(p, "null plan");
   BSP_REQUIRE(p->A.dtype == BSP_F64, "plan has FP16 values: use bsp_csr_spmv_f16");

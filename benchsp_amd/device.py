@@ -56,11 +56,11 @@ class CsrSpmv:
     """y = A*x with a per-matrix schedule built on the GPU (bsp_csr_plan_create)."""
 
     def __init__(self, A: DeviceCsr, method: str | int = "adaptive", seg_cols: int = 0,
-                 chunk: int = 0, seg_mode: int = 0):
+                 chunk: int = 0, seg_mode: int = 0, narrow_cols: int = 0):
         self.A = A
         self.method = METHODS[method] if isinstance(method, str) else int(method)
         self._desc = A.desc()
-        self._opt = CsrOptions(seg_cols, chunk, seg_mode)
+        self._opt = CsrOptions(seg_cols, chunk, seg_mode, narrow_cols)
         h = C.c_void_p()
         lib().bsp_csr_plan_create_ex(C.byref(self._desc), self.method, C.byref(self._opt),
                                      stream_handle(), C.byref(h))
@@ -71,6 +71,12 @@ class CsrSpmv:
         w, k = i64(), i64()
         lib().bsp_csr_plan_segments(self._h, C.byref(w), C.byref(k))
         return w.value, k.value
+
+    def narrow(self) -> bool:
+        """True when the plan reads 16-bit column offsets."""
+        v = C.c_int32()
+        lib().bsp_csr_plan_narrow(self._h, C.byref(v))
+        return bool(v.value)
 
     def info(self) -> tuple[int, int]:
         nb, nl = i64(), i64()

@@ -98,7 +98,15 @@ typedef struct bsp_csr_options {
    * kernel; automatic width then targets 2 MB of x per segment),
    * 1 = one launch per segment, later ones accumulating into y */
   int32_t seg_mode;
-  int64_t reserved[6];
+  /* unsegmented plans: > 0 = use 16-bit column offsets where possible,
+   * 0 (default) or < 0 = int32 indices. When every block's column indices
+   * lie within int16 of its first row (banded matrices such as stencils)
+   * and the values are 16-byte aligned, the plan then keeps 16-bit offsets
+   * from each block's first row instead of the int32 indices (2 bytes per
+   * entry less to read; y bit-identical) */
+  int32_t narrow_cols;
+  int32_t reserved32;
+  int64_t reserved[5];
 } bsp_csr_options;
 
 /* Build the per-matrix schedule for `method` on the GPU (stream-ordered).
@@ -111,6 +119,9 @@ bsp_status bsp_csr_plan_create_ex(const bsp_csr* A, int method, const bsp_csr_op
 /* Column-segment width of an adaptive plan (0 = not segmented) and the
  * number of segments. */
 bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* nseg);
+/* 1 when an adaptive plan reads 16-bit column offsets (bsp_csr_options
+ * narrow_cols), else 0. */
+bsp_status bsp_csr_plan_narrow(bsp_spmv_plan p, int32_t* narrow);
 /* y = A*x. x has ncols, y has nrows doubles (device). Stream-ordered. */
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s);
 /* Number of workgroups one bsp_csr_spmv launch uses (its main kernel). */

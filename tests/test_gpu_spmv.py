@@ -197,6 +197,48 @@ def test_adaptive_variants(oracle, bsp_host, chunk):
                               abs_spmv(oracle, A, x), what=f"n={A.nrows} {chunk}/{seg}")
 
 
+def _diag_plus(n, extra_row, extra_col):
+    """n x n identity-like CSR (one entry per row on the diagonal, ragged
+    values) plus one entry (extra_row, extra_col)."""
+    from benchsp_amd.host import Csr
+    rows = np.concatenate([np.arange(n), [extra_row]])
+    cols = np.concatenate([np.arange(n), [extra_col]])
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rng = np.random.default_rng(n + extra_col)
+    return Csr(n, n, np.cumsum(rp), cols.astype(np.int32), rng.uniform(-1, 1, rows.size))
+
+
+@pytest.mark.parametrize("chunk", [1024, 2048])
+def test_adaptive_narrow_cols(oracle, bsp_host, chunk):
+    # 16-bit column offsets: the same bits as the int32 plan wherever every
+    # block's columns lie within int16 of its first row, and refused past it
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    cases = [("stencil27-64", bsp_host.stencil27_64(), True),
+             ("stencil7-80", bsp_host.stencil3d(80, 80, 80, 7), True),
+             ("powerlaw-300k", bsp_host.powerlaw(300_000, 20.0, 1.2, 5000, seed=1), False),
+             ("edge+32767", _diag_plus(40_000, 0, 32767), True),
+             ("edge+32768", _diag_plus(40_000, 0, 32768), False),
+             ("edge-32768", _diag_plus(40_000, 32768, 0), True),
+             ("edge-far", _diag_plus(40_000, 39_999, 0), False)]
+    for name, A, expect in cases:
+        d = DeviceCsr.from_host(A)
+        x = bsp_host.uniform(A.ncols, 6)
+        xd = torch.from_numpy(x).cuda()
+        on = CsrSpmv(d, "adaptive", seg_cols=-1, chunk=chunk, narrow_cols=1)
+        off = CsrSpmv(d, "adaptive", seg_cols=-1, chunk=chunk)
+        if expect is not None:
+            assert on.narrow() == expect, name
+        assert not off.narrow()
+        y_on, y_off = on(xd).cpu().numpy(), off(xd).cpu().numpy()
+        assert np.array_equal(y_on, y_off), name
+        assert_spmv_close(y_on, oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                          abs_spmv(oracle, A, x), what=f"{name} narrow")
+
+
 def test_adaptive_segments_reject_unsorted():
     import torch
     from benchsp_amd import BspError
