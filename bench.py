@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--warm", action="store_true", help="single copy (cache-warm) timing")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="time the power-law SpMV's gathers and streams without its arithmetic")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB three times (counter calibration run)")
     ap.add_argument("--traffic-summary", nargs=4, metavar="CSV",
@@ -517,6 +519,53 @@ def copy_probe():
     print("copied 256 MiB x3", flush=True)
 
 
+def gather_probe():
+    """The power-law 2^20 SpMV's data movement without its arithmetic
+    (bsp_csr_plan_probe over the same XCD-pinned column segments): x gathers
+    with the column-index stream, then with the value stream too, beside the
+    SpMV launch itself (kernel + partial sums). Two rotated copies of the
+    matrix, as in the headline leg; median of 20 event-timed launches each.
+    Prints one JSON line (microseconds)."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from benchsp_amd import host
+    from benchsp_amd._lib import lib
+    from benchsp_amd.device import CsrSpmv, DeviceCsr, ptr, stream_handle
+    A = host.powerlaw_1m()
+    xd = torch.from_numpy(host.uniform(A.ncols, 1)).cuda()
+    y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+    ops = [CsrSpmv(DeviceCsr.from_host(A), "adaptive") for _ in range(2)]
+    g = C.c_int64()
+    lib().bsp_csr_plan_probe(ops[0]._h, 0, 0, None, 0, C.byref(g), None)
+    sink = torch.empty(g.value, dtype=torch.float64, device="cuda")
+
+    def probe(i, vals):
+        lib().bsp_csr_plan_probe(ops[i % 2]._h, vals, ptr(xd), ptr(sink), sink.numel(), None,
+                                 stream_handle())
+
+    runs = {"gathers_col_stream": lambda i: probe(i, 0),
+            "gathers_col_val_stream": lambda i: probe(i, 1),
+            "spmv": lambda i: ops[i % 2](xd, y)}
+    out = {"matrix": f"power-law 2^20, nnz={A.nnz}, {ops[0].segments()[1]} XCD-pinned segments",
+           "gathers": A.nnz, "workgroups": g.value}
+    for name, fn in runs.items():
+        for i in range(4):
+            fn(i)
+        ts = []
+        for i in range(20):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn(i)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e3)
+        us = float(np.median(ts))
+        out[name + "_us"] = round(us, 2)
+        out[name + "_Greq_per_s"] = round(A.nnz / (us * 1e-6) / 1e9, 1)
+    print(json.dumps(out), flush=True)
+
+
 def measured_peaks() -> dict:
     """What the box's HBM delivers to plain torch kernels: a 1 GiB device
     copy (bytes read + written over time) and a 1 GiB FP64 sum (bytes read
@@ -640,6 +689,9 @@ def main():
         return
     if args.copy_probe:
         copy_probe()
+        return
+    if args.gather_probe:
+        gather_probe()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library "

@@ -155,6 +155,12 @@ bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* ns
   return BSP_OK;
 }
 
+bsp_status bsp_csr_plan_probe(bsp_spmv_plan p, int with_values, const double* x, double* sink,
+                             int64_t sink_len, int64_t* grid, bsp_stream s) {
+  BSP_REQUIRE(p && (x || !sink), "null argument");
+  return csr_plan_probe(p, with_values, x, sink, sink_len, grid, (hipStream_t)s);
+}
+
 bsp_status bsp_csr_plan_narrow(bsp_spmv_plan p, int32_t* narrow) {
   BSP_REQUIRE(p && narrow, "null argument");
   *narrow = p->narrow ? 1 : 0;

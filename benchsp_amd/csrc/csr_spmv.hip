@@ -114,6 +114,42 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_xcd_f64(const SegDesc* __res
   adaptive_block<CHUNK, false, false>(d.rowptr, d.col, d.val, x, d.out, d.blk, b);
 }
 
+// Data movement of the column-segmented launch without its arithmetic: on
+// the same XCD pinning (workgroup i on XCD i % 8 serves segment (i % 8) %
+// nseg), each workgroup streams 2048 consecutive entries of its segment's
+// column indices (and values, VALS) and gathers x at every one; a sum per
+// workgroup is stored so nothing is optimised away.
+template <bool VALS>
+__global__ __launch_bounds__(kWG) void probe_gather(const SegDesc* __restrict__ segs, int nseg,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ sink) {
+  __shared__ double red[kWG / kWave];
+  const int xcd = blockIdx.x % 8;
+  const int64_t local = blockIdx.x / 8;
+  const int s = xcd % nseg, q = xcd / nseg, reps = 8 / nseg;
+  const SegDesc d = segs[s];
+  const int64_t nnz = d.blk[d.nblk].y;  // the segment's last window bound
+  const int64_t b0 = (local * reps + q) * 2048;
+  double acc = 0.0;
+  if (b0 < nnz) {
+    constexpr int IT = 2048 / kWG;
+    int c[IT];
+    double v[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t j = min(b0 + threadIdx.x + k * kWG, nnz - 1);
+      c[k] = ld_stream(&d.col[j]);
+      v[k] = VALS ? ld_stream(&d.val[j]) : 1.0;
+    }
+#pragma unroll
+    for (int k = 0; k < IT; ++k) acc += v[k] * x[c[k]];
+  }
+  acc = group_sum<kWave>(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sink[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void sum_partials(const double* __restrict__ part, int nseg, int64_t n,
                              double* __restrict__ y) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -653,6 +689,27 @@ bsp_status csr_plan_build(CsrPlan* p, hipStream_t s) {
     default:
       return fail(BSP_ERR_INVALID, "unknown CSR SpMV method " + std::to_string(p->method));
   }
+  return BSP_OK;
+}
+
+bsp_status csr_plan_probe(const CsrPlan* p, int with_values, const double* x, double* sink,
+                          int64_t sink_len, int64_t* grid_out, hipStream_t s) {
+  if (p->method != BSP_CSR_ADAPTIVE || !p->xcd_mode)
+    return fail(BSP_ERR_UNSUPPORTED, "the gather probe runs over an XCD-pinned segmented plan");
+  const int nseg = (int)p->seg.size();
+  int64_t most = 0;
+  for (int k = 0; k < nseg; ++k)
+    most = std::max(most, cdiv(cdiv(p->seg[k].nnz, 2048), 8 / nseg));
+  const int64_t grid = 8 * most;
+  if (grid_out) *grid_out = grid;
+  if (!sink || grid == 0) return BSP_OK;  // grid query
+  BSP_REQUIRE(sink_len >= grid, "sink must hold one double per workgroup");
+  const auto* tab = reinterpret_cast<const SegDesc*>(p->segtab.get());
+  if (with_values)
+    probe_gather<true><<<grid, kWG, 0, s>>>(tab, nseg, x, sink);
+  else
+    probe_gather<false><<<grid, kWG, 0, s>>>(tab, nseg, x, sink);
+  BSP_HIP(hipGetLastError());
   return BSP_OK;
 }
 

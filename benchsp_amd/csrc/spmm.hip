@@ -120,28 +120,36 @@ __global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ id
   }
 }
 
-// Y[row, :] = sum of its chunk partials, in chunk order.
+// Y[row, :] = sum of its chunk partials; a workgroup per row. Thread t owns
+// feature t % F of chunk group t / F (256 / F groups): group g adds chunks
+// g, g + G, g + 2G, ... in four interleaved running sums, and the groups'
+// sums are added in group order through LDS. Fixed order throughout: the
+// same bits every run. (A wave per row with four loads in flight took
+// 15 us on a graph whose largest row has 188 chunks.)
 __global__ __launch_bounds__(kTB) void spmm_combine(const int4* __restrict__ comb, int64_t ncomb,
                                                     const float* __restrict__ partial, int64_t F,
                                                     float* __restrict__ Y) {
-  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
+  __shared__ float red[kTB];
+  const int64_t w = blockIdx.x;
   if (w >= ncomb) return;
   const int4 c = comb[w];
-  for (int64_t f = lane; f < F; f += kWave) {
-    // four interleaved running sums (chunk k goes to sum k % 4), added in a
-    // fixed order at the end: deterministic, four loads in flight
-    const float* p = partial + (int64_t)c.y * F + f;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int k = 0;
-    for (; k + 4 <= c.z; k += 4) {
-      s0 += p[(int64_t)k * F];
-      s1 += p[(int64_t)(k + 1) * F];
-      s2 += p[(int64_t)(k + 2) * F];
-      s3 += p[(int64_t)(k + 3) * F];
-    }
-    for (; k < c.z; ++k) s0 += p[(int64_t)k * F];
-    Y[(int64_t)c.x * F + f] = (s0 + s1) + (s2 + s3);
+  const int G = kTB / (int)F, f = threadIdx.x % (int)F, g = threadIdx.x / (int)F;
+  const float* p = partial + (int64_t)c.y * F + f;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int k = g;
+  for (; k + 3 * G < c.z; k += 4 * G) {
+    s0 += p[(int64_t)k * F];
+    s1 += p[(int64_t)(k + G) * F];
+    s2 += p[(int64_t)(k + 2 * G) * F];
+    s3 += p[(int64_t)(k + 3 * G) * F];
+  }
+  for (; k < c.z; k += G) s0 += p[(int64_t)k * F];
+  red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (g == 0) {
+    float t = red[f];
+    for (int q = 1; q < G; ++q) t += red[q * F + f];
+    Y[(int64_t)c.x * F + f] = t;
   }
 }
 
@@ -350,8 +358,8 @@ static bsp_status run_spmm_t(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd
   }
   BSP_HIP(hipGetLastError());
   if (sd.ncombine > 0) {
-    spmm_combine<<<cdiv(sd.ncombine * kWave, kTB), kTB, 0, s>>>(sd.combine.get(), sd.ncombine,
-                                                                 p->partial.get(), F, Y);
+    spmm_combine<<<sd.ncombine, kTB, 0, s>>>(sd.combine.get(), sd.ncombine, p->partial.get(), F,
+                                              Y);
     BSP_HIP(hipGetLastError());
   }
   return BSP_OK;

@@ -122,6 +122,15 @@ bsp_status bsp_csr_plan_segments(bsp_spmv_plan p, int64_t* seg_cols, int64_t* ns
 /* 1 when an adaptive plan reads 16-bit column offsets (bsp_csr_options
  * narrow_cols), else 0. */
 bsp_status bsp_csr_plan_narrow(bsp_spmv_plan p, int32_t* narrow);
+/* Measurement aid, no counterpart in the reference: the data movement of an
+ * XCD-pinned column-segmented adaptive plan without its arithmetic. Each
+ * workgroup streams 2048 entries of its segment's column indices (and
+ * values, with_values != 0) on the segment's XCDs and gathers x at each,
+ * storing one sum per workgroup in sink. sink == NULL returns the number of
+ * workgroups in *grid only. Stream-ordered, one launch. BSP_ERR_UNSUPPORTED
+ * for other plans. */
+bsp_status bsp_csr_plan_probe(bsp_spmv_plan p, int with_values, const double* x, double* sink,
+                              int64_t sink_len, int64_t* grid, bsp_stream s);
 /* y = A*x. x has ncols, y has nrows doubles (device). Stream-ordered. */
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s);
 /* Number of workgroups one bsp_csr_spmv launch uses (its main kernel). */
