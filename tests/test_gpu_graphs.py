@@ -11,6 +11,14 @@ from tests._parity import abs_spmv, assert_spmv_close
 pytestmark = pytest.mark.gpu
 
 
+def _noalloc(dA, x, y):
+    import ctypes as C
+    from benchsp_amd._lib import lib
+    from benchsp_amd.device import ptr, stream_handle
+    d = dA.desc()
+    lib().bsp_csr_spmv_noalloc(C.byref(d), ptr(x), ptr(y), stream_handle())
+
+
 def _ops(bsp_host):
     from benchsp_amd.device import Bsr, CsrSpmv, Dasp, Dbsr, DeviceCsr
     mats = {"27pt-64": bsp_host.stencil27_64(),
@@ -20,9 +28,16 @@ def _ops(bsp_host):
         for method in ("adaptive", "scalar", "balanced", "split"):
             yield f"{name}/{method}", A, dA, CsrSpmv(dA, method)
         yield f"{name}/dasp", A, dA, Dasp(dA)
+        yield f"{name}/dasp-mfma", A, dA, (lambda h: (lambda x, y: h(x, y, kernel=Dasp.MFMA)))(
+            Dasp(dA))
         yield f"{name}/dbsr", A, dA, Dbsr(dA)
+        yield f"{name}/dbsr-mfma", A, dA, (lambda h: (lambda x, y: h(x, y, kernel=Dbsr.MFMA)))(
+            Dbsr(dA))
         B = Bsr(dA)
-        yield f"{name}/bsr4-mfma", A, dA, (lambda b: (lambda x, y: b(x, y, kernel=Bsr.MFMA)))(B)
+        for kname, k in (("mfma", Bsr.MFMA), ("mfma-balanced", Bsr.MFMA_BALANCED),
+                         ("valu-balanced", Bsr.VALU_BALANCED)):
+            yield f"{name}/bsr4-{kname}", A, dA, (lambda b, k: (lambda x, y: b(x, y, kernel=k)))(B, k)
+        yield f"{name}/noalloc", A, dA, (lambda d: (lambda x, y: _noalloc(d, x, y)))(dA)
     # forced column segments, both modes (XCD-pinned one launch; one launch per segment)
     A = mats["powerlaw-300k"]
     dA = DeviceCsr.from_host(A)
