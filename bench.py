@@ -24,11 +24,16 @@ CSR SpMM in FP32; each step is replayed from a captured graph, and the
 forward's error against a dense FP64 product is reported.
 
 Formats leg: the same two matrices through every SpMV format and method of
-the library (scalar / balanced / split / adaptive CSR, 4x4 BSR on MFMA and
-VALU, DBSR, DASP), the layout the per-matrix autotuner picks, and
-rocSPARSE, as CSR-equivalent effective GB/s, with the wall time of each GPU
-format build (plan, analysis or format conversion from a device CSR). The CPU baseline leg times the restated host builders of the
-BSR4, DBSR and DASP formats on one core beside it.
+the library (scalar / balanced / split / adaptive CSR, adaptive CSR with
+16-bit column offsets, 4x4 BSR on MFMA and VALU per block row and balanced
+over 64-block chunks, DBSR and DASP on VALU and on FP64 MFMA), the layout
+the per-matrix autotuner picks, and rocSPARSE, as CSR-equivalent effective
+GB/s, with the wall time of each GPU format build (plan, analysis or format
+conversion from a device CSR). The CPU baseline leg times the restated host
+builders of the BSR4, DBSR and DASP formats on one core beside it.
+
+`--gather-probe` times the power-law matrix's x gathers and streams without
+the SpMV's arithmetic, beside the SpMV launch (the bound of its layout).
 
 GNN leg: the differentiable CSR SpMM (forward, transpose-free A^T dY,
 SDDMM) on a 200,000-node power-law graph with 64 features, FP32 and BF16.
