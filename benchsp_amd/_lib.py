@@ -186,6 +186,7 @@ _SIGS: dict[str, tuple] = {
                                        C.POINTER(vp)]),
     "bsp_csr_plan_segments": (None, [vp, i64p, i64p]),
     "bsp_csr_plan_narrow": (None, [vp, C.POINTER(i32)]),
+    "bsp_csr_check": (None, [C.POINTER(CsrDesc), vp, C.POINTER(i32)]),
     "bsp_csr_plan_probe": (None, [vp, C.c_int, vp, vp, i64, i64p, vp]),
     "bsp_csr_spmv": (None, [vp, vp, vp, vp]),
     "bsp_csr_plan_info": (None, [vp, i64p, i64p]),

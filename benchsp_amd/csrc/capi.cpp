@@ -161,6 +161,21 @@ bsp_status bsp_csr_plan_probe(bsp_spmv_plan p, int with_values, const double* x,
   return csr_plan_probe(p, with_values, x, sink, sink_len, grid, (hipStream_t)s);
 }
 
+bsp_status bsp_csr_check(const bsp_csr* A, bsp_stream s, int32_t* sorted) {
+  BSP_TRY(check_csr_pattern(A));
+  if (sorted) *sorted = 1;
+  if (A->nrows == 0) {
+    BSP_REQUIRE(A->nnz == 0, "a CSR without rows has no entries");
+    return BSP_OK;
+  }
+  int f = 0;
+  BSP_TRY(csr_check(*A, (hipStream_t)s, &f));
+  if (f & 1) return fail(BSP_ERR_INVALID, "rowptr must start at 0, never decrease and end at nnz");
+  if (f & 2) return fail(BSP_ERR_INVALID, "column index out of range");
+  if (sorted) *sorted = (f & 4) ? 0 : 1;
+  return BSP_OK;
+}
+
 bsp_status bsp_csr_plan_narrow(bsp_spmv_plan p, int32_t* narrow) {
   BSP_REQUIRE(p && narrow, "null argument");
   *narrow = p->narrow ? 1 : 0;

@@ -47,6 +47,9 @@ using CsrPlan = bsp_spmv_plan_s;
 bsp_status csr_plan_build(CsrPlan* p, hipStream_t s);
 bsp_status csr_plan_run(const CsrPlan* p, const double* x, double* y, hipStream_t s);
 bsp_status csr_plan_run_f16(const CsrPlan* p, const uint16_t* x, float* y, hipStream_t s);
+// GPU validation of a CSR's pattern: *flags bit 1 = bad row pointers, bit 2 =
+// column out of range, bit 4 = columns not strictly ascending (synchronises).
+bsp_status csr_check(const bsp_csr& A, hipStream_t s, int* flags);
 bsp_status csr_plan_probe(const CsrPlan* p, int with_values, const double* x, double* sink,
                           int64_t sink_len, int64_t* grid, hipStream_t s);
 }  // namespace bsp

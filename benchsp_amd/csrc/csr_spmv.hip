@@ -378,6 +378,30 @@ constexpr int64_t kXcdSegCols = int64_t(1) << 18;      // 2 MB slices when XCDs 
 // row's entries), so a 5,000-entry row is 79 steps of one wave rather than
 // one thread's loop.
 
+// Validation of a device CSR, wave per row: bit 1 of *bad for a row pointer
+// out of order or out of [0, nnz], bit 2 for a column index outside
+// [0, ncols), bit 4 for columns not strictly ascending in a row.
+__global__ __launch_bounds__(kWG) void csr_check_rows(const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ col, int64_t n,
+                                                      int64_t ncols, int64_t nnz,
+                                                      int* __restrict__ bad, int64_t first) {
+  const int64_t r = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= n) return;
+  const int64_t s0 = rowptr[r], e0 = rowptr[r + 1];
+  if (s0 > e0 || s0 < 0 || e0 > nnz || (r == 0 && s0 != 0) || (r == n - 1 && e0 != nnz)) {
+    if (lane == 0) atomicOr(bad, 1);
+    return;
+  }
+  int f = 0;
+  for (int64_t j = s0 + lane; j < e0; j += kWave) {
+    const int c = col[j];
+    if (c < 0 || c >= ncols) f |= 2;
+    if (j > s0 && col[j - 1] >= c) f |= 4;
+  }
+  if (f) atomicOr(bad, f);
+}
+
 // Counts entries farther than `half` columns from the (scaled) diagonal and
 // rows whose columns are not strictly ascending.
 __global__ __launch_bounds__(kWG) void csr_scatter_stats(const int32_t* __restrict__ rowptr,
@@ -710,6 +734,19 @@ bsp_status csr_plan_probe(const CsrPlan* p, int with_values, const double* x, do
   else
     probe_gather<false><<<grid, kWG, 0, s>>>(tab, nseg, x, sink);
   BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status csr_check(const bsp_csr& A, hipStream_t s, int* flags) {
+  DevBuf<int> bad;
+  BSP_TRY(bad.alloc(1));
+  BSP_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+  in_slices(A.nrows, kWave, [&](int64_t f, int64_t c) {
+    csr_check_rows<<<cdiv(c * kWave, kWG), kWG, 0, s>>>(A.rowptr, A.colidx, A.nrows, A.ncols,
+                                                        A.nnz, bad.get(), f);
+  });
+  BSP_HIP(hipGetLastError());
+  BSP_TRY(fetch(bad.get(), flags, s));
   return BSP_OK;
 }
 

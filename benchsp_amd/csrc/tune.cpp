@@ -45,8 +45,12 @@ bsp_status check_record(const bsp_tune_record& r) {
                   "BSR8 record: kernel must be BSP_BSR_VALU or BSP_BSR_MFMA");
       return BSP_OK;
     case BSP_TUNE_DBSR:
+      BSP_REQUIRE(r.kernel == BSP_DBSR_VALU || r.kernel == BSP_DBSR_MFMA,
+                  "DBSR record: kernel must be BSP_DBSR_VALU or BSP_DBSR_MFMA");
       return BSP_OK;
     case BSP_TUNE_DASP:
+      BSP_REQUIRE(r.kernel == BSP_DASP_VALU || r.kernel == BSP_DASP_MFMA,
+                  "DASP record: kernel must be BSP_DASP_VALU or BSP_DASP_MFMA");
       BSP_REQUIRE(r.threshold > 0.0 && r.threshold <= 1.0, "DASP record: threshold in (0, 1]");
       BSP_REQUIRE(r.block_longest >= 5, "DASP record: block_longest >= 5");
       return BSP_OK;
@@ -81,8 +85,8 @@ bsp_status run(const bsp_tuned_s* h, const double* x, double* y, hipStream_t s) 
     case BSP_TUNE_CSR: return bsp_csr_spmv(h->csr, x, y, s);
     case BSP_TUNE_BSR4:
     case BSP_TUNE_BSR8: return bsp_bsr_spmv(h->bsr, h->rec.kernel, x, y, s);
-    case BSP_TUNE_DBSR: return bsp_dbsr_spmv(h->dbsr, x, y, 0, s);
-    default: return bsp_dasp_spmv(h->dasp, x, y, 0, s);
+    case BSP_TUNE_DBSR: return bsp_dbsr_spmv_ex(h->dbsr, h->rec.kernel, x, y, 0, s);
+    default: return bsp_dasp_spmv_ex(h->dasp, h->rec.kernel, x, y, 0, s);
   }
 }
 
@@ -153,11 +157,13 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
       cands.push_back(r);
     }
   }
-  if (formats & (1 << BSP_TUNE_DBSR)) {
-    bsp_tune_record r{};
-    r.format = BSP_TUNE_DBSR;
-    cands.push_back(r);
-  }
+  if (formats & (1 << BSP_TUNE_DBSR))
+    for (int kern : {BSP_DBSR_VALU, BSP_DBSR_MFMA}) {
+      bsp_tune_record r{};
+      r.format = BSP_TUNE_DBSR;
+      r.kernel = kern;
+      cands.push_back(r);
+    }
   if (formats & (1 << BSP_TUNE_DASP))
     for (double t : {0.5, 0.75, 1.0})
       for (int bl : {64, 256}) {
@@ -166,6 +172,10 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
         r.threshold = t;
         r.block_longest = bl;
         cands.push_back(r);
+        if (t == 0.75 && bl == 256) {  // the reference's setting, also on MFMA
+          r.kernel = BSP_DASP_MFMA;
+          cands.push_back(r);
+        }
       }
 
   DevBuf<double> x, y;

@@ -109,6 +109,13 @@ typedef struct bsp_csr_options {
   int64_t reserved[5];
 } bsp_csr_options;
 
+/* Validates a device CSR on the GPU before it is handed to any plan or
+ * builder (which trust their input): rowptr starts at 0, never decreases and
+ * ends at nnz, every column index lies in [0, ncols). BSP_ERR_INVALID
+ * otherwise. *sorted (may be NULL) = 1 when columns are strictly ascending
+ * in every row, as the column-segmented plan needs. Synchronises the stream. */
+bsp_status bsp_csr_check(const bsp_csr* A, bsp_stream s, int32_t* sorted);
+
 /* Build the per-matrix schedule for `method` on the GPU (stream-ordered).
  * The plan keeps the descriptor's device pointers; they must outlive it
  * (a segmented adaptive plan also holds its own regrouped copy of col/val:
@@ -312,15 +319,15 @@ bsp_status bsp_dasp_destroy(bsp_dasp h);
  * rebuild the same layout later with bsp_tuned_from_record, no timing. */
 typedef enum bsp_tune_format {
   BSP_TUNE_CSR = 0,  /* adaptive plan: chunk, seg_cols as in bsp_csr_options */
-  BSP_TUNE_BSR4 = 1, /* 4x4 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
-  BSP_TUNE_DBSR = 2,
-  BSP_TUNE_DASP = 3, /* threshold, block_longest as in bsp_dasp_build */
+  BSP_TUNE_BSR4 = 1, /* 4x4 BSR: kernel = any bsp_bsr_kernel but AUTO */
+  BSP_TUNE_DBSR = 2, /* kernel = BSP_DBSR_VALU or BSP_DBSR_MFMA */
+  BSP_TUNE_DASP = 3, /* threshold, block_longest as in bsp_dasp_build; kernel = bsp_dasp_kernel */
   BSP_TUNE_BSR8 = 4  /* 8x8 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
 } bsp_tune_format;
 
 typedef struct bsp_tune_record {
   int32_t format;        /* bsp_tune_format */
-  int32_t kernel;        /* BSR4 / BSR8 kernel */
+  int32_t kernel;        /* bsp_bsr_kernel (BSR4, BSR8), bsp_dbsr_kernel, bsp_dasp_kernel */
   int32_t chunk;         /* CSR window, 128..2048 */
   int32_t block_longest; /* DASP */
   int64_t seg_cols;      /* CSR: 0 = automatic, < 0 = never segment */

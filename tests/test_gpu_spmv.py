@@ -268,3 +268,32 @@ def test_reference_surface_wrappers(oracle, bsp_host):
                                 A.val.ctypes.data, x.ctypes.data, out.ctypes.data, 3, C.byref(t))
         assert rc == 0 and t.value >= 0
         assert_spmv_close(out, y_ref, abs_spmv(oracle, A, x), what=fn)
+
+
+@pytest.mark.parametrize("rp,col,ncols,ok,sorted_", [
+    ([0, 2, 3, 5], [0, 3, 1, 2, 4], 5, True, True),
+    ([0, 2, 3, 5], [3, 0, 1, 2, 4], 5, True, False),   # unsorted: valid, reported
+    ([0, 2, 2, 5], [0, 3, 1, 2, 2], 5, True, False),   # repeated column: not strictly ascending
+    ([0, 3, 2, 5], [0, 3, 1, 2, 4], 5, False, None),   # rowptr decreases
+    ([0, 2, 3, 4], [0, 3, 1, 2, 4], 5, False, None),   # rowptr ends before nnz
+    ([1, 2, 3, 5], [0, 3, 1, 2, 4], 5, False, None),   # rowptr starts past 0
+    ([0, 2, 3, 5], [0, 5, 1, 2, 4], 5, False, None),   # column == ncols
+    ([0, 2, 3, 5], [0, -1, 1, 2, 4], 5, False, None),  # negative column
+])
+def test_device_csr_check(rp, col, ncols, ok, sorted_):
+    import ctypes as C
+    import torch
+    from benchsp_amd._lib import BspError, CsrDesc, lib
+    from benchsp_amd.device import ptr, stream_handle
+    rpd = torch.tensor(rp, dtype=torch.int32, device="cuda")
+    cd = torch.tensor(col, dtype=torch.int32, device="cuda")
+    vd = torch.ones(len(col), dtype=torch.float64, device="cuda")
+    d = CsrDesc(len(rp) - 1, ncols, len(col), ptr(rpd), ptr(cd), ptr(vd), 0)
+    s = C.c_int32(-1)
+    if ok:
+        lib().bsp_csr_check(C.byref(d), stream_handle(), C.byref(s))
+        assert s.value == int(sorted_)
+    else:
+        with pytest.raises(BspError) as e:
+            lib().bsp_csr_check(C.byref(d), stream_handle(), C.byref(s))
+        assert e.value.status == 1

@@ -62,6 +62,11 @@ def test_autotune_format_mask(oracle, bsp_host, fmt):
     if fmt == "dasp":
         assert {(t["threshold"], t["block_longest"]) for t in T.trials()} == {
             (t, b) for t in (0.5, 0.75, 1.0) for b in (64, 256)}
+    # kernel variants: VALU and MFMA for DBSR and DASP (at 0.75 / 256), the
+    # four 4x4 BSR kernels, VALU and MFMA for 8x8
+    kernels = {t["kernel"] for t in T.trials()}
+    assert kernels == {"dbsr": {0, 1}, "dasp": {0, 1}, "bsr4": {1, 2, 3, 4}, "bsr8": {1, 2},
+                       "csr": {0}}[fmt]
     x = bsp_host.uniform(A.ncols, 3)
     y = T(torch.from_numpy(x).cuda()).cpu().numpy()
     assert_spmv_close(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x), abs_spmv(oracle, A, x),
