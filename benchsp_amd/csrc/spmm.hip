@@ -153,16 +153,42 @@ __global__ __launch_bounds__(kTB) void spmm_combine(const int4* __restrict__ com
   }
 }
 
+// v read from another lane of its row of 16 through a DPP lane pattern.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Sum of p over each aligned group of G lanes (G = 16, 32 or 64), in every
+// lane of the group. Within a row of 16 lanes: quad swaps [1,0,3,2] and
+// [2,3,0,1], then the half-row and row mirrors, as DPP operands of the adds
+// (no LDS); wider groups add their rows with lane shuffles.
+template <int G>
+__device__ __forceinline__ float group_sum(float p) {
+  p += dpp_f32<0xB1>(p);
+  p += dpp_f32<0x4E>(p);
+  p += dpp_f32<0x141>(p);
+  p += dpp_f32<0x140>(p);
+#pragma unroll
+  for (int o = 16; o < G; o <<= 1) p += __shfl_xor(p, o, 64);
+  return p;
+}
+
 // dvals[j] = dot(dY[row_j, :], X[col_j, :]); wave per work item, G = F/4
-// lanes per entry (float4 each), 64/G entries per step; the item's dY row is
-// loaded once and reused for all its entries.
+// lanes per entry (float4 each), NZ = 64/G lane groups; the item's dY row is
+// loaded once and reused for all its entries. A step covers NZ*U entries:
+// group `sub` takes the U consecutive entries from t0 + sub*U, with all U
+// X rows in flight before the dot products (U = 8: BF16 F = 64 215 -> 201 us
+// against 4). After the group sums, lane q < U of each group holds entry
+// t0 + sub*U + q, so one store writes the step's NZ*U results contiguously.
 template <class TX, int F>
 __global__ __launch_bounds__(kTB) void sddmm_items(const int32_t* __restrict__ col,
                                                    const TX* __restrict__ dY,
                                                    const TX* __restrict__ X,
                                                    const int4* __restrict__ items, int64_t nitems,
                                                    float* __restrict__ dvals) {
-  constexpr int G = F / 4, NZ = kWave / G;
+  constexpr int G = F / 4, NZ = kWave / G, U = 8;
+  static_assert(G >= U && G % 16 == 0, "F must be a multiple of 64");
   const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (w >= nitems) return;
@@ -170,28 +196,25 @@ __global__ __launch_bounds__(kTB) void sddmm_items(const int32_t* __restrict__ c
   const int q = lane % G, sub = lane / G;
   float dy[4];
   load_feats4<TX>(dY + (int64_t)it.x * F + q * 4, dy);
-  // 64 column indices per wave load, then U steps of NZ entries with their X
-  // rows all in flight before the dot products (8 steps: BF16 F = 64
-  // 215 -> 201 us against 4; the SpMM kernel keeps 4, its best)
-  constexpr int U = 8;
   for (int base = it.y; base < it.z; base += kWave) {
     const int cl = base + lane < it.z ? ld_stream(&col[base + lane]) : 0;
     const int n = min(kWave, it.z - base);
     for (int t0 = 0; t0 < n; t0 += NZ * U) {
+      const int j0 = t0 + sub * U;
       float xv[U][4];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int c = __shfl(cl, min(t0 + u * NZ + sub, kWave - 1), 64);
+        const int c = __shfl(cl, min(j0 + u, kWave - 1), 64);
         load_feats4<TX>(X + (int64_t)c * F + q * 4, xv[u]);
       }
+      float r = 0.f;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        float p = dy[0] * xv[u][0] + dy[1] * xv[u][1] + dy[2] * xv[u][2] + dy[3] * xv[u][3];
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
-        const int jl = t0 + u * NZ + sub;
-        if (q == 0 && jl < n) dvals[base + jl] = p;
+        const float p = group_sum<G>(dy[0] * xv[u][0] + dy[1] * xv[u][1] + dy[2] * xv[u][2] +
+                                     dy[3] * xv[u][3]);
+        r = q == u ? p : r;
       }
+      if (q < U && j0 + q < n) dvals[base + j0 + q] = r;
     }
   }
 }
