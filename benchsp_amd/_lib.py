@@ -155,6 +155,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_spmm": (None, [vp, vp, C.c_int, vp, C.c_int, i64, vp, vp]),
     "bsp_spmm_t": (None, [vp, vp, C.c_int, vp, C.c_int, i64, vp, vp]),
     "bsp_sddmm": (None, [vp, vp, vp, C.c_int, i64, vp, vp]),
+    "bsp_spmm_t_sddmm": (None, [vp, vp, C.c_int, vp, vp, C.c_int, i64, vp, vp, vp]),
     "bsp_spmm_plan_destroy": (None, [vp]),
     "bsp_dasp_build": (None, [C.POINTER(CsrDesc), f64, i32, vp, C.POINTER(vp)]),
     "bsp_dasp_view_get": (None, [vp, C.POINTER(DaspView)]),

@@ -5,6 +5,7 @@ torch.autograd.Function, computed by libbsp's HIP kernels:
   backward  dX = A^T dY         bsp_spmm_t  (CSC traversal of the pattern, values
                                              read through the CSR permutation)
             dvals = SDDMM(dY, X) bsp_sddmm   (dY[row_j] . X[col_j] for every entry)
+  (both gradients at once: bsp_spmm_t_sddmm, one CSC pass, the same bits)
 
 Values and dense operands may be float32 or bfloat16 (bf16 inputs, FP32
 accumulation); gradients are returned in the dtype of the input they belong to.
@@ -62,6 +63,16 @@ class SparsePattern:
                         stream_handle())
         return dv
 
+    def spmm_t_sddmm(self, vals: torch.Tensor, dY: torch.Tensor, X: torch.Tensor):
+        """(A^T dY, SDDMM(dY, X)) in one pass over the CSC: the same bits as
+        spmm_t and sddmm."""
+        assert dY.dtype == X.dtype
+        dX = torch.empty(self.shape[1], dY.shape[1], dtype=torch.float32, device=dY.device)
+        dv = torch.empty(self.nnz, dtype=torch.float32, device=X.device)
+        lib().bsp_spmm_t_sddmm(self._h, ptr(vals), _DT[vals.dtype], ptr(dY), ptr(X),
+                               _DT[X.dtype], X.shape[1], ptr(dX), ptr(dv), stream_handle())
+        return dX, dv
+
     def __del__(self):
         try:
             if self._h:
@@ -88,9 +99,12 @@ class SpMMFunction(torch.autograd.Function):
         # the dense operands of the backward kernels share X's precision
         dYc = dY.contiguous().to(X.dtype)
         dvals = dX = None
-        if ctx.needs_input_grad[1]:
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            dX, dvals = p.spmm_t_sddmm(vals, dYc, X)
+            dX, dvals = dX.to(X.dtype), dvals.to(vals.dtype)
+        elif ctx.needs_input_grad[1]:
             dX = p.spmm_t(vals, dYc).to(X.dtype)
-        if ctx.needs_input_grad[0]:
+        elif ctx.needs_input_grad[0]:
             dvals = p.sddmm(dYc, X).to(vals.dtype)
         return dvals, dX, None
 

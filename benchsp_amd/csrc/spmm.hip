@@ -32,6 +32,10 @@ struct bsp_spmm_plan_s {
   bsp::DevBuf<int32_t> colptr, rowidx, perm;
   bsp::DevBuf<float> partial;       // max slots * F floats, grown on demand
   int64_t partial_cap = 0;
+  // fused backward: CSC position of every CSR entry and the CSC-ordered
+  // SDDMM results (nnz each, built on its first call)
+  bsp::DevBuf<int32_t> iperm;
+  bsp::DevBuf<float> dvt;
 };
 
 namespace bsp {
@@ -65,49 +69,97 @@ __device__ __forceinline__ void load_feats4<uint16_t>(const uint16_t* p, float (
   f[2] = __uint_as_float(v.y << 16); f[3] = __uint_as_float(v.y & 0xffff0000u);
 }
 
+// v read from another lane of its row of 16 through a DPP lane pattern.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Sum of p over each aligned group of G lanes (G = 16, 32 or 64), in every
+// lane of the group. Within a row of 16 lanes: quad swaps [1,0,3,2] and
+// [2,3,0,1], then the half-row and row mirrors, as DPP operands of the adds
+// (no LDS); wider groups add their rows with lane shuffles.
+template <int G>
+__device__ __forceinline__ float group_sum(float p) {
+  p += dpp_f32<0xB1>(p);
+  p += dpp_f32<0x4E>(p);
+  p += dpp_f32<0x141>(p);
+  p += dpp_f32<0x140>(p);
+#pragma unroll
+  for (int o = 16; o < G; o <<= 1) p += __shfl_xor(p, o, 64);
+  return p;
+}
+
 // Y[row, :] (or a partial) = sum over the item's entries of a * X[col, :];
 // wave per work item. G = F/4 lanes share an entry, each owning four
-// features (one 16-byte FP32 or 8-byte BF16 load), so NZ = 64/G entries sit
-// side by side in a wave and U steps of them are in flight at once; the
+// features (one 16-byte FP32 or 8-byte BF16 load), so NZ = 64/G lane groups
+// sit side by side in a wave; a step covers NZ*U entries, group `sub` taking
+// the U consecutive ones from t0 + sub*U with all U rows in flight. The
 // item's (col, a) pairs are loaded 64 at a time, one per lane, and handed
-// out with lane shuffles. Lane group `sub` sums the entries e = sub (mod NZ)
-// in order; the NZ group sums are added with a fixed shuffle tree.
-template <class TV, class TX, int F, bool GATHER>
+// out with lane shuffles. Each group sums its entries in order; the NZ group
+// sums are added with a fixed shuffle tree.
+//
+// SDD (transposed traversal only): the same pass also writes the SDDMM of
+// the backward in traversal order, dvals[j] = dot(X[row_j, :], D[item, :])
+// with D the item's own row of the second operand, so every gathered row
+// serves both products. After the group sums lane q < U of a group holds
+// the dot product of the group's entry q, and one store writes the step's
+// results contiguously. (Storing each straight to its CSR position instead,
+// a random 4-byte write per entry, cost 140 us FP32 / 176 us BF16 on a
+// 10 M-entry graph; non-temporal, 165 / 234 us.)
+template <class TV, class TX, int F, bool GATHER, bool SDD = false>
 __global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ idx,
                                                   const TV* __restrict__ val,
                                                   const int32_t* __restrict__ perm,
                                                   const TX* __restrict__ X, float* __restrict__ Y,
                                                   const int4* __restrict__ items, int64_t nitems,
-                                                  float* __restrict__ partial) {
+                                                  float* __restrict__ partial,
+                                                  const TX* __restrict__ D = nullptr,
+                                                  float* __restrict__ dvals = nullptr) {
   constexpr int G = F / 4, NZ = kWave / G, U = 4;
+  static_assert(!SDD || GATHER, "the fused SDDMM runs on the transposed traversal");
   const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (w >= nitems) return;
   const int4 it = items[w];
   const int q = lane % G, sub = lane / G;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  float dr[4] = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (SDD) load_feats4<TX>(D + (int64_t)it.x * F + q * 4, dr);
   for (int base = it.y; base < it.z; base += kWave) {
     const int j = base + lane;
-    int c = 0;
+    int c = 0, pj = 0;
     float a = 0.f;
     if (j < it.z) {
       c = ld_stream(&idx[j]);
-      a = to_f32<TV>(GATHER ? val[perm[j]] : ld_stream(&val[j]));
+      if constexpr (GATHER) pj = ld_stream(&perm[j]);
+      a = to_f32<TV>(GATHER ? val[pj] : ld_stream(&val[j]));
     }
     const int n = min(kWave, it.z - base);
     for (int t0 = 0; t0 < n; t0 += NZ * U) {
+      const int j0 = t0 + sub * U;
       float xv[U][4], av[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = min(t0 + u * NZ + sub, kWave - 1);
+        const int e = min(j0 + u, kWave - 1);
         av[u] = __shfl(a, e, 64);
         load_feats4<TX>(X + (int64_t)__shfl(c, e, 64) * F + q * 4, xv[u]);
       }
+      float d = 0.f;
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (t0 + u * NZ + sub < n)
+      for (int u = 0; u < U; ++u) {
+        if (j0 + u < n)
 #pragma unroll
           for (int v = 0; v < 4; ++v) acc[v] += av[u] * xv[u][v];
+        if constexpr (SDD) {
+          const float p = group_sum<G>(dr[0] * xv[u][0] + dr[1] * xv[u][1] + dr[2] * xv[u][2] +
+                                       dr[3] * xv[u][3]);
+          d = q == u ? p : d;
+        }
+      }
+      if constexpr (SDD) {
+        if (q < U && j0 + q < n) dvals[base + j0 + q] = d;
+      }
     }
   }
 #pragma unroll
@@ -118,6 +170,22 @@ __global__ __launch_bounds__(kTB) void spmm_items(const int32_t* __restrict__ id
     float* dst = it.w < 0 ? Y + (int64_t)it.x * F : partial + (int64_t)it.w * F;
     *reinterpret_cast<float4*>(dst + q * 4) = make_float4(acc[0], acc[1], acc[2], acc[3]);
   }
+}
+
+// dst[j] = src[iperm[j]]: results in CSC order back to CSR order, a thread
+// per entry (streamed index, gathered value, coalesced store).
+__global__ __launch_bounds__(kTB) void to_csr_order(const int32_t* __restrict__ iperm,
+                                                    const float* __restrict__ src, int64_t nnz,
+                                                    float* __restrict__ dst) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nnz) dst[j] = src[ld_stream(&iperm[j])];
+}
+
+// iperm[perm[k]] = k: the inverse of the CSC permutation.
+__global__ __launch_bounds__(kTB) void invert_perm(const int32_t* __restrict__ perm, int64_t nnz,
+                                                   int32_t* __restrict__ iperm) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) iperm[perm[k]] = (int32_t)k;
 }
 
 // Y[row, :] = sum of its chunk partials; a workgroup per row. Thread t owns
@@ -151,27 +219,6 @@ __global__ __launch_bounds__(kTB) void spmm_combine(const int4* __restrict__ com
     for (int q = 1; q < G; ++q) t += red[q * F + f];
     Y[(int64_t)c.x * F + f] = t;
   }
-}
-
-// v read from another lane of its row of 16 through a DPP lane pattern.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f32(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-
-// Sum of p over each aligned group of G lanes (G = 16, 32 or 64), in every
-// lane of the group. Within a row of 16 lanes: quad swaps [1,0,3,2] and
-// [2,3,0,1], then the half-row and row mirrors, as DPP operands of the adds
-// (no LDS); wider groups add their rows with lane shuffles.
-template <int G>
-__device__ __forceinline__ float group_sum(float p) {
-  p += dpp_f32<0xB1>(p);
-  p += dpp_f32<0x4E>(p);
-  p += dpp_f32<0x141>(p);
-  p += dpp_f32<0x140>(p);
-#pragma unroll
-  for (int o = 16; o < G; o <<= 1) p += __shfl_xor(p, o, 64);
-  return p;
 }
 
 // dvals[j] = dot(dY[row_j, :], X[col_j, :]); wave per work item, G = F/4
@@ -350,12 +397,16 @@ static bsp_status build_spmm_plan(bsp_spmm_plan_s* p, hipStream_t s) {
   return BSP_OK;
 }
 
+// One traversal of `sd`: Y (or chunk partials, then their combine) from
+// the gathered rows of X; with dvals set (transposed side only) also the
+// fused SDDMM against the item rows of D.
 template <class TV, class TX, bool GATHER>
 static bsp_status run_spmm_t(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd, const void* V,
-                             const void* X,
-                             int64_t F, float* Y, hipStream_t s) {
+                             const void* X, int64_t F, float* Y, const void* D, float* dvals,
+                             hipStream_t s) {
   const auto* val = static_cast<const TV*>(V);
   const auto* x = static_cast<const TX*>(X);
+  const auto* d = static_cast<const TX*>(D);
   const int64_t grid = cdiv(sd.nitems * kWave, kTB);
   if (sd.nslots * F > p->partial_cap) {
     BSP_HIP(hipStreamSynchronize(s));
@@ -363,19 +414,24 @@ static bsp_status run_spmm_t(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd
     p->partial_cap = sd.nslots * F;
   }
   if (sd.nitems == 0) return BSP_OK;
+  auto launch = [&](auto fw) -> bsp_status {
+    constexpr int FW = decltype(fw)::value;
+    if constexpr (GATHER) {
+      if (dvals) {
+        spmm_items<TV, TX, FW, true, true><<<grid, kTB, 0, s>>>(
+            sd.idx, val, sd.perm, x, Y, sd.items.get(), sd.nitems, p->partial.get(), d, dvals);
+        return BSP_OK;
+      }
+    }
+    spmm_items<TV, TX, FW, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y,
+                                                        sd.items.get(), sd.nitems,
+                                                        p->partial.get());
+    return BSP_OK;
+  };
   switch (F) {
-    case 64:
-      spmm_items<TV, TX, 64, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
-                                                         sd.nitems, p->partial.get());
-      break;
-    case 128:
-      spmm_items<TV, TX, 128, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
-                                                         sd.nitems, p->partial.get());
-      break;
-    case 256:
-      spmm_items<TV, TX, 256, GATHER><<<grid, kTB, 0, s>>>(sd.idx, val, sd.perm, x, Y, sd.items.get(),
-                                                         sd.nitems, p->partial.get());
-      break;
+    case 64: BSP_TRY(launch(std::integral_constant<int, 64>{})); break;
+    case 128: BSP_TRY(launch(std::integral_constant<int, 128>{})); break;
+    case 256: BSP_TRY(launch(std::integral_constant<int, 256>{})); break;
     default:
       return fail(BSP_ERR_UNSUPPORTED, "SpMM feature width must be 64, 128 or 256");
   }
@@ -390,13 +446,13 @@ static bsp_status run_spmm_t(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd
 
 template <bool GATHER>
 static bsp_status run_spmm(bsp_spmm_plan_s* p, const bsp_spmm_plan_s::Side& sd, const void* V,
-                           bsp_dtype vt, const void* X,
-                           bsp_dtype xt, int64_t F, float* Y, hipStream_t s) {
+                           bsp_dtype vt, const void* X, bsp_dtype xt, int64_t F, float* Y,
+                           hipStream_t s, const void* D = nullptr, float* dvals = nullptr) {
   const bool vb = vt == BSP_BF16, xb = xt == BSP_BF16;
-  if (vb && xb) return run_spmm_t<uint16_t, uint16_t, GATHER>(p, sd, V, X, F, Y, s);
-  if (vb) return run_spmm_t<uint16_t, float, GATHER>(p, sd, V, X, F, Y, s);
-  if (xb) return run_spmm_t<float, uint16_t, GATHER>(p, sd, V, X, F, Y, s);
-  return run_spmm_t<float, float, GATHER>(p, sd, V, X, F, Y, s);
+  if (vb && xb) return run_spmm_t<uint16_t, uint16_t, GATHER>(p, sd, V, X, F, Y, D, dvals, s);
+  if (vb) return run_spmm_t<uint16_t, float, GATHER>(p, sd, V, X, F, Y, D, dvals, s);
+  if (xb) return run_spmm_t<float, uint16_t, GATHER>(p, sd, V, X, F, Y, D, dvals, s);
+  return run_spmm_t<float, float, GATHER>(p, sd, V, X, F, Y, D, dvals, s);
 }
 
 }  // namespace bsp
@@ -468,6 +524,32 @@ bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype d
     case 256: bf ? launch(uint16_t{}, F256c{}) : launch(float{}, F256c{}); break;
     default: return fail(BSP_ERR_UNSUPPORTED, "SDDMM feature width must be 64, 128 or 256");
   }
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status bsp_spmm_t_sddmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* dY,
+                           const void* X, bsp_dtype dtype, int64_t F, float* dX, float* dvals,
+                           bsp_stream s) {
+  BSP_REQUIRE(p && (dY || p->A.nrows == 0) && (X || p->A.ncols == 0), "null argument");
+  BSP_REQUIRE((dX || p->A.ncols == 0) && (dvals || p->A.nnz == 0), "null output");
+  BSP_REQUIRE(aligned16(dY) && aligned16(X) && aligned16(dX),
+              "dY, X and dX must be 16-byte aligned");
+  BSP_REQUIRE(val || p->A.nnz == 0, "null values");
+  BSP_REQUIRE(dtype == BSP_F32 || dtype == BSP_BF16, "dY and X must be FP32 or BF16");
+  BSP_REQUIRE(vdtype == BSP_F32 || vdtype == BSP_BF16, "values must be FP32 or BF16");
+  hipStream_t st = (hipStream_t)s;
+  const int64_t nnz = p->A.nnz;
+  if (nnz == 0) return run_spmm<true>(p, p->bwd, val, vdtype, dY, dtype, F, dX, st);
+  if (!p->iperm.get()) {
+    BSP_HIP(hipStreamSynchronize(st));
+    BSP_TRY(p->iperm.alloc(nnz));
+    BSP_TRY(p->dvt.alloc(nnz));
+    invert_perm<<<cdiv(nnz, kTB), kTB, 0, st>>>(p->perm.get(), nnz, p->iperm.get());
+    BSP_HIP(hipGetLastError());
+  }
+  BSP_TRY(run_spmm<true>(p, p->bwd, val, vdtype, dY, dtype, F, dX, st, X, p->dvt.get()));
+  to_csr_order<<<cdiv(nnz, kTB), kTB, 0, st>>>(p->iperm.get(), p->dvt.get(), nnz, dvals);
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

@@ -382,6 +382,12 @@ bsp_status bsp_spmm_t(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const 
                       bsp_dtype dtype, int64_t F, float* dX, bsp_stream s);
 bsp_status bsp_sddmm(bsp_spmm_plan p, const void* dY, const void* X, bsp_dtype dtype, int64_t F,
                      float* dvals, bsp_stream s);
+/* The whole backward in one traversal of the CSC: dX as bsp_spmm_t and
+ * dvals as bsp_sddmm, each bit-identical to those calls, every gathered row
+ * of dY serving both products (dY and X share `dtype`). */
+bsp_status bsp_spmm_t_sddmm(bsp_spmm_plan p, const void* val, bsp_dtype vdtype, const void* dY,
+                           const void* X, bsp_dtype dtype, int64_t F, float* dX, float* dvals,
+                           bsp_stream s);
 bsp_status bsp_spmm_plan_destroy(bsp_spmm_plan p);
 
 /* ------------------------------------------------------------------------ */

@@ -90,6 +90,7 @@ def test_spmm_and_block_spmm_replay_in_graphs(bsp_host):
             outs["Y"] = P.spmm(vals, X)
             outs["dX"] = P.spmm_t(vals, dY)
             outs["dv"] = P.sddmm(dY, X)
+            outs["fdX"], outs["fdv"] = P.spmm_t_sddmm(vals, dY, X)
         step()
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()

@@ -41,6 +41,10 @@ def test_spmm_fwd_bwd_small(oracle, bsp_host, dtype_name, F, symmetric):
     Y = P.spmm(tv, tX).cpu().numpy()
     dX = P.spmm_t(tv, tdY).cpu().numpy()
     dv = P.sddmm(tdY, tX).cpu().numpy()
+    # the fused backward gives the same bits as the two separate products
+    fdX, fdv = P.spmm_t_sddmm(tv, tdY, tX)
+    assert np.array_equal(fdX.cpu().numpy(), dX)
+    assert np.array_equal(fdv.cpu().numpy(), dv)
     # dense FP64 references
     D = A.to_dense()
     if dtype == torch.bfloat16:
@@ -164,6 +168,9 @@ def test_spmm_gnn_200k(oracle, bsp_host, dtype_name):
         assert rel_fro(dv[js], ref) <= tol
         # deterministic: a second run is bit-identical
         assert np.array_equal(P.spmm(tv, tX).cpu().numpy(), Y)
+        fdX, fdv = P.spmm_t_sddmm(tv, tdY, tX)
+        assert np.array_equal(fdX.cpu().numpy(), dX)
+        assert np.array_equal(fdv.cpu().numpy(), dv)
 
 
 def test_spmm_rejects_unaligned_operands(bsp_host):
