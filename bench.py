@@ -3,7 +3,9 @@
 value) and FP16 block-SpMM forward+backward GFLOP/s against a plain FP32 CSR
 SpMM (secondary leg).
 
-SpMV step: y = A*x with the library's adaptive CSR kernel on
+SpMV step: y = A*x, each matrix in the layout the library's autotuner picks
+for it (bsp_autotune, timed once before the run; `--method adaptive` keeps
+the adaptive CSR plan on both), on
   (a) the 27-point stencil on a 64^3 grid (n=262,144, nnz=6,859,000) and
   (b) the power-law matrix (n=2^20, lognormal row lengths mean 20, sigma 1.2,
       cap 5000, ~21M nnz),
@@ -11,7 +13,8 @@ synthetic data from the library's deterministic generators, resident in HBM.
 Each matrix is replicated so the copies together exceed 512 MB and every
 launch reads a copy that is not in the 256 MB Infinity Cache (cold HBM).
 value = sum of CSR-equivalent algorithmic bytes (12*nnz + 4*(n+1) + 8*ncols
-+ 8*n per SpMV) over the timed steps / timed wall time, in GB/s. `roofline`
++ 8*n per SpMV, whatever the layout streams) over the timed steps / timed
+wall time, in GB/s. `roofline`
 holds the dominant launch against the nominal 8 TB/s, its measured HBM
 traffic, and (`measured_peaks`) what a plain 1 GiB copy and read reach on
 this box.
@@ -26,8 +29,9 @@ forward's error against a dense FP64 product is reported.
 Formats leg: the same two matrices through every SpMV format and method of
 the library (scalar / balanced / split / adaptive CSR, adaptive CSR with
 16-bit column offsets, 4x4 BSR on MFMA and VALU per block row and balanced
-over 64-block chunks, DBSR and DASP on VALU and on FP64 MFMA), the layout
-the per-matrix autotuner picks, and rocSPARSE, as CSR-equivalent effective
+over 64-block chunks, DBSR and DASP on VALU and on FP64 MFMA, the diagonal
+layout where the matrix has at most 32 diagonals), the layout the
+per-matrix autotuner picks, and rocSPARSE, as CSR-equivalent effective
 GB/s, with the wall time of each GPU format build (plan, analysis or format
 conversion from a device CSR). The CPU baseline leg times the restated host
 builders of the BSR4, DBSR and DASP formats on one core beside it.
@@ -67,7 +71,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--method", default="adaptive")
+    ap.add_argument("--method", default="tuned",
+                    help="tuned (the autotuner's layout per matrix) or a CSR method: adaptive, "
+                         "scalar, balanced, split")
+    ap.add_argument("--records", metavar="JSON",
+                    help="with --method tuned: read each matrix's layout record from this file "
+                         "instead of running the autotuner, or write the tuned records to it "
+                         "when it does not exist (profiler runs free of tuning launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -138,17 +148,35 @@ def spmv_leg(args, dev):
     import numpy as np
     import torch
     from benchsp_amd import host
-    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    from benchsp_amd.device import CsrSpmv, DeviceCsr, Tuned
 
     mats = [("27pt-64^3", host.stencil27_64()), ("powerlaw-1M", host.powerlaw_1m())]
     xs = [host.uniform(A.ncols, seed=100 + i) for i, (_, A) in enumerate(mats)]
+    records = {}
+    if args.records and os.path.exists(args.records):
+        with open(args.records) as f:
+            records = json.load(f)
     work = []
     for (name, A), x in zip(mats, xs):
         ncopy = _copies(A.csr_bytes(), args.warm)
-        ops = [CsrSpmv(DeviceCsr.from_host(A, dev), args.method) for _ in range(ncopy)]
+        dAs = [DeviceCsr.from_host(A, dev) for _ in range(ncopy)]
+        if args.method == "tuned":
+            # the autotuner picks the layout once (cache-warm timing of every
+            # candidate, bsp_autotune), or a stored record names it
+            # (--records); every copy is rebuilt from the record
+            rec = records.get(name) or Tuned(dAs[0]).record
+            records[name] = rec
+            ops = [Tuned.from_record(d, rec) for d in dAs]
+            layout = {k: rec[k] for k in ("format", "kernel", "chunk", "seg_cols")}
+            seg = (CsrSpmv(dAs[0], "adaptive", seg_cols=rec["seg_cols"], chunk=rec["chunk"])
+                   .segments() if rec["format"] == "csr" else (0, 1))
+        else:
+            ops = [CsrSpmv(d, args.method) for d in dAs]
+            layout = {"format": "csr", "method": args.method}
+            seg = ops[0].segments()
         work.append(dict(name=name, A=A, ops=ops, x=torch.from_numpy(x).to(dev),
                          y=torch.empty(A.nrows, dtype=torch.float64, device=dev),
-                         bytes=A.csr_bytes(), ncopy=ncopy, seg=ops[0].segments()))
+                         bytes=A.csr_bytes(), ncopy=ncopy, seg=seg, layout=layout))
     torch.cuda.synchronize()
 
     def step(i, evs=None):
@@ -182,7 +210,11 @@ def spmv_leg(args, dev):
         per[w["name"]] = {"nrows": w["A"].nrows, "nnz": w["A"].nnz, "bytes": w["bytes"],
                           "us": round(ms * 1e3, 2), "eff_GBs": round(gbs, 1),
                           "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
-                          "copies": w["ncopy"], "column_segments": w["seg"][1]}
+                          "copies": w["ncopy"], "column_segments": w["seg"][1],
+                          "layout": w["layout"]}
+    if args.records and args.method == "tuned" and not os.path.exists(args.records):
+        with open(args.records, "w") as f:
+            json.dump(records, f, indent=1)
     step_bytes = sum(w["bytes"] for w in work)
     return dict(value=step_bytes * args.steps / wall / 1e9, wall=wall, per=per,
                 achieved=tot_b / (tot_t * 1e-3) / 1e9,
@@ -375,7 +407,9 @@ def formats_leg(args, dev, mats, xs):
     median event-timed launch."""
     import numpy as np
     import torch
-    from benchsp_amd.device import (Bsr, CsrSpmv, Dasp, Dbsr, DeviceCsr, RocsparseSpmv, Tuned)
+    from benchsp_amd._lib import BspError
+    from benchsp_amd.device import (Bsr, CsrSpmv, Dasp, Dbsr, DeviceCsr, Dia, RocsparseSpmv,
+                                    Tuned)
 
     out = {}
     for (name, A), x in zip(mats, xs):
@@ -407,6 +441,12 @@ def formats_leg(args, dev, mats, xs):
             cands["bsr4_valu"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=1)))(Bsr(d))
             cands["bsr4_mfma_bal"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=3)))(Bsr(d))
             cands["bsr4_valu_bal"] = lambda d: (lambda b: (lambda xx, yy: b(xx, yy, kernel=4)))(Bsr(d))
+        dia_bytes = None
+        try:  # the diagonal layout where the matrix has at most 32 diagonals
+            dia_bytes = Dia(dAs[0]).info()["bytes"] + 8 * (A.ncols + A.nrows)
+            cands["dia"] = lambda d: Dia(d)
+        except BspError:
+            pass
         res = {}
         for key, make in cands.items():
             ops, build = [], []
@@ -432,6 +472,9 @@ def formats_leg(args, dev, mats, xs):
                         "build_ms": round(float(np.median(build)) * 1e3, 2)}
             del ops
         res["autotuned"].update(tune_ms=round(tune_ms, 1), record=rec)
+        if dia_bytes is not None:  # what the diagonal layout streams (values, masks, x, y)
+            res["dia"].update(bytes=dia_bytes,
+                              actual_GBs=round(dia_bytes / (res["dia"]["us"] * 1e-6) / 1e9, 1))
         out[name] = res
         del dAs
     return out
@@ -612,7 +655,7 @@ def measured_peaks() -> dict:
             "read_GBs": round(nbytes / read_s / 1e9, 1)}
 
 
-SPMV_KERNELS = ("csr_adaptive", "sum_partials")
+SPMV_KERNELS = ("csr_adaptive", "sum_partials", "dia_spmv")
 
 
 def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=None):
@@ -643,7 +686,8 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=No
         if not any(k in name for k in SPMV_KERNELS):
             continue
         w = wr.get((name, grid), 0.0)
-        launches[f"{name.split('(')[0]} grid={grid}"] = {
+        short = name.replace("(anonymous namespace)::", "").split("(")[0]
+        launches[f"{short} grid={grid}"] = {
             "FETCH_SIZE_KiB": round(f, 1), "WRITE_SIZE_KiB": round(w, 1),
             "hbm_bytes": int(round((rs * f + ws * w) * 1024))}
     d = {"read_scale": round(rs, 4), "write_scale": round(ws, 4), "launches": launches}
@@ -653,40 +697,46 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=No
     print(json.dumps(d, indent=1))
 
 
-def measured_traffic(segmented: bool):
-    """HBM bytes of one SpMV launch of the adaptive plan, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement scaled by a plain copy
-    (profiles/csr_adaptive_traffic.json): the XCD-pinned segment kernel plus
-    its partial-sum kernel for a column-segmented plan, the single adaptive
-    kernel otherwise."""
+def measured_traffic(keys):
+    """HBM bytes of one SpMV launch, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement scaled by a plain copy
+    (profiles/csr_adaptive_traffic.json): the sum over the kernels named by
+    `keys` (the XCD-pinned segment kernel plus its partial sums for a
+    column-segmented plan; the one kernel otherwise)."""
     p = os.path.join(ROOT, "profiles", "csr_adaptive_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         d = json.load(f)
-    keys = ("csr_adaptive_xcd", "sum_partials") if segmented else ("csr_adaptive_f64<",)
     total = 0
     for q in keys:
         hits = [v["hbm_bytes"] for k, v in d["launches"].items() if q in k]
         if len(hits) != 1:  # the measurement must hold the one launch of the SpMV leg
-            raise RuntimeError(f"{p}: {len(hits)} launches match {q!r}; re-run the PMC passes "
-                               "with --no-formats --no-gnn")
+            return None  # not measured for this kernel (or ambiguous): no figure
         total += hits[0]
     return int(total)
 
 
 def roofline(per: dict, method: str) -> dict:
     """The dominant launch of the step (the matrix whose SpMV takes longest):
-    its algorithmic bytes over its event-timed duration, and its measured HBM
-    bytes; the other matrix's launch is reported alongside."""
+    its algorithmic (CSR-equivalent) bytes over its event-timed duration, and
+    its measured HBM bytes; the other matrix's launch is reported alongside."""
     def one(name, m):
         segmented = m["column_segments"] > 1
+        fmt = m["layout"]["format"]
+        adaptive = method == "adaptive" or (fmt == "csr" and m["layout"].get("chunk") == 2048)
         gbs = m["bytes"] / (m["us"] * 1e-6) / 1e9
+        if fmt == "dia":
+            kernel, keys = "dia_spmv_f64", ("dia_spmv_f64<",)
+        elif adaptive:
+            kernel = ("csr_adaptive_xcd_f64<2048> + sum_partials" if segmented else
+                      "csr_adaptive_f64<2048>")
+            keys = ("csr_adaptive_xcd", "sum_partials") if segmented else ("csr_adaptive_f64<",)
+        else:
+            kernel, keys = f"{fmt} ({method})", None
         r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(gbs / HBM_PEAK_GBS, 4),
-             "traffic": measured_traffic(segmented) if method == "adaptive" else None,
-             "kernel": ("csr_adaptive_xcd_f64<2048> + sum_partials" if segmented else
-                        "csr_adaptive_f64") if method == "adaptive" else method,
+             "traffic": measured_traffic(keys) if keys else None, "kernel": kernel,
              "launch": f"{name} SpMV: {m['bytes']} algorithmic bytes in {m['us']} us"}
         return r
     ordered = sorted(per.items(), key=lambda kv: -kv[1]["us"])
@@ -733,8 +783,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "FP64 CSR SpMV: 27-pt stencil 64^3 + power-law n=2^20 (lognormal "
-                               "mean 20, sigma 1.2, cap 5000)",
+        "config": {"workload": "FP64 SpMV y=A*x: 27-pt stencil 64^3 + power-law n=2^20 "
+                               "(lognormal mean 20, sigma 1.2, cap 5000); CSR-equivalent bytes",
                    "method": args.method, "cache": "warm" if args.warm else "cold (rotated copies)",
                    "pct_hbm_peak": round(100 * sp["value"] / HBM_PEAK_GBS, 1),
                    "matrices": sp["per"]},

@@ -162,6 +162,10 @@ _SIGS: dict[str, tuple] = {
     "bsp_dasp_spmv": (None, [vp, vp, vp, C.c_int, vp]),
     "bsp_dasp_spmv_ex": (None, [vp, C.c_int, vp, vp, C.c_int, vp]),
     "bsp_dasp_destroy": (None, [vp]),
+    "bsp_dia_build": (None, [C.POINTER(CsrDesc), vp, C.POINTER(vp)]),
+    "bsp_dia_info": (None, [vp, vp, vp, vp]),
+    "bsp_dia_spmv": (None, [vp, vp, vp, vp]),
+    "bsp_dia_destroy": (None, [vp]),
     "bsp_autotune": (None, [C.POINTER(CsrDesc), C.POINTER(TuneOptions), vp, C.POINTER(vp)]),
     "bsp_tuned_from_record": (None, [C.POINTER(CsrDesc), C.POINTER(TuneRecord), vp,
                                      C.POINTER(vp)]),

@@ -18,11 +18,13 @@ struct bsp_tuned_s {
   bsp_bsr bsr = nullptr;
   bsp_dbsr dbsr = nullptr;
   bsp_dasp dasp = nullptr;
+  bsp_dia dia = nullptr;
   ~bsp_tuned_s() {
     if (csr) bsp_plan_destroy(csr);
     if (bsr) bsp_bsr_destroy(bsr);
     if (dbsr) bsp_dbsr_destroy(dbsr);
     if (dasp) bsp_dasp_destroy(dasp);
+    if (dia) bsp_dia_destroy(dia);
   }
 };
 
@@ -54,6 +56,8 @@ bsp_status check_record(const bsp_tune_record& r) {
       BSP_REQUIRE(r.threshold > 0.0 && r.threshold <= 1.0, "DASP record: threshold in (0, 1]");
       BSP_REQUIRE(r.block_longest >= 5, "DASP record: block_longest >= 5");
       return BSP_OK;
+    case BSP_TUNE_DIA:
+      return BSP_OK;
     default:
       return fail(BSP_ERR_INVALID, "unknown tune format");
   }
@@ -75,6 +79,8 @@ bsp_status build(const bsp_csr& A, const bsp_tune_record& r, hipStream_t s, bsp_
       return bsp_bsr_build(&A, 8, 8, s, &h->bsr);
     case BSP_TUNE_DBSR:
       return bsp_dbsr_build(&A, s, &h->dbsr);
+    case BSP_TUNE_DIA:
+      return bsp_dia_build(&A, s, &h->dia);
     default:
       return bsp_dasp_build(&A, r.threshold, r.block_longest, s, &h->dasp);
   }
@@ -86,6 +92,7 @@ bsp_status run(const bsp_tuned_s* h, const double* x, double* y, hipStream_t s) 
     case BSP_TUNE_BSR4:
     case BSP_TUNE_BSR8: return bsp_bsr_spmv(h->bsr, h->rec.kernel, x, y, s);
     case BSP_TUNE_DBSR: return bsp_dbsr_spmv_ex(h->dbsr, h->rec.kernel, x, y, 0, s);
+    case BSP_TUNE_DIA: return bsp_dia_spmv(h->dia, x, y, s);
     default: return bsp_dasp_spmv_ex(h->dasp, h->rec.kernel, x, y, 0, s);
   }
 }
@@ -121,9 +128,10 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
   BSP_TRY(check_csr_pattern(A));
   BSP_REQUIRE(out, "null output handle");
   BSP_REQUIRE(A->dtype == BSP_F64 && (A->nnz == 0 || A->val), "FP64 values required");
-  const int formats = opt && opt->formats ? opt->formats : 0x1f;
+  constexpr int kAllFormats = (1 << (BSP_TUNE_DIA + 1)) - 1;
+  const int formats = opt && opt->formats ? opt->formats : kAllFormats;
   const int reps = opt && opt->reps > 0 ? opt->reps : 10;
-  BSP_REQUIRE((formats & ~0x1f) == 0, "unknown format bits");
+  BSP_REQUIRE((formats & ~kAllFormats) == 0, "unknown format bits");
   BSP_REQUIRE(!opt || opt->reps >= 0, "reps must be >= 0");
   hipStream_t s = (hipStream_t)stream;
 
@@ -178,6 +186,12 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
         }
       }
 
+  if (formats & (1 << BSP_TUNE_DIA)) {
+    bsp_tune_record r{};
+    r.format = BSP_TUNE_DIA;
+    cands.push_back(r);
+  }
+
   DevBuf<double> x, y;
   BSP_TRY(x.alloc(A->ncols));
   BSP_TRY(y.alloc(A->nrows));
@@ -193,7 +207,7 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
     h->A = *A;
     h->rec = c;
     last = build(*A, c, s, h.get());
-    if (last != BSP_OK) continue;  // e.g. unsorted columns for segments, memory
+    if (last != BSP_OK) continue;  // e.g. unsorted columns for segments, too many diagonals
     double us = 0.0;
     BSP_TRY(time_spmv(h.get(), x.get(), y.get(), reps, s, ev, &us));
     h->rec.time_us = us;

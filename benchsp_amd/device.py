@@ -254,7 +254,41 @@ class Dasp:
             pass
 
 
-TUNE_FORMATS = {"csr": 0, "bsr4": 1, "dbsr": 2, "dasp": 3, "bsr8": 4}
+class Dia:
+    """Diagonal layout (bsp_dia_build): every distinct diagonal one contiguous
+    run of values, a 32-bit presence mask per row; at most 32 diagonals."""
+
+    def __init__(self, A: DeviceCsr):
+        self.A = A
+        d = A.desc()
+        h = C.c_void_p()
+        lib().bsp_dia_build(C.byref(d), stream_handle(), C.byref(h))
+        self._h = h
+
+    def info(self) -> dict:
+        n = C.c_int32()
+        offs = (C.c_int32 * 32)()
+        b = C.c_int64()
+        lib().bsp_dia_info(self._h, C.byref(n), offs, C.byref(b))
+        return {"ndiag": n.value, "offsets": list(offs[:n.value]), "bytes": b.value}
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        assert x.dtype == torch.float64 and x.is_contiguous() and x.numel() == self.A.ncols
+        if y is None:
+            y = torch.empty(self.A.nrows, dtype=torch.float64, device=x.device)
+        lib().bsp_dia_spmv(self._h, ptr(x), ptr(y), stream_handle())
+        return y
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().bsp_dia_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+TUNE_FORMATS = {"csr": 0, "bsr4": 1, "dbsr": 2, "dasp": 3, "bsr8": 4, "dia": 5}
 _TUNE_NAMES = {v: k for k, v in TUNE_FORMATS.items()}
 
 

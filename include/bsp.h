@@ -305,6 +305,25 @@ bsp_status bsp_dasp_spmv_ex(bsp_dasp h, int kernel, const double* x, double* y, 
 bsp_status bsp_dasp_destroy(bsp_dasp h);
 
 /* ------------------------------------------------------------------------ */
+/* Diagonal layout for banded and stencil matrices. No counterpart in the
+ * reference, whose answer to index bytes is the bitmap-block BSR of
+ * src/amgT.cu:160-209, 781-834: here every distinct diagonal (column - row)
+ * is one contiguous run of values over the rows, with a 32-bit mask per row
+ * of the diagonals the row holds, so the SpMV reads no column indices
+ * (8 bytes per stored slot + 4 per row). y is bit-identical to the
+ * row-sequential CSR sum of src/ref.cpp:26-37 (ascending columns, products
+ * rounded before each add). BSP_ERR_UNSUPPORTED when the matrix has more
+ * than 32 distinct diagonals or a row's columns are not strictly ascending;
+ * BSP_ERR_INVALID for a malformed pattern (checked on the GPU). */
+typedef struct bsp_dia_s* bsp_dia;
+bsp_status bsp_dia_build(const bsp_csr* A, bsp_stream s, bsp_dia* out);
+/* Number of diagonals, their offsets ascending (offsets may be NULL, else
+ * room for 32) and the bytes one SpMV streams for the matrix. */
+bsp_status bsp_dia_info(bsp_dia h, int32_t* ndiag, int32_t* offsets, int64_t* bytes);
+bsp_status bsp_dia_spmv(bsp_dia h, const double* x, double* y, bsp_stream s);
+bsp_status bsp_dia_destroy(bsp_dia h);
+
+/* ------------------------------------------------------------------------ */
 /* Per-matrix SpMV autotuner. No counterpart in the reference, where format
  * and thresholds are fixed per call site (bench/spmvfp64.cpp:135-177; DASP
  * threshold 0.75 and block_longest 256 at :177; AmgT's stand / avgnz cuts,
@@ -313,7 +332,8 @@ bsp_status bsp_dasp_destroy(bsp_dasp h);
  * adaptive CSR with windows 512 / 1024 / 2048 (and, when the automatic plan
  * segments columns, the unsegmented plan too); 4x4 and 8x8 BSR on VALU and
  * MFMA when their blocks are at least half full on average (bsp_block_info); DBSR; DASP
- * with threshold 0.5 / 0.75 / 1.0 x block_longest 64 / 256. Timing is
+ * with threshold 0.5 / 0.75 / 1.0 x block_longest 64 / 256; the diagonal
+ * layout when the matrix has at most 32 diagonals. Timing is
  * cache-warm (the same matrix and x every launch), so it ranks the layouts'
  * kernels, not a cold HBM read. The choice is a plain record: store it and
  * rebuild the same layout later with bsp_tuned_from_record, no timing. */
@@ -322,7 +342,8 @@ typedef enum bsp_tune_format {
   BSP_TUNE_BSR4 = 1, /* 4x4 BSR: kernel = any bsp_bsr_kernel but AUTO */
   BSP_TUNE_DBSR = 2, /* kernel = BSP_DBSR_VALU or BSP_DBSR_MFMA */
   BSP_TUNE_DASP = 3, /* threshold, block_longest as in bsp_dasp_build; kernel = bsp_dasp_kernel */
-  BSP_TUNE_BSR8 = 4  /* 8x8 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
+  BSP_TUNE_BSR8 = 4, /* 8x8 BSR: kernel = BSP_BSR_VALU or BSP_BSR_MFMA */
+  BSP_TUNE_DIA = 5   /* diagonal layout (bsp_dia_build) */
 } bsp_tune_format;
 
 typedef struct bsp_tune_record {

@@ -20,7 +20,7 @@ def _noalloc(dA, x, y):
 
 
 def _ops(bsp_host):
-    from benchsp_amd.device import Bsr, CsrSpmv, Dasp, Dbsr, DeviceCsr
+    from benchsp_amd.device import Bsr, CsrSpmv, Dasp, Dbsr, DeviceCsr, Dia
     mats = {"27pt-64": bsp_host.stencil27_64(),
             "powerlaw-300k": bsp_host.powerlaw(300_000, 20.0, 1.2, 5000, seed=6)}
     for name, A in mats.items():
@@ -38,6 +38,8 @@ def _ops(bsp_host):
                          ("valu-balanced", Bsr.VALU_BALANCED)):
             yield f"{name}/bsr4-{kname}", A, dA, (lambda b, k: (lambda x, y: b(x, y, kernel=k)))(B, k)
         yield f"{name}/noalloc", A, dA, (lambda d: (lambda x, y: _noalloc(d, x, y)))(dA)
+        if name == "27pt-64":
+            yield f"{name}/dia", A, dA, Dia(dA)
     # forced column segments, both modes (XCD-pinned one launch; one launch per segment)
     A = mats["powerlaw-300k"]
     dA = DeviceCsr.from_host(A)

@@ -38,6 +38,8 @@ def test_autotune_parity_and_replay(oracle, bsp_host):
         assert {"csr", "dbsr", "dasp"} <= formats
         assert ("bsr4" in formats) == (name in ("blocks-4x4", "blocks-8x8"))
         assert ("bsr8" in formats) == (name == "blocks-8x8")
+        if name in ("stencil27-64", "powerlaw-100k"):  # 27 diagonals / rows too long
+            assert ("dia" in formats) == (name == "stencil27-64")
         seen.add(rec["format"])
         y = T(xd)
         torch.cuda.synchronize()
@@ -82,3 +84,15 @@ def test_autotune_sparse_matrix_skips_bsr4(bsp_host):
     with pytest.raises(BspError) as e:
         Tuned(DeviceCsr.from_host(A), formats=["bsr4"], reps=2)
     assert e.value.status == 5  # BSP_ERR_UNSUPPORTED
+
+
+def test_autotune_dia_mask(oracle, bsp_host):
+    # the diagonal layout alone on a 7-point stencil: y is the oracle's bits
+    import torch
+    from benchsp_amd.device import DeviceCsr, Tuned
+    A = bsp_host.stencil3d(30, 20, 10, 7, seed=2)
+    T = Tuned(DeviceCsr.from_host(A), formats=["dia"], reps=2)
+    assert [t["format"] for t in T.trials()] == ["dia"] and T.record["format"] == "dia"
+    x = bsp_host.uniform(A.ncols, 8)
+    y = T(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert np.array_equal(y, oracle.spmv_f64(A.rowptr, A.col, A.val, x))
