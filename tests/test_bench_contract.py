@@ -18,12 +18,19 @@ def test_more_than_one_gpu_is_refused():
 
 def test_roofline_reports_the_dominant_launch():
     import bench
-    per = {"a": {"bytes": 1000_000_000, "us": 500.0, "column_segments": 1},
-           "b": {"bytes": 300_000_000, "us": 100.0, "column_segments": 4}}
+    csr = {"format": "csr", "method": "adaptive"}
+    per = {"a": {"bytes": 1000_000_000, "us": 500.0, "column_segments": 1, "layout": csr},
+           "b": {"bytes": 300_000_000, "us": 100.0, "column_segments": 4, "layout": csr}}
     r = bench.roofline(per, "adaptive")
     assert r["launch"].startswith("a SpMV") and r["bound"] == "hbm" and r["unit"] == "GB/s"
     assert abs(r["achieved"] - 2000.0) < 1e-6 and abs(r["frac"] - 0.25) < 1e-9
     assert r["peak"] == 8000.0 and "b" in r["other_launches"]
+    assert r["other_launches"]["b"]["kernel"].startswith("csr_adaptive_xcd_f64")
+    # the autotuner's picks: the diagonal layout and a 2048-unit CSR plan
+    per["a"]["layout"] = {"format": "dia", "kernel": 0, "chunk": 0, "seg_cols": 0}
+    per["b"]["layout"] = {"format": "csr", "kernel": 0, "chunk": 2048, "seg_cols": 0}
+    r = bench.roofline(per, "tuned")
+    assert r["kernel"] == "dia_spmv_f64" and abs(r["achieved"] - 2000.0) < 1e-6
     assert r["other_launches"]["b"]["kernel"].startswith("csr_adaptive_xcd_f64")
 
 
