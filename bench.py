@@ -557,7 +557,9 @@ def batch_leg(args, dev):
            "grid": {"fixed": fixed.info()["grid"], "tuned": tuned.info()["grid"]},
            "tuner_choice": {
                "formats": {name: int((ch["format"] == code).sum()) for code, name in
-                           enumerate(("csr", "bsr4", "bsr8", "dasp"))},
+                           enumerate(("csr", "bsr4", "bsr8", "dasp", "dia"))},
+               "spmm_formats": {name: int((ch["spmm_format"] == code).sum()) for code, name in
+                                ((0, "csr"), (4, "dia"))},
                "spmv_windows": {int(k): int(v) for k, v in
                                 zip(*np.unique(ch["spmv_window"], return_counts=True))},
                "spmm_windows": {int(k): int(v) for k, v in

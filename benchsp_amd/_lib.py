@@ -113,6 +113,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_batch_create": (None, [vp, i64, vp, vp, C.POINTER(vp)]),
     "bsp_batch_layout": (None, [vp, vp, vp]),
     "bsp_batch_choice": (None, [vp, vp, vp, vp]),
+    "bsp_batch_spmm_format": (None, [vp, vp]),
     "bsp_batch_info": (None, [vp, i64p, i64p, i64p]),
     "bsp_batch_run": (None, [vp, C.c_int, vp, vp, vp, vp, vp]),
     "bsp_batch_graph_capture": (None, [vp, C.c_int, vp, vp, vp, vp]),

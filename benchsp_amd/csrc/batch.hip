@@ -1,9 +1,11 @@
 // Many CSR matrices of uneven size per step: FP64 SpMV and an 8-column FP16
 // SpMM for each, every matrix's work cut into tiles of one persistent launch
-// per operation. A per-matrix autotuner picks the format (CSR or 4x4 BSR) and
-// the tile window (rows per workgroup, hence lanes per row) by timing the
-// candidates; a hipGraph holds a whole step. A plain schedule (one
-// thread-per-row launch per matrix and operation) is kept as the comparison.
+// per operation. A per-matrix autotuner picks the format (CSR, 4x4 or 8x8
+// BSR, DASP or the diagonal layout for the SpMV; CSR or the diagonal layout
+// for the SpMM) and the tile window (rows per workgroup, hence lanes per
+// row) by timing the candidates; a hipGraph holds a whole step. A plain
+// schedule (one thread-per-row launch per matrix and operation) is kept as
+// the comparison.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,13 +14,17 @@
 
 #include "csr_adaptive.hpp"
 #include "dasp_kernel.hpp"
+#include "dia.hpp"
 
 namespace bsp {
 namespace {
 
 constexpr int kCap = 1024;  // window capacity of adaptive_rows<kCap>
 constexpr int kCols = 8;    // SpMM columns
-constexpr int kKindCsr = 0, kKindBsr4 = 1, kKindBsr8 = 2, kKindDasp = 3;
+constexpr int kKindCsr = 0, kKindBsr4 = 1, kKindBsr8 = 2, kKindDasp = 3, kKindDia = 4;
+// Diagonals a batch matrix may have to run in the diagonal layout (the
+// 5- and 7-point stencils of a mixed batch have 5 and 7).
+constexpr int kBatchDia = 8;
 constexpr int kDaspPerTile = 4;  // workgroups of a matrix's DASP launch per tile
 // A BSR (4x4 or 8x8) is a candidate only where it moves no more bytes than
 // CSR: at least 11/16 of its block entries filled on average.
@@ -48,8 +54,13 @@ struct BMat {
   const int32_t* bci8;
   const double* bval8;
   const dasp::DaspArgs* dasp;  // DASP layout (a candidate or chosen), x/y unset
+  const double* dval;          // diagonal layout (a candidate or chosen): K x nrows
+  const uint16_t* dhval;       // its FP16 copy, for the SpMM
+  const uint32_t* dmask;       // row masks
   int64_t xoff, yoff;
   int32_t nrows, ncols;
+  int32_t ndiag, pad;
+  int32_t doff[kBatchDia];     // ascending diagonal offsets
 };
 
 __device__ __forceinline__ float half_to_f32(uint16_t h) {
@@ -117,6 +128,79 @@ __device__ __forceinline__ void bsr8_rows(const int32_t* __restrict__ brp,
   for (int o = 8; o < kWave; o <<= 1) acc += __shfl_xor(acc, o, 64);
   const int row = br * 8 + r;
   if (slot == 0 && row < nrows) y[row] = acc;
+}
+
+// Rows [r0, r1) (at most kWG) of y = A x for a matrix in the diagonal layout,
+// a thread per row, four diagonals at a time (registers: the batch kernel is
+// held to 64 VGPRs): their value slots and x at the clamped columns loaded
+// first, then added in ascending order where the row's mask bit is set,
+// each product rounded before its add (the CPU loop's bits).
+constexpr int kDiaStep = 4;
+__device__ __forceinline__ void dia_rows_f64(const BMat* __restrict__ d, const double* x,
+                                             double* y, int r0, int r1) {
+#pragma clang fp contract(off)
+  const int r = r0 + (int)threadIdx.x;
+  if (r >= r1) return;
+  const int n = d->nrows, K = d->ndiag, nc = d->ncols;
+  const uint32_t m = d->dmask[r];
+  double s = 0.0;
+#pragma unroll
+  for (int k0 = 0; k0 < kBatchDia; k0 += kDiaStep) {
+    if (k0 >= K) break;
+    double a[kDiaStep], b[kDiaStep];
+#pragma unroll
+    for (int i = 0; i < kDiaStep; ++i)
+      if (k0 + i < K) {
+        a[i] = ld_stream(&d->dval[(int64_t)(k0 + i) * n + r]);
+        b[i] = x[min(max(r + d->doff[k0 + i], 0), nc - 1)];
+      }
+#pragma unroll
+    for (int i = 0; i < kDiaStep; ++i)
+      if (k0 + i < K) {
+        const double t = s + a[i] * b[i];
+        s = ((m >> (k0 + i)) & 1u) ? t : s;
+      }
+  }
+  y[r] = s;
+}
+
+// Rows [r0, r1) (at most kWG) of Y = A X for a matrix in the diagonal layout,
+// a thread per row, four diagonals at a time: per diagonal a 2-byte value
+// and the 16-byte X row at the clamped column (consecutive rows: coalesced),
+// FP32 sums in ascending diagonal order over the slots the row holds.
+__device__ __forceinline__ void dia_rows_spmm8(const BMat* __restrict__ d, const uint4* X,
+                                               float* Y, int r0, int r1) {
+  const int r = r0 + (int)threadIdx.x;
+  if (r >= r1) return;
+  const int n = d->nrows, K = d->ndiag, nc = d->ncols;
+  const uint32_t m = d->dmask[r];
+  float acc[kCols] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k0 = 0; k0 < kBatchDia; k0 += kDiaStep) {
+    if (k0 >= K) break;
+    uint16_t hv[kDiaStep];
+    uint4 xq[kDiaStep];
+#pragma unroll
+    for (int i = 0; i < kDiaStep; ++i)
+      if (k0 + i < K) {
+        hv[i] = ld_stream(&d->dhval[(int64_t)(k0 + i) * n + r]);
+        xq[i] = X[min(max(r + d->doff[k0 + i], 0), nc - 1)];
+      }
+#pragma unroll
+    for (int i = 0; i < kDiaStep; ++i)
+      if (k0 + i < K && ((m >> (k0 + i)) & 1u)) {
+        const float v = half_to_f32(hv[i]);
+        const uint32_t w[4] = {xq[i].x, xq[i].y, xq[i].z, xq[i].w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          acc[2 * c] += v * half_to_f32((uint16_t)(w[c] & 0xffff));
+          acc[2 * c + 1] += v * half_to_f32((uint16_t)(w[c] >> 16));
+        }
+      }
+  }
+  float4* out = reinterpret_cast<float4*>(Y + (int64_t)r * kCols);
+  out[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  out[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
 }
 
 // Rows [r0, r1) with entries [p0, p1) of Y = A X (at most kCapM rows and
@@ -255,6 +339,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     else if (T.kind == kKindBsr8)
       bsr8_rows(d->brp8, d->bci8, d->bval8, x + d->xoff, y + d->yoff, T.r0, T.r1, d->nrows,
                 d->ncols);
+    else if (T.kind == kKindDia)
+      dia_rows_f64(d, x + d->xoff, y + d->yoff, T.r0, T.r1);
     else if constexpr (DASP)
       dasp_tile(d->dasp, x + d->xoff, y + d->yoff, T.r0, T.r1);
     __syncthreads();
@@ -272,8 +358,11 @@ __global__ __launch_bounds__(kWG) void batch_spmm8_f16(const BTile* __restrict__
   for (; t < t1; ++t) {
     const BTile T = tiles[t];
     const BMat* m = mats + T.m;
-    spmm8_rows(m->rowptr, m->col, m->hval, X + m->xoff, Y + m->yoff * kCols, T.r0, T.p0, T.r1,
-               T.p1);
+    if (T.kind == kKindDia)
+      dia_rows_spmm8(m, X + m->xoff, Y + m->yoff * kCols, T.r0, T.r1);
+    else
+      spmm8_rows(m->rowptr, m->col, m->hval, X + m->xoff, Y + m->yoff * kCols, T.r0, T.p0,
+                 T.r1, T.p1);
     __syncthreads();  // LDS is reused by the next tile
   }
 }
@@ -371,9 +460,18 @@ void dasp_tiles(const dasp::DaspArgs& a, const std::vector<int32_t>& lrpt,
   }
 }
 
+// Tiles of a diagonal-layout matrix: kWG rows each (a thread per row); p1
+// carries the slots a tile reads (rows x diagonals), its weight.
+void dia_tiles(int n, int ndiag, int m, std::vector<BTile>& out) {
+  for (int r = 0; r < n; r += kWG) {
+    const int e = std::min(n, r + kWG);
+    out.push_back(BTile{m, kKindDia, r, e, 0, (e - r) * ndiag, 0, 0});
+  }
+}
+
 int64_t tile_weight(const BTile& t) {
   if (t.kind == kKindCsr) return (int64_t)(t.r1 - t.r0) + (t.p1 - t.p0) + kTileOverhead;
-  if (t.kind == kKindDasp) return (int64_t)(t.p1 - t.p0) + kTileOverhead;
+  if (t.kind == kKindDasp || t.kind == kKindDia) return (int64_t)(t.p1 - t.p0) + kTileOverhead;
   return (int64_t)(t.p1 - t.p0) * (t.kind == kKindBsr4 ? 16 : 64) + kTileOverhead;
 }
 
@@ -425,8 +523,11 @@ struct bsp_batch_s {
   std::vector<bsp_csr> A;
   std::vector<int64_t> row_off, col_off;
   std::vector<int32_t> fmt, win_spmv, win_spmm;
+  std::vector<int32_t> fmt_spmm;   // BSP_BATCH_CSR or BSP_BATCH_DIA per matrix
   std::vector<bsp_bsr> bsr, bsr8;  // owned 4x4 / 8x8 BSR per matrix (null unless chosen)
   std::vector<bsp_dasp> dasp;      // owned DASP layout per matrix (null unless chosen)
+  std::vector<bsp_dia> dia;        // owned diagonal layout per matrix (null unless chosen)
+  std::vector<DevBuf<uint16_t>> dia_h;  // its FP16 values (the SpMM's)
   DevBuf<dasp::DaspArgs> dargs;    // their SpMV arguments (x, y unset)
   DevBuf<uint16_t> hval;     // FP16 copies of every matrix's values
   DevBuf<BMat> mats;
@@ -452,6 +553,8 @@ struct bsp_batch_s {
       if (h) bsp_bsr_destroy(h);
     for (bsp_dasp h : dasp)
       if (h) bsp_dasp_destroy(h);
+    for (bsp_dia h : dia)
+      if (h) bsp_dia_destroy(h);
   }
 };
 
@@ -493,6 +596,14 @@ bsp_status upload_mats(bsp_batch_s* b, hipStream_t s) {
       bsp_dasp_view_get(b->dasp[m], &v);
       da[m] = dasp::dasp_args(v, nullptr, nullptr, 0);
       d.dasp = b->dargs.get() + m;
+    }
+    if (b->dia[m]) {
+      const bsp_dia_s* g = b->dia[m];
+      d.dval = g->val.get();
+      d.dhval = b->dia_h[m].get();
+      d.dmask = g->mask.get();
+      d.ndiag = g->ndiag;
+      for (int k = 0; k < g->ndiag && k < kBatchDia; ++k) d.doff[k] = g->off.d[k];
     }
     h[m] = d;
     hoff += A.nnz;
@@ -603,6 +714,8 @@ bsp_status op_tiles(const bsp_batch_s* b, const std::vector<std::vector<int32_t>
       bsr_tiles(brp, (int)m, f == BSP_BATCH_BSR4 ? kKindBsr4 : kKindBsr8, out);
     } else if (!spmm && uniform == 0 && f == BSP_BATCH_DASP) {
       BSP_TRY(dasp_matrix_tiles(b->dasp[m], (int)m, s, out));
+    } else if (uniform == 0 && (spmm ? b->fmt_spmm[m] : f) == BSP_BATCH_DIA) {
+      dia_tiles((int)b->A[m].nrows, b->dia[m]->ndiag, (int)m, out);
     } else {
       const int c = uniform > 0 ? uniform : spmm ? b->win_spmm[m] : b->win_spmv[m];
       csr_tiles(rp[m], (int)m, c, out);
@@ -685,7 +798,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   hipStream_t s = (hipStream_t)stream;
   const int tune = opt ? opt->tune : 0;
   const int grid = opt ? opt->grid : 0;
-  BSP_REQUIRE(tune >= 0 && tune <= 3, "tune must be 0, 1, 2 or 3");
+  BSP_REQUIRE(tune >= 0 && tune <= 4, "tune must be 0, 1, 2, 3 or 4");
   BSP_REQUIRE(grid >= 0, "grid must be >= 0");
   auto b = std::make_unique<bsp_batch_s>();
   const size_t M = (size_t)count;
@@ -712,6 +825,9 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
   b->bsr.assign(M, nullptr);
   b->bsr8.assign(M, nullptr);
   b->dasp.assign(M, nullptr);
+  b->fmt_spmm.assign(M, BSP_BATCH_CSR);
+  b->dia.assign(M, nullptr);
+  b->dia_h.resize(M);
   BSP_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   BSP_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   BSP_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
@@ -771,6 +887,28 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       BSP_TRY(bsp_dasp_build(&A, 0.75, 256, stream, &b->dasp[m]));
       if (tune == 3) b->fmt[m] = BSP_BATCH_DASP;
     }
+  // The diagonal layout: a candidate (tune = 1), or the format of both
+  // operations (tune = 4, no timing), of every matrix with at most kBatchDia
+  // diagonals and ascending columns (the builder refuses the others).
+  if (tune == 1 || tune == 4)
+    for (size_t m = 0; m < M; ++m) {
+      const bsp_csr& A = b->A[m];
+      if (A.nrows == 0 || A.nnz == 0 || A.nnz > (int64_t)kBatchDia * A.nrows) continue;
+      bsp_dia h = nullptr;
+      const bsp_status st = bsp_dia_build(&A, stream, &h);
+      if (st == BSP_ERR_UNSUPPORTED) continue;
+      BSP_TRY(st);
+      if (h->ndiag > kBatchDia) {
+        bsp_dia_destroy(h);
+        continue;
+      }
+      b->dia[m] = h;
+      const int64_t slots = (int64_t)h->ndiag * A.nrows;
+      BSP_TRY(b->dia_h[m].alloc(slots));
+      f64_to_f16<<<cdiv(slots, kWG), kWG, 0, s>>>(h->val.get(), slots, b->dia_h[m].get());
+      BSP_HIP(hipGetLastError());
+      if (tune == 4) b->fmt[m] = b->fmt_spmm[m] = BSP_BATCH_DIA;
+    }
   if (tune == 2)  // every eligible matrix in BSR (8x8 where eligible), no timing
     for (size_t m = 0; m < M; ++m) {
       if (b->bsr8[m]) {
@@ -822,6 +960,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       b->win_spmv[m] = best_v < kMargin * def_v ? win_v : kCap;
       b->win_spmm[m] = best_m < kMargin * def_m ? win_m : kCapM;
       best_v = b->win_spmv[m] == kCap ? def_v : best_v;
+      best_m = b->win_spmm[m] == kCapM ? def_m : best_m;
       // each BSR candidate must beat the best so far by the margin
       for (const int f : {BSP_BATCH_BSR4, BSP_BATCH_BSR8}) {
         const bsp_bsr h = f == BSP_BATCH_BSR4 ? b->bsr[m] : b->bsr8[m];
@@ -843,6 +982,17 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
           st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
         if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = BSP_BATCH_DASP;
       }
+      // the diagonal layout, for each operation against its best so far
+      if (st == BSP_OK && b->dia[m]) {
+        std::vector<BTile> one;
+        dia_tiles((int)b->A[m].nrows, b->dia[m]->ndiag, (int)m, one);
+        float t = 0;
+        st = time_candidate(one, false, b->spmv.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
+        if (st == BSP_OK && t < kMargin * best_v) best_v = t, b->fmt[m] = BSP_BATCH_DIA;
+        if (st == BSP_OK)
+          st = time_candidate(one, true, b->spmm.resident, b->mats.get(), x.get(), y.get(), X.get(), Y.get(), s, e0, e1, &t);
+        if (st == BSP_OK && t < kMargin * best_m) best_m = t, b->fmt_spmm[m] = BSP_BATCH_DIA;
+      }
       // only the chosen format's layout is kept (decided after DASP's turn)
       if (st == BSP_OK && b->fmt[m] != BSP_BATCH_BSR4 && b->bsr[m]) {
         bsp_bsr_destroy(b->bsr[m]);
@@ -855,6 +1005,12 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       if (st == BSP_OK && b->fmt[m] != BSP_BATCH_DASP && b->dasp[m]) {
         bsp_dasp_destroy(b->dasp[m]);
         b->dasp[m] = nullptr;
+      }
+      if (st == BSP_OK && b->fmt[m] != BSP_BATCH_DIA && b->fmt_spmm[m] != BSP_BATCH_DIA &&
+          b->dia[m]) {
+        bsp_dia_destroy(b->dia[m]);
+        b->dia[m] = nullptr;
+        b->dia_h[m].reset();
       }
     }
     if (st == BSP_OK) st = upload_mats(b.get(), s);
@@ -879,7 +1035,16 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
         b->dasp[m] = nullptr;
       }
     }
-    if (uni_m > 0) std::fill(b->win_spmm.begin(), b->win_spmm.end(), uni_m);
+    if (uni_m > 0) {  // a uniform SpMM window won: every matrix's SpMM in CSR with it
+      std::fill(b->win_spmm.begin(), b->win_spmm.end(), uni_m);
+      std::fill(b->fmt_spmm.begin(), b->fmt_spmm.end(), BSP_BATCH_CSR);
+    }
+    for (size_t m = 0; m < M; ++m)  // diagonal layouts neither operation kept
+      if (b->dia[m] && b->fmt[m] != BSP_BATCH_DIA && b->fmt_spmm[m] != BSP_BATCH_DIA) {
+        bsp_dia_destroy(b->dia[m]);
+        b->dia[m] = nullptr;
+        b->dia_h[m].reset();
+      }
   }
   BSP_TRY(upload_mats(b.get(), s));
 
@@ -905,6 +1070,12 @@ bsp_status bsp_batch_choice(bsp_batch b, int32_t* format, int32_t* spmv_window,
   if (format) std::copy(b->fmt.begin(), b->fmt.end(), format);
   if (spmv_window) std::copy(b->win_spmv.begin(), b->win_spmv.end(), spmv_window);
   if (spmm_window) std::copy(b->win_spmm.begin(), b->win_spmm.end(), spmm_window);
+  return BSP_OK;
+}
+
+bsp_status bsp_batch_spmm_format(bsp_batch b, int32_t* format) {
+  BSP_REQUIRE(b && format, "null argument");
+  std::copy(b->fmt_spmm.begin(), b->fmt_spmm.end(), format);
   return BSP_OK;
 }
 

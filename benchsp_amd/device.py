@@ -440,11 +440,14 @@ class Batch:
         self.row_off, self.col_off = ro, co
         self.nrows, self.ncols = int(ro[-1]), int(co[-1])
 
+    CSR, BSR4, BSR8, DASP, DIA = 0, 1, 2, 3, 4  # formats (bsp_batch_choice)
+
     def choice(self) -> dict:
         M = len(self.mats)
-        f, v, m = (np.zeros(M, np.int32) for _ in range(3))
+        f, v, m, fm = (np.zeros(M, np.int32) for _ in range(4))
         lib().bsp_batch_choice(self._h, f.ctypes.data, v.ctypes.data, m.ctypes.data)
-        return dict(format=f, spmv_window=v, spmm_window=m)
+        lib().bsp_batch_spmm_format(self._h, fm.ctypes.data)
+        return dict(format=f, spmv_window=v, spmm_window=m, spmm_format=fm)
 
     def info(self) -> dict:
         a, b, g = i64(), i64(), i64()

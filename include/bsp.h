@@ -617,12 +617,16 @@ bsp_status bsp_csr_spmv_f16(bsp_spmv_plan p, const uint16_t* x, float* y, bsp_st
  * size: 4x4 and 8x8 BSR wherever their blocks are at least 11/16 full on
  * average, and DASP (threshold 0.75, block_longest 256; its fused launch's
  * workgroups become tiles) for matrices of at least 1024 entries, and
- * keeps a candidate 10% faster than the best before it; then, per
+ * keeps a candidate 10% faster than the best before it; for both
+ * operations the diagonal layout (bsp_dia_build) of every matrix with at
+ * most 8 diagonals (stencils), a thread per row, also against the best
+ * before it; then, per
  * operation, it times the whole batch with each uniform window and with the
  * per-matrix choice, at several launch sizes, and keeps the fastest.
  * tune = 0 uses windows of 1024 (SpMV) and 512 (SpMM); tune = 2 puts every
- * eligible matrix in BSR (8x8 where eligible, else 4x4) and tune = 3 every
- * matrix in DASP, without timing.
+ * eligible matrix in BSR (8x8 where eligible, else 4x4), tune = 3 every
+ * matrix in DASP and tune = 4 every matrix with at most 8 diagonals in the
+ * diagonal layout for both operations, without timing.
  * When a step has both operations, the SpMM runs on a side stream of the
  * batch, forked from and joined back to the caller's stream with events.
  * A captured graph holds one step; replaying it needs no host work. */
@@ -636,15 +640,24 @@ typedef struct bsp_batch_options {
   int32_t grid;
   int64_t reserved[7];
 } bsp_batch_options;
-enum { BSP_BATCH_CSR = 0, BSP_BATCH_BSR4 = 1, BSP_BATCH_BSR8 = 2, BSP_BATCH_DASP = 3 };
+enum {
+  BSP_BATCH_CSR = 0,
+  BSP_BATCH_BSR4 = 1,
+  BSP_BATCH_BSR8 = 2,
+  BSP_BATCH_DASP = 3,
+  BSP_BATCH_DIA = 4
+};
 enum { BSP_BATCH_PERSISTENT = 0, BSP_BATCH_PLAIN = 1 };
 bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_options* opt,
                             bsp_stream s, bsp_batch* out);
 /* row_off, col_off: count+1 entries each. */
 bsp_status bsp_batch_layout(bsp_batch b, int64_t* row_off, int64_t* col_off);
-/* Per matrix: BSP_BATCH_CSR/BSR4/BSR8/DASP and the CSR windows chosen (count each). */
+/* Per matrix: the SpMV's format (BSP_BATCH_CSR/BSR4/BSR8/DASP/DIA) and the
+ * CSR windows chosen (count each). */
 bsp_status bsp_batch_choice(bsp_batch b, int32_t* format, int32_t* spmv_window,
                             int32_t* spmm_window);
+/* Per matrix: the SpMM's format, BSP_BATCH_CSR or BSP_BATCH_DIA (count). */
+bsp_status bsp_batch_spmm_format(bsp_batch b, int32_t* format);
 bsp_status bsp_batch_info(bsp_batch b, int64_t* spmv_tiles, int64_t* spmm_tiles, int64_t* grid);
 /* One step on stream s; x/y or X/Y may both be NULL to skip that operation. */
 bsp_status bsp_batch_run(bsp_batch b, int schedule, const double* x, double* y, const uint16_t* X,

@@ -1,6 +1,7 @@
 """Batched FP64 SpMV + 8-column FP16 SpMM over many matrices (bsp_batch_*):
-persistent schedule (fixed, autotuned, forced BSR, forced DASP), plain schedule and
-graph replays, each matrix against the oracle."""
+persistent schedule (fixed, autotuned, forced BSR, forced DASP, forced
+diagonal layout), plain schedule and graph replays, each matrix against the
+oracle."""
 import numpy as np
 import pytest
 
@@ -65,7 +66,7 @@ def _device(mats, xs, Xs):
     return dm, x, X
 
 
-@pytest.mark.parametrize("tune", [0, 1, 2, 3])
+@pytest.mark.parametrize("tune", [0, 1, 2, 3, 4])
 def test_batch_persistent(oracle, bsp_host, tune):
     import torch
     from benchsp_amd.device import Batch
@@ -82,11 +83,22 @@ def test_batch_persistent(oracle, bsp_host, tune):
         assert np.all(ch["format"][:-1] == 3)
     if tune == 0:
         assert np.all(ch["format"] == 0) and np.all(ch["spmv_window"] == 1024)
+    if tune == 4:  # the stencils (index mod 3 = 0, 1) in the diagonal layout, for both
+        dia = np.array([i < 30 and i % 3 < 2 for i in range(len(mats))])
+        assert np.all((ch["format"] == Batch.DIA) == dia), ch["format"]
+        assert np.all((ch["spmm_format"] == Batch.DIA) == dia), ch["spmm_format"]
+    if tune != 4:
+        assert np.all((ch["spmm_format"] == Batch.CSR) | (ch["spmm_format"] == Batch.DIA))
     y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
     Y = torch.full((B.nrows, COLS), float("nan"), dtype=torch.float32, device="cuda")
     B.run(x, y, X, Y)
     torch.cuda.synchronize()
-    _check(oracle, mats, xs, Xs, y.cpu().numpy(), Y.cpu().numpy(), B)
+    yh = y.cpu().numpy()
+    _check(oracle, mats, xs, Xs, yh, Y.cpu().numpy(), B)
+    for i in np.nonzero(ch["format"] == Batch.DIA)[0]:  # the CPU loop's bits
+        A = mats[i]
+        assert np.array_equal(yh[B.row_off[i]:B.row_off[i + 1]],
+                              oracle.spmv_f64(A.rowptr, A.col, A.val, xs[i])), i
 
 
 def test_batch_plain_schedule_matches(oracle, bsp_host):
@@ -107,11 +119,11 @@ def test_batch_plain_schedule_matches(oracle, bsp_host):
                               oracle.spmv_f64(A.rowptr, A.col, A.val, xs[i]))
 
 
-@pytest.mark.parametrize("tune", [1, 2, 3])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4])
 def test_batch_graph_replays(oracle, bsp_host, tune):
     # the graph reads x and X at replay time: change them between replays,
     # with eager work and synchronisation in between, and check every replay
-    # (tune 2 and 3 put BSR and DASP tiles in the captured launch)
+    # (tune 2, 3 and 4 put BSR, DASP and diagonal tiles in the captured launch)
     import torch
     from benchsp_amd.device import Batch
     mats = _mix(bsp_host)

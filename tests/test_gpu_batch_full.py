@@ -20,7 +20,7 @@ def full_mix(bsp_host):
     return mats
 
 
-@pytest.mark.parametrize("tune", [0, 1])
+@pytest.mark.parametrize("tune", [0, 1, 4])
 def test_batch_full_workload(oracle, bsp_host, full_mix, tune):
     import torch
     from benchsp_amd.device import Batch
@@ -32,6 +32,10 @@ def test_batch_full_workload(oracle, bsp_host, full_mix, tune):
     ch = B.choice()
     if tune == 0:
         assert np.all(ch["format"] == 0)
+    if tune == 4:  # every stencil in the diagonal layout, the random third in CSR
+        stencil = np.arange(len(mats)) % 3 < 2
+        assert np.all((ch["format"] == Batch.DIA) == stencil)
+        assert np.all((ch["spmm_format"] == Batch.DIA) == stencil)
     y = torch.full((B.nrows,), float("nan"), dtype=torch.float64, device="cuda")
     Y = torch.full((B.nrows, COLS), float("nan"), dtype=torch.float32, device="cuda")
     print(f"[batch-full] tune={tune} eager", flush=True)
