@@ -73,10 +73,16 @@ _KERNELS = {"valu": 1, "mfma": 2, "mfma_bal": 3, "valu_bal": 4}
 def test_bsr_spmv(oracle, bsp_host, kernel, tile):
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
-    if kernel == "mfma" and tile not in (4, 8):
-        pytest.skip("MFMA kernels take 4x4 and 8x8 blocks")
-    if kernel.endswith("_bal") and tile != 4:
-        pytest.skip("the balanced kernels take 4x4 blocks")
+    if (kernel == "mfma" and tile not in (4, 8)) or (kernel.endswith("_bal") and tile != 4):
+        # no such kernel for this block size (MFMA: 4x4 and 8x8; balanced:
+        # 4x4): the call is refused with BSP_ERR_UNSUPPORTED before any launch
+        from benchsp_amd._lib import BspError
+        A = bsp_host.blocksparse(30, 30, tile, 100, seed=1)
+        B = Bsr(DeviceCsr.from_host(A), tile, tile)
+        with pytest.raises(BspError) as e:
+            B(torch.zeros(A.ncols, dtype=torch.float64, device="cuda"), kernel=_KERNELS[kernel])
+        assert e.value.status == 5
+        return
     mats = list(_mats(bsp_host))
     if tile in (4, 8):  # dense blocks of the kernel's size, odd block-row lengths
         mats.append((f"blocks-{tile}", bsp_host.blocksparse(300, 257, tile, 25_000, seed=tile)))
