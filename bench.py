@@ -130,6 +130,20 @@ def cpu_baseline(mats, xs, seconds: float):
     builders = {name: {k: round(O.build_seconds(k, A.nrows, A.ncols, A.rowptr, A.col, A.val) * 1e3, 1)
                        for k in ("bsr4", "dbsr", "dasp")}
                 for name, A in zip(("27pt-64^3", "powerlaw-1M"), mats)}
+    # the block SpMM's FP64 oracle (dense 16x16 blocks, N = 128) on one
+    # core, on the first 64 block rows of the 16384^2 matrix
+    from benchsp_amd import host
+    Bm = host.blocksparse_16k()
+    b = O.bcsr(Bm.nrows, Bm.ncols, Bm.rowptr, Bm.col, Bm.val, 16, 16)
+    nbr = 64
+    nb = int(b["rowptr"][nbr])
+    Bd = host.normal(Bm.ncols * 128, seed=200).reshape(Bm.ncols, 128)
+    t0 = time.perf_counter()
+    O.bsr_spmm_f64(b["rowptr"][:nbr + 1], b["colidx"][:nb], b["val"][:nb * 256], 16, Bd)
+    t_blk = time.perf_counter() - t0
+    block_oracle = {"GFLOPs": round(2.0 * nb * 256 * 128 / t_blk / 1e9, 2), "cores": 1,
+                    "sample": f"{nbr} of {b['brows']} block rows ({nb} blocks), N = 128, "
+                              f"{t_blk:.2f} s"}
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -141,6 +155,7 @@ def cpu_baseline(mats, xs, seconds: float):
             "sample": f"{reps} x (SpMV 27pt-64^3 + SpMV powerlaw-1M), {t_total:.1f} s on "
                       f"{threads} threads, same CSR-equivalent bytes as the GPU value",
             "one_core_GBs": one, "builders_ms_one_core": builders,
+            "block_spmm_oracle_one_core": block_oracle,
             "host": {"nproc": os.cpu_count(), "cpu_model": model}}
 
 
