@@ -896,7 +896,7 @@ bsp_status bsp_batch_create(const bsp_csr* mats, int64_t count, const bsp_batch_
       if (A.nrows == 0 || A.nnz == 0 || A.nnz > (int64_t)kBatchDia * A.nrows) continue;
       bsp_dia h = nullptr;
       const bsp_status st = bsp_dia_build(&A, stream, &h);
-      if (st == BSP_ERR_UNSUPPORTED) continue;
+      if (st == BSP_ERR_UNSUPPORTED || st == BSP_ERR_UNSORTED) continue;
       BSP_TRY(st);
       if (h->ndiag > kBatchDia) {
         bsp_dia_destroy(h);

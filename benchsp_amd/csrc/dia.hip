@@ -120,7 +120,7 @@ bsp_status bsp_dia_build(const bsp_csr* A, bsp_stream stream, bsp_dia* out) {
   BSP_TRY(csr_check(*A, s, &flags));
   if (flags & 3) return fail(BSP_ERR_INVALID, "malformed CSR (row pointers or column range)");
   if (flags & 4)
-    return fail(BSP_ERR_UNSUPPORTED, "the diagonal layout needs strictly ascending columns");
+    return fail(BSP_ERR_UNSORTED, "the diagonal layout needs strictly ascending columns");
   auto h = std::make_unique<bsp_dia_s>();
   h->nrows = n;
   h->ncols = A->ncols;

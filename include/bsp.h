@@ -313,8 +313,9 @@ bsp_status bsp_dasp_destroy(bsp_dasp h);
  * (8 bytes per stored slot + 4 per row). y is bit-identical to the
  * row-sequential CSR sum of src/ref.cpp:26-37 (ascending columns, products
  * rounded before each add). BSP_ERR_UNSUPPORTED when the matrix has more
- * than 32 distinct diagonals or a row's columns are not strictly ascending;
- * BSP_ERR_INVALID for a malformed pattern (checked on the GPU). */
+ * than 32 distinct diagonals, BSP_ERR_UNSORTED when a row's columns are not
+ * strictly ascending, BSP_ERR_INVALID for a malformed pattern (all checked
+ * on the GPU). */
 typedef struct bsp_dia_s* bsp_dia;
 bsp_status bsp_dia_build(const bsp_csr* A, bsp_stream s, bsp_dia* out);
 /* Number of diagonals, their offsets ascending (offsets may be NULL, else
