@@ -32,7 +32,23 @@ def _mix(bsp_host):
         np.add.at(rp, r + 1, 1)
         mats.append(Csr(keep[0], keep[1], np.cumsum(rp), c.astype(np.int32), d[r, c]))
     mats.append(Csr(0, 5, np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0)))
+    # 8 diagonals with holes, wider than tall (the diagonal layout's masks and
+    # clamped columns; eligible for it)
+    mats.append(_banded_holes(rng, 700, 900, [-300, -2, -1, 0, 1, 5, 64, 899]))
     return mats
+
+
+def _banded_holes(rng, nrows, ncols, offsets, keep=0.7):
+    rows, cols = [], []
+    for r in range(nrows):
+        for d in offsets:
+            if 0 <= r + d < ncols and rng.random() < keep:
+                rows.append(r)
+                cols.append(r + d)
+    rp = np.zeros(nrows + 1, np.int64)
+    np.add.at(rp, np.array(rows, np.int64) + 1, 1)
+    return Csr(nrows, ncols, np.cumsum(rp), np.array(cols, np.int32),
+               rng.uniform(-1, 1, len(cols)))
 
 
 def _vectors(bsp_host, mats, seed):
@@ -77,14 +93,15 @@ def test_batch_persistent(oracle, bsp_host, tune):
     ch = B.choice()
     assert set(np.unique(ch["spmv_window"])) <= {128, 256, 512, 1024}
     if tune == 2:  # the dense-block matrices run as 8x8 and 4x4 BSR
-        assert ch["format"][-3] == 2 and ch["format"][-2] == 1
-        assert np.all(ch["format"][:-3] == 0)
+        assert ch["format"][-4] == 2 and ch["format"][-3] == 1
+        assert np.all(ch["format"][:-4] == 0) and ch["format"][-1] == 0
     if tune == 3:  # every matrix with rows as DASP tiles
-        assert np.all(ch["format"][:-1] == 3)
+        assert np.all(ch["format"][:-2] == 3) and ch["format"][-1] == 3
     if tune == 0:
         assert np.all(ch["format"] == 0) and np.all(ch["spmv_window"] == 1024)
-    if tune == 4:  # the stencils (index mod 3 = 0, 1) in the diagonal layout, for both
-        dia = np.array([i < 30 and i % 3 < 2 for i in range(len(mats))])
+    if tune == 4:  # the stencils (index mod 3 = 0, 1) and the banded matrix (last) in the
+        # diagonal layout, for both operations
+        dia = np.array([(i < 30 and i % 3 < 2) or i == len(mats) - 1 for i in range(len(mats))])
         assert np.all((ch["format"] == Batch.DIA) == dia), ch["format"]
         assert np.all((ch["spmm_format"] == Batch.DIA) == dia), ch["spmm_format"]
     if tune != 4:
