@@ -600,9 +600,11 @@ def gather_probe():
     """The power-law 2^20 SpMV's data movement without its arithmetic
     (bsp_csr_plan_probe over the same XCD-pinned column segments): x gathers
     with the column-index stream, then with the value stream too, beside the
-    SpMV launch itself (kernel + partial sums). Two rotated copies of the
-    matrix, as in the headline leg; median of 20 event-timed launches each.
-    Prints one JSON line (microseconds)."""
+    SpMV launch itself (kernel + partial sums), and the same number of
+    random gathers from the same 2 MB slices with no stream at all
+    (bsp_gather_probe: the L2's gather rate, the ceiling of this layout).
+    Two rotated copies of the matrix, as in the headline leg; median of 20
+    event-timed launches each. Prints one JSON line (microseconds)."""
     import ctypes as C
     import numpy as np
     import torch
@@ -621,7 +623,18 @@ def gather_probe():
         lib().bsp_csr_plan_probe(ops[i % 2]._h, vals, ptr(xd), ptr(sink), sink.numel(), None,
                                  stream_handle())
 
-    runs = {"gathers_col_stream": lambda i: probe(i, 0),
+    # the same number of random gathers from the same four 2 MB slices of x,
+    # without any stream: the L2's random-gather rate
+    gp = C.c_int64()
+    lib().bsp_gather_probe(None, A.ncols, 4, A.nnz, None, 0, C.byref(gp), None)
+    gsink = torch.empty(gp.value, dtype=torch.float64, device="cuda")
+
+    def ceiling(i):
+        lib().bsp_gather_probe(ptr(xd), A.ncols, 4, A.nnz, ptr(gsink), gsink.numel(), None,
+                               stream_handle())
+
+    runs = {"gathers_only": ceiling,
+            "gathers_col_stream": lambda i: probe(i, 0),
             "gathers_col_val_stream": lambda i: probe(i, 1),
             "spmv": lambda i: ops[i % 2](xd, y)}
     out = {"matrix": f"power-law 2^20, nnz={A.nnz}, {ops[0].segments()[1]} XCD-pinned segments",

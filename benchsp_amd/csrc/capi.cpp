@@ -161,6 +161,12 @@ bsp_status bsp_csr_plan_probe(bsp_spmv_plan p, int with_values, const double* x,
   return csr_plan_probe(p, with_values, x, sink, sink_len, grid, (hipStream_t)s);
 }
 
+bsp_status bsp_gather_probe(const double* x, int64_t n, int32_t slices, int64_t gathers,
+                            double* sink, int64_t sink_len, int64_t* grid, bsp_stream s) {
+  BSP_REQUIRE(x || !sink, "null argument");
+  return gather_probe(x, n, slices, gathers, sink, sink_len, grid, (hipStream_t)s);
+}
+
 bsp_status bsp_csr_check(const bsp_csr* A, bsp_stream s, int32_t* sorted) {
   BSP_TRY(check_csr_pattern(A));
   if (sorted) *sorted = 1;

@@ -52,4 +52,6 @@ bsp_status csr_plan_run_f16(const CsrPlan* p, const uint16_t* x, float* y, hipSt
 bsp_status csr_check(const bsp_csr& A, hipStream_t s, int* flags);
 bsp_status csr_plan_probe(const CsrPlan* p, int with_values, const double* x, double* sink,
                           int64_t sink_len, int64_t* grid, hipStream_t s);
+bsp_status gather_probe(const double* x, int64_t n, int nslices, int64_t gathers, double* sink,
+                        int64_t sink_len, int64_t* grid, hipStream_t s);
 }  // namespace bsp

@@ -150,6 +150,34 @@ __global__ __launch_bounds__(kWG) void probe_gather(const SegDesc* __restrict__ 
   if (threadIdx.x == 0) sink[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Random 8-byte gathers alone (no streams, no arithmetic beyond a sum): the
+// rate at which the L2 serves them. Workgroup i, on XCD i % 8, gathers from
+// slice (i % 8) % nslices of x (slice length a power of two), each thread
+// kProbePer times at xorshift-generated positions; one sum per workgroup is
+// stored so nothing is optimised away.
+constexpr int kProbePer = 32;
+__global__ __launch_bounds__(kWG) void probe_random_gather(const double* __restrict__ x,
+                                                           int nslices, int slice_log2,
+                                                           double* __restrict__ sink) {
+  __shared__ double red[kWG / kWave];
+  const int s = (blockIdx.x % 8) % nslices;
+  const double* xs = x + ((int64_t)s << slice_log2);
+  const uint32_t mask = (1u << slice_log2) - 1;
+  uint32_t h = (uint32_t)(blockIdx.x * kWG + threadIdx.x) * 2654435761u + 12345u;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < kProbePer; ++k) {
+    h ^= h << 13;
+    h ^= h >> 17;
+    h ^= h << 5;
+    acc += xs[h & mask];
+  }
+  acc = group_sum<kWave>(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sink[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void sum_partials(const double* __restrict__ part, int nseg, int64_t n,
                              double* __restrict__ y) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -733,6 +761,25 @@ bsp_status csr_plan_probe(const CsrPlan* p, int with_values, const double* x, do
     probe_gather<true><<<grid, kWG, 0, s>>>(tab, nseg, x, sink);
   else
     probe_gather<false><<<grid, kWG, 0, s>>>(tab, nseg, x, sink);
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status gather_probe(const double* x, int64_t n, int nslices, int64_t gathers, double* sink,
+                        int64_t sink_len, int64_t* grid_out, hipStream_t s) {
+  BSP_REQUIRE(nslices == 1 || nslices == 2 || nslices == 4 || nslices == 8,
+              "slices must be 1, 2, 4 or 8");
+  BSP_REQUIRE(n > 0 && n % nslices == 0 && gathers >= 0, "bad probe sizes");
+  const int64_t len = n / nslices;
+  int lg = 0;
+  while ((int64_t(1) << lg) < len) ++lg;
+  BSP_REQUIRE((int64_t(1) << lg) == len && lg <= 30, "slice length must be a power of two");
+  const int64_t grid = cdiv(cdiv(gathers, kProbePer), kWG);
+  BSP_REQUIRE(grid * kWG < kDispatchLimit, "too many gathers for one launch");
+  if (grid_out) *grid_out = grid;
+  if (!sink || grid == 0) return BSP_OK;  // grid query
+  BSP_REQUIRE(sink_len >= grid, "sink must hold one double per workgroup");
+  probe_random_gather<<<grid, kWG, 0, s>>>(x, nslices, lg, sink);
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

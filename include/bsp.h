@@ -138,6 +138,14 @@ bsp_status bsp_csr_plan_narrow(bsp_spmv_plan p, int32_t* narrow);
  * for other plans. */
 bsp_status bsp_csr_plan_probe(bsp_spmv_plan p, int with_values, const double* x, double* sink,
                               int64_t sink_len, int64_t* grid, bsp_stream s);
+/* Measurement aid, no counterpart in the reference: `gathers` random 8-byte
+ * reads of x alone (no streams), x split into `slices` equal slices (1, 2, 4
+ * or 8; a power-of-two length each), workgroup i gathering from slice
+ * (i % 8) % slices, i.e. on the XCDs that own it, as a segmented plan's
+ * launch does: the rate at which the L2 serves random gathers. One sum per
+ * workgroup goes to sink; sink == NULL returns the workgroup count only. */
+bsp_status bsp_gather_probe(const double* x, int64_t n, int32_t slices, int64_t gathers,
+                            double* sink, int64_t sink_len, int64_t* grid, bsp_stream s);
 /* y = A*x. x has ncols, y has nrows doubles (device). Stream-ordered. */
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s);
 /* Number of workgroups one bsp_csr_spmv launch uses (its main kernel). */

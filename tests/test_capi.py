@@ -318,3 +318,16 @@ def test_noalloc_spmv_rejects_sizes_beyond_its_schedule_array():
     with pytest.raises(BspError) as e:
         lib().bsp_csr_spmv_noalloc(C.byref(d), 8, 8, None)
     assert e.value.status == 1
+
+
+def test_gather_probe_grid_query_and_argument_checks():
+    # sink == NULL: the workgroup count only, no device access; bad slice
+    # counts and non-power-of-two slices are argument errors
+    from benchsp_amd._lib import BspError, lib
+    g = C.c_int64()
+    lib().bsp_gather_probe(None, 1 << 20, 4, 20_985_088, None, 0, C.byref(g), None)
+    assert g.value == -(-(-(-20_985_088 // 32)) // 256)
+    for n, slices in ((1 << 20, 3), (3 << 18, 4), (0, 1)):
+        with pytest.raises(BspError) as e:
+            lib().bsp_gather_probe(None, n, slices, 1000, None, 0, C.byref(g), None)
+        assert e.value.status == 1
