@@ -251,8 +251,9 @@ def test_compat_header_driver_builds(tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
 
 
-@pytest.mark.parametrize("which", ["bsr", "dbsr", "dasp", "spmm", "blockinfo", "rocsparse",
-                                   "rocsparse_spmm", "batch", "autotune", "tuned_from_record"])
+@pytest.mark.parametrize("which", ["bsr", "dbsr", "dasp", "dia", "spmm", "blockinfo",
+                                   "rocsparse", "rocsparse_spmm", "batch", "autotune",
+                                   "tuned_from_record"])
 @pytest.mark.parametrize("size", ["nnz", "nrows", "ncols", "negative"])
 def test_builders_reject_bad_sizes_before_any_gpu_work(which, size):
     # every builder validates the descriptor first: sizes past 2^31 - 2^20
@@ -273,6 +274,7 @@ def test_builders_reject_bad_sizes_before_any_gpu_work(which, size):
         "bsr": lambda: L.bsp_bsr_build(C.byref(d), 4, 4, None, C.byref(h)),
         "dbsr": lambda: L.bsp_dbsr_build(C.byref(d), None, C.byref(h)),
         "dasp": lambda: L.bsp_dasp_build(C.byref(d), 0.75, 256, None, C.byref(h)),
+        "dia": lambda: L.bsp_dia_build(C.byref(d), None, C.byref(h)),
         "spmm": lambda: L.bsp_spmm_plan_create(C.byref(d), None, C.byref(h)),
         "blockinfo": lambda: L.bsp_block_info(C.byref(d), 4, None, C.byref(BlockInfo())),
         "rocsparse": lambda: L.bsp_rocsparse_plan_create(C.byref(d), 0, None, C.byref(h)),
