@@ -31,7 +31,9 @@ the library (scalar / balanced / split / adaptive CSR, adaptive CSR with
 16-bit column offsets, 4x4 BSR on MFMA and VALU per block row and balanced
 over 64-block chunks, DBSR and DASP on VALU and on FP64 MFMA, the diagonal
 layout where the matrix has at most 32 diagonals), the layout the
-per-matrix autotuner picks, and rocSPARSE, as CSR-equivalent effective
+per-matrix autotuner picks, and rocSPARSE with each of its CSR algorithms
+(default, adaptive, rowsplit, LRB; the fastest as
+`rocsparse_best`), as CSR-equivalent effective
 GB/s, with the wall time of each GPU format build (plan, analysis or format
 conversion from a device CSR). The CPU baseline leg times the restated host
 builders of the BSR4, DBSR and DASP formats on one core beside it.
@@ -443,6 +445,9 @@ def formats_leg(args, dev, mats, xs):
             "dbsr": lambda d: Dbsr(d),
             "dbsr_mfma": lambda d: (lambda h: (lambda xx, yy: h(xx, yy, kernel=1)))(Dbsr(d)),
             "rocsparse": lambda d: RocsparseSpmv(d, 0),
+            "rocsparse_adaptive": lambda d: RocsparseSpmv(d, 2),
+            "rocsparse_rowsplit": lambda d: RocsparseSpmv(d, 3),
+            "rocsparse_lrb": lambda d: RocsparseSpmv(d, 7),
         }
         # the autotuner picks a layout once (cache-warm timing of every
         # candidate); each copy is then rebuilt from its record
@@ -487,6 +492,9 @@ def formats_leg(args, dev, mats, xs):
                         "build_ms": round(float(np.median(build)) * 1e3, 2)}
             del ops
         res["autotuned"].update(tune_ms=round(tune_ms, 1), record=rec)
+        rs = [k for k in res if k.startswith("rocsparse")]
+        res["rocsparse_best"] = dict(res[min(rs, key=lambda k: res[k]["us"])],
+                                     algorithm=min(rs, key=lambda k: res[k]["us"]))
         if dia_bytes is not None:  # what the diagonal layout streams (values, masks, x, y)
             res["dia"].update(bytes=dia_bytes,
                               actual_GBs=round(dia_bytes / (res["dia"]["us"] * 1e-6) / 1e9, 1))

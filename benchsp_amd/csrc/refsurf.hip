@@ -206,9 +206,15 @@ bsp_status bsp_rocsparse_plan_create(const bsp_csr* A, int alg, bsp_stream s,
   BSP_TRY(check_csr_pattern(A));
   BSP_REQUIRE(A->dtype == BSP_F64 || A->dtype == BSP_F32 || A->dtype == BSP_F16,
               "rocSPARSE comparator takes FP64, FP32 or FP16 values");
+  BSP_REQUIRE(alg == 0 || alg == 2 || alg == 3 || alg == 7,
+              "rocSPARSE CSR algorithm must be 0 (default), 2 (adaptive), 3 (rowsplit) or 7 "
+              "(LRB)");
   auto p = std::make_unique<bsp_rocsparse_plan_s>();
   p->A = *A;
-  p->alg = (rocsparse_spmv_alg)alg;
+  p->alg = alg == 2   ? rocsparse_spmv_alg_csr_adaptive
+           : alg == 3 ? rocsparse_spmv_alg_csr_rowsplit
+           : alg == 7 ? rocsparse_spmv_alg_csr_lrb
+                      : rocsparse_spmv_alg_default;
   const rocsparse_datatype ta = A->dtype == BSP_F64   ? rocsparse_datatype_f64_r
                                 : A->dtype == BSP_F32 ? rocsparse_datatype_f32_r
                                                       : rocsparse_datatype_f16_r;

@@ -707,8 +707,9 @@ void bsp_ref_spmv_fp32(int* A_csrOffsets, int32_t* A_columns, float* A_values, f
 void bsp_ref_spmv_fp16(int* A_csrOffsets, int32_t* A_columns, float* A_values, float* x,
                        float* y, int32_t row, int32_t col);
 
-/* Device-level rocSPARSE SpMV with int32 indices and the given algorithm
- * (0 default, 1 CSR stream, 2 CSR adaptive), prepared once (buffer and
+/* Device-level rocSPARSE SpMV with int32 indices and the given CSR
+ * algorithm (rocsparse_spmv_alg codes: 0 default, 2 adaptive, 3 rowsplit,
+ * 7 LRB; BSP_ERR_INVALID otherwise), prepared once (buffer and
  * analysis) and run per call: the vendor point reported next to the
  * library's kernels. Value types by A->dtype: F64 (x, y FP64), F32 (x, y
  * FP32), F16 (x FP16, y FP32, FP32 compute). */

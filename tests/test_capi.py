@@ -333,3 +333,15 @@ def test_gather_probe_grid_query_and_argument_checks():
         with pytest.raises(BspError) as e:
             lib().bsp_gather_probe(None, n, slices, 1000, None, 0, C.byref(g), None)
         assert e.value.status == 1
+
+
+def test_rocsparse_comparator_rejects_non_csr_algorithms():
+    # rocsparse_spmv_alg codes other than the CSR ones (1 COO, 4 ELL, ...)
+    # are argument errors, checked before any device work
+    from benchsp_amd._lib import BspError, CsrDesc, lib
+    d = CsrDesc(0, 0, 0, None, None, None, 0)
+    h = C.c_void_p()
+    for alg in (1, 4, 5, 6, 8, 9, -1):
+        with pytest.raises(BspError) as e:
+            lib().bsp_rocsparse_plan_create(C.byref(d), alg, None, C.byref(h))
+        assert e.value.status == 1, alg

@@ -214,7 +214,10 @@ def test_cusparse_spmv_fp64_is_rocsparse(oracle, bsp_host):
                           what=name)
 
 
-def test_rocsparse_plan_on_device_arrays(oracle, bsp_host):
+@pytest.mark.parametrize("alg", [0, 2, 3, 7])
+def test_rocsparse_plan_on_device_arrays(oracle, bsp_host, alg):
+    # every CSR algorithm of the comparator (default, adaptive, rowsplit,
+    # LRB) against the oracle
     import torch
     from benchsp_amd.device import DeviceCsr
     from benchsp_amd._lib import CsrDesc
@@ -222,7 +225,7 @@ def test_rocsparse_plan_on_device_arrays(oracle, bsp_host):
     d = DeviceCsr.from_host(A)
     desc = d.desc()
     h = C.c_void_p()
-    _L().bsp_rocsparse_plan_create(C.byref(desc), 0, None, C.byref(h))
+    _L().bsp_rocsparse_plan_create(C.byref(desc), alg, None, C.byref(h))
     try:
         x = bsp_host.uniform(A.ncols, 16)
         xd = torch.from_numpy(x).cuda()
@@ -231,7 +234,7 @@ def test_rocsparse_plan_on_device_arrays(oracle, bsp_host):
             _L().bsp_rocsparse_spmv(h, xd.data_ptr(), yd.data_ptr(), None)
         torch.cuda.synchronize()
         assert_spmv_close(yd.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
-                          abs_spmv(oracle, A, x), what="rocsparse")
+                          abs_spmv(oracle, A, x), what=f"rocsparse alg {alg}")
     finally:
         _L().bsp_rocsparse_plan_destroy(h)
     assert isinstance(desc, CsrDesc)
