@@ -334,28 +334,43 @@ def spmm_leg(args, dev):
 def rocsparse_spmm_points(args, dev, A, X, dY, flop):
     """rocSPARSE CSR SpMM on the same matrix and dense operands: forward
     C = A X and transposed A^T dY, FP32 and FP16-in/FP32-out (ms per product
-    and GFLOP/s), the vendor points beside the library's kernels."""
+    and GFLOP/s), the vendor points beside the library's kernels: its default
+    algorithm, and the fastest of its CSR algorithms (csr, row split, nnz
+    split, merge path; combinations it refuses are skipped), each timed over
+    two warmups and three launches."""
     import torch
+    from benchsp_amd._lib import BspError
     from benchsp_amd.device import DeviceCsr, RocsparseSpmm
+    algs = {0: "default", 1: "csr", 4: "row_split", 5: "nnz_split", 9: "merge_path"}
     out = {}
     N = X.shape[1]
     for name, dt in (("fp32", torch.float32), ("fp16", torch.float16)):
         d = DeviceCsr.from_host(A, dev, dtype=dt)
         Xd, dYd = X.to(dt).contiguous(), dY.to(dt).contiguous()
-        for kind, op, operand in (("fwd", RocsparseSpmm(d, N, False), Xd),
-                                  ("transposed", RocsparseSpmm(d, N, True), dYd)):
-            for _ in range(args.warmup):
-                op(operand)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(args.steps):
-                op(operand)
-            e1.record()
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / args.steps
+        for kind, trans, operand in (("fwd", False, Xd), ("transposed", True, dYd)):
+            times = {}
+            for code, alg in algs.items():
+                try:
+                    op = RocsparseSpmm(d, N, trans, code)
+                    for _ in range(2):
+                        op(operand)
+                except BspError:
+                    continue
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(3):
+                    op(operand)
+                e1.record()
+                torch.cuda.synchronize()
+                times[alg] = e0.elapsed_time(e1) / 3
+                del op
+            ms = times["default"]
             out[f"{name}_{kind}_ms"] = round(ms, 4)
             out[f"{name}_{kind}_GFLOPs"] = round(flop / (ms * 1e-3) / 1e9, 1)
+            best = min(times, key=times.get)
+            out[f"{name}_{kind}_best"] = {"algorithm": best, "ms": round(times[best], 4),
+                                          "GFLOPs": round(flop / (times[best] * 1e-3) / 1e9, 1)}
     return out
 
 

@@ -282,8 +282,15 @@ bsp_status bsp_rocsparse_spmm_plan_create(const bsp_csr* A, int64_t N, int trans
   BSP_TRY(check_csr_pattern(A));
   BSP_REQUIRE(A->dtype == BSP_F32 || A->dtype == BSP_F16 || A->dtype == BSP_BF16,
               "rocSPARSE SpMM comparator takes FP32, FP16 or BF16 values");
+  BSP_REQUIRE(alg == 0 || alg == 1 || alg == 4 || alg == 5 || alg == 9,
+              "rocSPARSE CSR SpMM algorithm must be 0 (default), 1 (csr), 4 (row split), "
+              "5 (nnz split) or 9 (merge path)");
   auto p = std::make_unique<bsp_rocsparse_spmm_plan_s>();
-  p->alg = (rocsparse_spmm_alg)alg;
+  p->alg = alg == 1   ? rocsparse_spmm_alg_csr
+           : alg == 4 ? rocsparse_spmm_alg_csr_row_split
+           : alg == 5 ? rocsparse_spmm_alg_csr_nnz_split
+           : alg == 9 ? rocsparse_spmm_alg_csr_merge_path
+                      : rocsparse_spmm_alg_default;
   p->op = transpose ? rocsparse_operation_transpose : rocsparse_operation_none;
   const rocsparse_datatype ta = A->dtype == BSP_F32   ? rocsparse_datatype_f32_r
                                 : A->dtype == BSP_F16 ? rocsparse_datatype_f16_r

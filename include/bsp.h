@@ -723,7 +723,10 @@ bsp_status bsp_rocsparse_plan_destroy(bsp_rocsparse_plan p);
  * src/cusparse.cu:288-390, generalised): C = op(A) B with A a device CSR of
  * FP32, FP16 or BF16 values, B row-major (ncols x N, or nrows x N when
  * transpose) of the same type, C row-major FP32, FP32 compute; prepared once,
- * run per call. The vendor point next to the library's SpMM kernels. */
+ * run per call. The vendor point next to the library's SpMM kernels. `alg`
+ * is a rocsparse_spmm_alg code for CSR: 0 default, 1 csr, 4 row split,
+ * 5 nnz split, 9 merge path (BSP_ERR_INVALID otherwise; rocSPARSE itself
+ * may refuse a combination, reported as BSP_ERR_HIP). */
 typedef struct bsp_rocsparse_spmm_plan_s* bsp_rocsparse_spmm_plan;
 bsp_status bsp_rocsparse_spmm_plan_create(const bsp_csr* A, int64_t N, int transpose, int alg,
                                           bsp_stream s, bsp_rocsparse_spmm_plan* out);

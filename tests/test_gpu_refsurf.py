@@ -346,8 +346,12 @@ def test_compat_header_builders_match_oracle(oracle, tmp_path):
 
 @pytest.mark.parametrize("dtype_name", ["float32", "float16"])
 @pytest.mark.parametrize("transpose", [False, True])
-def test_rocsparse_spmm_comparator(oracle, bsp_host, dtype_name, transpose):
+@pytest.mark.parametrize("alg", [0, 1, 4, 5, 9])
+def test_rocsparse_spmm_comparator(oracle, bsp_host, dtype_name, transpose, alg):
+    # every CSR SpMM algorithm the comparator may report; a combination that
+    # rocSPARSE itself refuses comes back as BSP_ERR_HIP (2), never a wrong C
     import torch
+    from benchsp_amd._lib import BspError
     from benchsp_amd.device import DeviceCsr, RocsparseSpmm
     dt = getattr(torch, dtype_name)
     A = bsp_host.powerlaw(3000, 12.0, 1.2, 400, seed=6, ncols=2500)
@@ -355,9 +359,13 @@ def test_rocsparse_spmm_comparator(oracle, bsp_host, dtype_name, transpose):
     rows_b = A.nrows if transpose else A.ncols
     B = bsp_host.normal(rows_b * N, 19).reshape(rows_b, N)
     d = DeviceCsr.from_host(A, dtype=dt)
-    op = RocsparseSpmm(d, N, transpose)
     Bd = torch.from_numpy(B).to("cuda", dt)
-    Cm = op(Bd)
+    try:
+        op = RocsparseSpmm(d, N, transpose, alg)
+        Cm = op(Bd)
+    except BspError as e:
+        assert alg != 0 and e.status == 2, str(e)
+        return
     torch.cuda.synchronize()
     a = d.val.double().cpu().numpy()  # the operands as stored
     Bq = Bd.double().cpu().numpy()
