@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -55,8 +56,8 @@ bsp_status assemble(int64_t nrows, int64_t ncols, std::vector<int32_t>& r, std::
 
 extern "C" {
 
-bsp_status bsp_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rows,
-                          const int32_t* cols, const double* vals, bsp_host_csr* out) {
+static bsp_status coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rows,
+                             const int32_t* cols, const double* vals, bsp_host_csr* out) {
   BSP_REQUIRE(out && nrows >= 0 && ncols >= 0 && nnz >= 0, "bad COO arguments");
   BSP_REQUIRE(nnz == 0 || (rows && cols), "null COO arrays");
   if (!sizes_fit(nrows, ncols, nnz))
@@ -74,7 +75,7 @@ bsp_status bsp_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32
   return BSP_OK;
 }
 
-bsp_status bsp_mtx_read(const char* path, bsp_host_csr* out) {
+static bsp_status mtx_read(const char* path, bsp_host_csr* out) {
   BSP_REQUIRE(path && out, "null argument");
   std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "rb"), std::fclose);
   if (!f) return fail(BSP_ERR_INVALID, std::string("cannot open ") + path + ": " + std::strerror(errno));
@@ -117,6 +118,11 @@ bsp_status bsp_mtx_read(const char* path, bsp_host_csr* out) {
   if (M < 0 || N < 0 || L < 0 || end == p) return fail(BSP_ERR_INVALID, "bad size line");
   if (!sizes_fit(M, N, symm || skew ? 2 * L : L))
     return fail(BSP_ERR_TOO_LARGE, "MatrixMarket sizes must fit int32 indices (at most 2^31 - 2^20)");
+  // every entry takes at least four bytes of text ("1 1\n"): a header that
+  // claims more entries than the file can hold is malformed (and must not
+  // size the arrays below)
+  if (L > (long long)(buf.size() - (size_t)(end - buf.c_str())) / 4 + 1)
+    return fail(BSP_ERR_INVALID, "entry count larger than the file can hold");
   std::vector<int32_t> r, c;
   std::vector<double> v;
   const size_t cap = (size_t)(symm || skew ? 2 * L : L);
@@ -153,6 +159,25 @@ bsp_status bsp_mtx_read(const char* path, bsp_host_csr* out) {
   if (st != BSP_OK) return st;
   *out = (bsp_host_csr)h;
   return BSP_OK;
+}
+
+// Host arrays sized by the caller's (or the file's) counts: an allocation
+// that fails is a status, not an exception through the C interface.
+bsp_status bsp_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rows,
+                          const int32_t* cols, const double* vals, bsp_host_csr* out) {
+  try {
+    return coo_to_csr(nrows, ncols, nnz, rows, cols, vals, out);
+  } catch (const std::bad_alloc&) {
+    return fail(BSP_ERR_ALLOC, "host allocation failed");
+  }
+}
+
+bsp_status bsp_mtx_read(const char* path, bsp_host_csr* out) {
+  try {
+    return mtx_read(path, out);
+  } catch (const std::bad_alloc&) {
+    return fail(BSP_ERR_ALLOC, "host allocation failed");
+  }
 }
 
 }  // extern "C"

@@ -87,6 +87,9 @@ def test_coo_to_csr_order_and_errors():
     "%%MatrixMarket matrix coordinate complex general\n1 1 1\n1 1 1 0\n",
     "%%MatrixMarket matrix coordinate real general\n2 2 1\n3 1 1.0\n",
     "%%MatrixMarket matrix coordinate real general\n2 2 2\n1 1 1.0\n",
+    # an entry count the file cannot hold is refused before any array is sized
+    "%%MatrixMarket matrix coordinate real symmetric\n2000000000 2000000000 1000000000\n"
+    "1 1 1.0\n",
     "not a matrix market file\n"])
 def test_read_rejects(tmp_path, text):
     from benchsp_amd._lib import BspError
