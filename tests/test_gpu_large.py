@@ -4,7 +4,9 @@ in slices of at most 2^31 work-items; at this size those launches take two
 slices each: the column-segment scatter of the adaptive plan, DBSR placement,
 the 1x1 BSR SpMV (a block row per row), DASP's row-block fill and the CSC key
 pass of the transposed SpMM. Results against the oracle, and the transposed
-SpMM against its closed form on sampled columns."""
+SpMM against its closed form on sampled columns; the diagonal layout and the
+fused backward (inverse permutation, CSC-ordered SDDMM put back in CSR
+order) at the same size."""
 import numpy as np
 import pytest
 
@@ -71,6 +73,20 @@ def test_dasp(big):
     del D
 
 
+def test_dia_bit_identical(big):
+    # the wrapped offsets give nine diagonals: the diagonal layout holds the
+    # matrix in 9 x 33.6 M slots and returns the CPU loop's bits
+    import torch
+    from benchsp_amd.device import Dia
+    D = Dia(big[1])
+    assert D.info()["ndiag"] == 9
+    y = D(big[2])
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), big[3])
+    del D, y
+    torch.cuda.empty_cache()
+
+
 def test_spmm_transposed_sampled(big):
     # dX = A^T dY, F = 64: column c of A appears in rows (c - o) mod N, one
     # entry per offset o; checked on 256 sampled columns
@@ -81,6 +97,18 @@ def test_spmm_transposed_sampled(big):
     vals = dA.val.float()
     dY = torch.randn(N, 64, device="cuda", generator=torch.Generator("cuda").manual_seed(5))
     dX = S.spmm_t(vals, dY)
+    # the fused backward at this size: the same dX bits, and SDDMM entries on
+    # a sample against their dot products
+    X = torch.randn(N, 64, device="cuda", generator=torch.Generator("cuda").manual_seed(6))
+    fdX, fdv = S.spmm_t_sddmm(vals, dY, X)
+    assert torch.equal(fdX, dX)
+    js = torch.from_numpy(np.random.default_rng(14).choice(A.nnz, 4096, replace=False)).cuda()
+    rj = torch.div(js, 5, rounding_mode="floor")
+    cj = dA.col.long()[js]
+    ref = (dY[rj].double() * X[cj].double()).sum(1)
+    scale = (dY[rj].double().abs() * X[cj].double().abs()).sum(1)
+    assert torch.all((fdv[js].double() - ref).abs() <= 1e-5 * scale + 1e-30)
+    del X, fdX, fdv
     torch.cuda.synchronize()
     cs = np.random.default_rng(13).choice(N, 256, replace=False)
     got = dX[torch.from_numpy(cs).cuda()].cpu().numpy().astype(np.float64)
