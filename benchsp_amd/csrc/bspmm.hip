@@ -13,16 +13,19 @@
 //   A (16xK): lane l holds A[l%16][E*(l/16) .. +E-1]   (E = K/4)
 //   B (Kx16): lane l holds B[E*(l/16) .. +E-1][l%16]
 //   D (16x16 FP32): lane l holds D[4*(l/16)+v][l%16] in acc[v]
-// Both operands are read k-contiguous: A from hv (row-major), B from a
-// per-call panel copy of the dense operand that stores every 16-row slab in
-// MFMA fragment order, so a wave's operand load is 512 contiguous bytes.
+// Every operand load of a wave is one contiguous run of 1 KB (16 bytes per
+// lane) or 512 bytes: the number of cache lines a wave instruction touches,
+// not where the lines live, sets the cost of these gather-heavy products
+// (a layout whose wave loads touch 16 or 32 half-used lines ran at half the
+// speed). A is read from hv (row-major blocks); the dense operand goes
+// through a per-call panel copy in MFMA fragment order.
 //
 //   forward  C[16 rows of br, :] = sum_b scale_b * hv_b * B[16 rows of bc_b, :]
 //   dB       dB[16 rows of bc, :] = sum_b scale_b * hvt_b * dC[16 rows of br_b, :]
 //   dvals    dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T   (K = N)
-// One workgroup per block row (block column for dB); each wave owns a set of
-// 16-column tiles and adds the row's blocks in index order, so results are
-// deterministic.
+// The two products run a workgroup per block row (block column for dB) whose
+// waves take every WB-th block of the row; their partial tiles are added in
+// wave order through LDS, so results are deterministic.
 #include <cmath>
 #include <memory>
 
@@ -35,9 +38,10 @@ struct bsp_bspmm_s {
   const int32_t* bcol = nullptr;
   bsp::DevBuf<float> scale;
   bsp::DevBuf<uint16_t> hv, hvt;
-  bsp::DevBuf<int32_t> tcolptr, tperm, tbrow;
-  bsp::DevBuf<uint16_t> work;  // dense operand in panel layout
-  int64_t work_cap = 0;
+  bsp::DevBuf<int32_t> tcolptr, tperm, tbrow;  // blocks by (block column, block row)
+  bsp::DevBuf<uint16_t> work;   // pair panel: B (forward) or dC (dB)
+  bsp::DevBuf<uint16_t> work2;  // row panel of B (dvals)
+  int64_t work_cap = 0, work2_cap = 0;
 };
 
 namespace bsp {
@@ -135,16 +139,17 @@ __global__ void gather_i32(const int32_t* __restrict__ src, const int32_t* __res
   if (k < n) dst[k] = src[perm[k]];
 }
 
-// Panel layout of a dense R x N operand (row-major, N = 16*NT): for every
-// 16-row slab o and 16-column tile t the MFMA B-operand fragment in lane
-// order, P[((o*NT + t)*64 + l)*4 + e] = src[16o + 4(l/16) + e][16t + l%16]
-// (rows >= R read as zero), so a wave's fragment load is 512 contiguous bytes.
+// Pair panel of a dense R x N operand (row-major, N = 16*NT, NT even): for
+// every 16-row slab o and pair of 16-column tiles (2p, 2p+1), each lane's
+// two B-operand fragments side by side,
+//   P[((o*NT/2 + p)*64 + l)*8 + 4h + e] = src[16o + 4(l/16) + e][16(2p+h) + l%16]
+// (rows >= R read as zero), so one wave load of 16 bytes per lane is 1 KB
+// contiguous and feeds two MFMAs.
 template <int NT>
-__global__ __launch_bounds__(kTB) void panelize(const uint16_t* __restrict__ src, int64_t R,
-                                                uint16_t* __restrict__ dst, int64_t first) {
+__device__ __forceinline__ void pair_panel_slab(const uint16_t* __restrict__ src, int64_t R,
+                                                uint16_t* __restrict__ dst, int64_t o) {
   constexpr int N = NT * 16;
   __shared__ uint16_t t[16][N + 2];
-  const int64_t o = first + blockIdx.x;
   for (int idx = threadIdx.x; idx < 16 * N / 8; idx += kTB) {
     const int row = idx / (N / 8), c8 = (idx % (N / 8)) * 8;
     const int64_t r = o * 16 + row;
@@ -158,107 +163,150 @@ __global__ __launch_bounds__(kTB) void panelize(const uint16_t* __restrict__ src
     }
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < NT * 32; idx += kTB) {
+  for (int idx = threadIdx.x; idx < NT / 2 * kWave; idx += kTB) {
+    const int p = idx / kWave, l = idx % kWave, i = l % 16, kg = l / 16;
     uint32_t w[4];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int f = 2 * idx + h, tt = f / 64, l = f % 64, i = l % 16, kg = l / 16;
-      w[2 * h] = (uint32_t)t[4 * kg][tt * 16 + i] | ((uint32_t)t[4 * kg + 1][tt * 16 + i] << 16);
-      w[2 * h + 1] = (uint32_t)t[4 * kg + 2][tt * 16 + i] | ((uint32_t)t[4 * kg + 3][tt * 16 + i] << 16);
+      const int c = (2 * p + h) * 16 + i;
+      w[2 * h] = (uint32_t)t[4 * kg][c] | ((uint32_t)t[4 * kg + 1][c] << 16);
+      w[2 * h + 1] = (uint32_t)t[4 * kg + 2][c] | ((uint32_t)t[4 * kg + 3][c] << 16);
     }
-    *reinterpret_cast<uint4*>(dst + (o * NT * 64 + 2 * idx) * 4) = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(dst + (o * (NT / 2) * kWave + idx) * 8) =
+        make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
+template <int NT>
+__global__ __launch_bounds__(kTB) void panelize(const uint16_t* __restrict__ src, int64_t R,
+                                                uint16_t* __restrict__ dst, int64_t first) {
+  pair_panel_slab<NT>(src, R, dst, first + blockIdx.x);
+}
+
+// Row panel of a dense R x N operand, the A operand of the dvals product
+// (its rows against the block's K = N): for every 16-row slab o and K step s
+// (S = N/32 of them), lane l's 8 values
+//   Q[((o*S + s)*64 + l)*8 + e] = src[16o + l%16][32s + 8(l/16) + e]
+// (rows >= R read as zero): a wave's load is again 1 KB contiguous, where the
+// row-major operand gives 16 half-used lines per load.
+template <int NT>
+__device__ __forceinline__ void row_panel_slab(const uint16_t* __restrict__ src, int64_t R,
+                                               uint16_t* __restrict__ dst, int64_t o) {
+  constexpr int N = NT * 16, S = N / 32;
+  for (int idx = threadIdx.x; idx < S * kWave; idx += kTB) {
+    const int s = idx / kWave, l = idx % kWave;
+    const int64_t r = o * 16 + l % 16;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (r < R) v = *reinterpret_cast<const uint4*>(src + r * N + 32 * s + 8 * (l / 16));
+    *reinterpret_cast<uint4*>(dst + (o * S * kWave + idx) * 8) = v;
+  }
+}
+
+// The backward's two panels in one launch: workgroups [0, bslabs) write the
+// row panel of B (dvals), the next cslabs the pair panel of dC (dB).
+template <int NT>
+__global__ __launch_bounds__(kTB) void bwd_panels(const uint16_t* __restrict__ B, int64_t K,
+                                                  int64_t bslabs, uint16_t* __restrict__ Bq,
+                                                  const uint16_t* __restrict__ dC, int64_t M,
+                                                  uint16_t* __restrict__ dCp, int64_t first) {
+  const int64_t g = first + blockIdx.x;
+  if (g < bslabs)
+    row_panel_slab<NT>(B, K, Bq, g);
+  else
+    pair_panel_slab<NT>(dC, M, dCp, g - bslabs);
+}
+
 // forward (TRANS = false) and dB (TRANS = true). One workgroup per block
-// row (block column for dB) with NW waves; wave w owns column tiles
-// [w*TPW, (w+1)*TPW) across all of the row's blocks, so its accumulators are
-// final (no cross-wave reduction) and blocks are added in index order. The
-// row's blocks pass through LDS in chunks of 8*NW: their indices and scales
-// first, then their A fragments in lane order (each block read from HBM once
-// per workgroup rather than once per wave); each wave then streams its panel
-// fragments, U blocks in flight.
+// row (block column for dB, whose blocks `perm` lists) of WC*WB waves: wave
+// (wc, wb) owns column tiles [wc*TPW, (wc+1)*TPW) of the output and blocks
+// wb, wb + WB, ... of the row. Each block costs the wave one 8-byte A load,
+// TPW/2 16-byte panel loads and TPW MFMAs, whose results are scaled into the
+// FP32 sums. Block, slab and scale of 64 blocks at a time are loaded one per
+// lane and broadcast with readlane. The WB partial tiles are added through
+// LDS in wave order. At most 128 registers (four waves per SIMD): one
+// workgroup per block row then keeps every block row of a 16384-row matrix
+// resident at once.
 template <int NT>
 struct RowsShape {
-  static constexpr int NW = NT < 8 ? NT : 8;  // waves
-  static constexpr int TPW = NT / NW;         // 16-column tiles per wave
-  static constexpr int CH = 8 * NW;           // blocks per LDS chunk
+  static constexpr int TPW = NT < 8 ? NT : 8;  // 16-column tiles per wave
+  static constexpr int WC = NT / TPW;          // column groups
+  static constexpr int WB = 4;                 // block strides
+  static constexpr int NW = WC * WB;           // waves
 };
 
 template <int NT, bool BF, bool TRANS>
-__global__ __launch_bounds__(512) void bspmm_rows(const int32_t* __restrict__ ptr,
-                                                  const int32_t* __restrict__ perm,
-                                                  const int32_t* __restrict__ other,
-                                                  const uint16_t* __restrict__ blocks,
-                                                  const float* __restrict__ scale,
-                                                  const uint16_t* __restrict__ panel,
-                                                  float* __restrict__ out, int64_t out_rows,
-                                                  int64_t nouter, int64_t first) {
-  constexpr int NW = RowsShape<NT>::NW, TPW = RowsShape<NT>::TPW, CH = RowsShape<NT>::CH;
-  constexpr int U = 16 / TPW;
-  __shared__ u2 sa[CH][kWave];
-  __shared__ int32_t sb[CH], so[CH];
-  __shared__ float ss[CH];
+__global__ __launch_bounds__(RowsShape<NT>::NW * kWave, 4) void bspmm_rows(
+    const int32_t* __restrict__ ptr, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ other, const uint16_t* __restrict__ blocks,
+    const float* __restrict__ scale, const uint16_t* __restrict__ panel, float* __restrict__ out,
+    int64_t out_rows, int64_t nouter, int64_t first) {
+  constexpr int TPW = RowsShape<NT>::TPW, WC = RowsShape<NT>::WC, WB = RowsShape<NT>::WB;
+  constexpr int NP = TPW / 2;  // panel pairs per wave
+  __shared__ f4 red[WB - 1][WC][TPW][kWave];
   const int64_t r = first + blockIdx.x;
   if (r >= nouter) return;
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
+  const int wc = w % WC, wb = w / WC;
   const int i = lane & 15, kg = lane >> 4;
-  const uint16_t* pw = panel + ((int64_t)w * TPW * 64 + lane) * 4;
+  const uint16_t* pw = panel + ((int64_t)wc * NP * kWave + lane) * 8;
   f4 acc[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-  const int k0 = ptr[r], k1 = ptr[r + 1];
-  for (int c0 = k0; c0 < k1; c0 += CH) {
-    const int cn = min(CH, k1 - c0);
-    if (tid < cn) {
-      const int b = TRANS ? perm[c0 + tid] : c0 + tid;
-      sb[tid] = b;
-      so[tid] = other[c0 + tid];
-      ss[tid] = scale[b];
+  const int k0 = ptr[r], nb = ptr[r + 1] - k0;
+  const int cnt = nb > wb ? (nb - wb + WB - 1) / WB : 0;  // this wave's blocks
+  for (int base = 0; base < cnt; base += kWave) {
+    const int cn = min(kWave, cnt - base);
+    int myb = 0, myo = 0;
+    float mys = 0.f;
+    if (lane < cn) {
+      const int k = k0 + wb + WB * (base + lane);
+      myb = TRANS ? perm[k] : k;
+      myo = other[k];
+      mys = scale[myb];
     }
-    __syncthreads();
-    // unconditional loads (slots past cn repeat the last block and are never
-    // read), so all CH/NW of a wave's loads are in flight together
-    u2 stage[CH / NW];
+    for (int m = 0; m < cn; ++m) {
+      const int b = __builtin_amdgcn_readlane(myb, m);
+      const int64_t o = __builtin_amdgcn_readlane(myo, m);
+      const float sc =
+          __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mys), m));
+      const u2 a = *reinterpret_cast<const u2*>(blocks + (int64_t)b * 256 + i * 16 + 4 * kg);
+      u4 bo[NP];
 #pragma unroll
-    for (int q = 0; q < CH / NW; ++q)
-      stage[q] = *reinterpret_cast<const u2*>(blocks + (int64_t)sb[min(w + q * NW, cn - 1)] * 256 +
-                                              i * 16 + 4 * kg);
+      for (int p = 0; p < NP; ++p)
+        bo[p] = *reinterpret_cast<const u4*>(pw + (o * (NT / 2) + p) * (kWave * 8));
+      const f4 z = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < CH / NW; ++q) sa[w + q * NW][lane] = stage[q];
-    __syncthreads();
-    for (int j = 0; j < cn; j += U) {
-      u2 bo[U][TPW];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t o = so[min(j + u, cn - 1)];
-#pragma unroll
-        for (int t = 0; t < TPW; ++t)
-          bo[u][t] = *reinterpret_cast<const u2*>(pw + (o * NT + t) * 256);
+      for (int p = 0; p < NP; ++p) {
+        acc[2 * p] += sc * mma16<BF>(a, u2{bo[p].x, bo[p].y}, z);
+        acc[2 * p + 1] += sc * mma16<BF>(a, u2{bo[p].z, bo[p].w}, z);
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (j + u < cn) {
-          const u2 a = sa[j + u][lane];
-          const float sc = ss[j + u];
-#pragma unroll
-          for (int t = 0; t < TPW; ++t) acc[t] += sc * mma16<BF>(a, bo[u][t], f4{0.f, 0.f, 0.f, 0.f});
-        }
     }
-    __syncthreads();  // the next chunk overwrites the LDS arrays
   }
+  if (wb > 0)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) red[wb - 1][wc][t][lane] = acc[t];
+  __syncthreads();
+  if (wb != 0) return;
+#pragma unroll
+  for (int q = 0; q < WB - 1; ++q)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] += red[q][wc][t][lane];
   constexpr int N = NT * 16;
 #pragma unroll
   for (int t = 0; t < TPW; ++t)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int64_t row = r * 16 + 4 * kg + v;
-      if (row < out_rows) out[row * N + (w * TPW + t) * 16 + i] = acc[t][v];
+      if (row < out_rows) out[row * N + (wc * TPW + t) * 16 + i] = acc[t][v];
     }
 }
 
-// dvals: dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T (K = N), one
-// workgroup of kDvWaves waves per block row, each wave every kDvWaves-th block.
+// dvals: dA_b = dC[16 rows of br, :] * B[16 rows of bc, :]^T (K = N) on the
+// K = 32 MFMA with B's rows (from the row panel) as the A operand and dC's
+// rows as the B operand, so lane (i, kg) ends with dA_b[i][4kg .. 4kg+3] and
+// a block's 256 results leave in one contiguous 1 KB store. One workgroup of
+// kDvWaves waves per block row, wave w taking blocks w, w + kDvWaves, ...;
+// block columns of 64 blocks at a time loaded one per lane.
 constexpr int kDvWaves = 4;
 
 template <int NT, bool BF>
@@ -266,8 +314,8 @@ __global__ __launch_bounds__(kDvWaves * 64) void bspmm_dvals(const int32_t* __re
                                                              const int32_t* __restrict__ bcol,
                                                              const uint16_t* __restrict__ dC,
                                                              int64_t M,
-                                                             const uint16_t* __restrict__ B,
-                                                             int64_t K, float* __restrict__ dvals,
+                                                             const uint16_t* __restrict__ Bq,
+                                                             float* __restrict__ dvals,
                                                              int64_t brows, int64_t first) {
   constexpr int N = NT * 16, S = N / 32;
   const int64_t br = first + blockIdx.x;
@@ -276,37 +324,31 @@ __global__ __launch_bounds__(kDvWaves * 64) void bspmm_dvals(const int32_t* __re
   const int i = lane & 15, kg = lane >> 4;
   u4 a[S];
   const int64_t arow = br * 16 + i;
-  // clamped unconditional loads, rows past M (K) zeroed afterwards: no branch
-  // splits a batch of loads
+  // clamped unconditional loads, rows past M zeroed afterwards: no branch
+  // splits the batch of loads
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const u4 v = *reinterpret_cast<const u4*>(dC + min(arow, M - 1) * N + 32 * s + 8 * kg);
     a[s] = arow < M ? v : u4{0, 0, 0, 0};
   }
-  // two blocks per step (b and b + kDvWaves), their B loads in flight together
-  const int e = browptr[br + 1];
-  for (int b = browptr[br] + w; b < e; b += 2 * kDvWaves) {
-    const int b2 = b + kDvWaves;
-    const bool two = b2 < e;
-    const int64_t brow[2] = {(int64_t)bcol[b] * 16 + i, (int64_t)bcol[two ? b2 : b] * 16 + i};
-    u4 bo[2][S];
+  const int k0 = browptr[br], nb = browptr[br + 1] - k0;
+  const int cnt = nb > w ? (nb - w + kDvWaves - 1) / kDvWaves : 0;  // this wave's blocks
+  for (int base = 0; base < cnt; base += kWave) {
+    const int cn = min(kWave, cnt - base);
+    int myo = 0;
+    if (lane < cn) myo = bcol[k0 + w + kDvWaves * (base + lane)];
+    for (int m = 0; m < cn; ++m) {
+      const int64_t o = __builtin_amdgcn_readlane(myo, m);
+      u4 bo[S];
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+      for (int s = 0; s < S; ++s)
+        bo[s] = *reinterpret_cast<const u4*>(Bq + ((o * S + s) * kWave + lane) * 8);
+      f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const u4 v = *reinterpret_cast<const u4*>(B + min(brow[h], K - 1) * N + 32 * s + 8 * kg);
-        bo[h][s] = brow[h] < K ? v : u4{0, 0, 0, 0};
-      }
-    f4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int s = 0; s < S; ++s) acc[h] = mma32<BF>(a[s], bo[h][s], acc[h]);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) dvals[(int64_t)b * 256 + (4 * kg + v) * 16 + i] = acc[0][v];
-    if (two)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) dvals[(int64_t)b2 * 256 + (4 * kg + v) * 16 + i] = acc[1][v];
+      for (int s = 0; s < S; ++s) acc = mma32<BF>(bo[s], a[s], acc);
+      const int64_t b = k0 + w + kDvWaves * (base + m);
+      *reinterpret_cast<f4*>(dvals + b * 256 + i * 16 + 4 * kg) = acc;
+    }
   }
 }
 
@@ -340,13 +382,13 @@ static bsp_status launch_rows(bsp_bspmm_s* h, int64_t N, const uint16_t* panel, 
 }
 
 template <bool BF>
-static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, const uint16_t* B,
+static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, const uint16_t* Bq,
                                float* dvals, hipStream_t s) {
   if (h->brows == 0) return BSP_OK;
 #define DV_CASE(NT)                                                                             \
   case NT:                                                                                      \
     in_slices(h->brows, kDvWaves * 64, [&](int64_t f, int64_t c) {                             \
-      bspmm_dvals<NT, BF><<<c, kDvWaves * 64, 0, s>>>(h->browptr, h->bcol, dC, h->M, B, h->K,  \
+      bspmm_dvals<NT, BF><<<c, kDvWaves * 64, 0, s>>>(h->browptr, h->bcol, dC, h->M, Bq,       \
                                                       dvals, h->brows, f);                      \
     });                                                                                         \
     break;
@@ -363,27 +405,58 @@ static bsp_status launch_dvals(bsp_bspmm_s* h, int64_t N, const uint16_t* dC, co
   return BSP_OK;
 }
 
-// The dense operand (R x N) in panel layout, nslab 16-row slabs, in h->work.
-static bsp_status paneled(bsp_bspmm_s* h, const uint16_t* src, int64_t R, int64_t nslab,
-                          int64_t N, hipStream_t s) {
-  if (nslab * 16 * N > h->work_cap) {
-    BSP_HIP(hipStreamSynchronize(s));
-    BSP_TRY(h->work.alloc(nslab * 16 * N));
-    h->work_cap = nslab * 16 * N;
+// Grows a panel buffer (the first call of a product, outside any capture).
+static bsp_status ensure(DevBuf<uint16_t>& buf, int64_t& cap, int64_t need, hipStream_t s) {
+  if (need <= cap) return BSP_OK;
+  BSP_HIP(hipStreamSynchronize(s));
+  BSP_TRY(buf.alloc(need));
+  cap = need;
+  return BSP_OK;
+}
+
+#define BSPMM_NT_SWITCH(N, CASE)                                                    \
+  switch ((N) / 16) {                                                               \
+    CASE(2)                                                                         \
+    CASE(4)                                                                         \
+    CASE(8)                                                                         \
+    CASE(16)                                                                        \
+    default:                                                                        \
+      return fail(BSP_ERR_UNSUPPORTED, "block SpMM needs N in {32, 64, 128, 256}"); \
   }
-  if (nslab == 0) return BSP_OK;
-  auto go = [&](auto kernel) {
-    in_slices(nslab, kTB, [&](int64_t f, int64_t c) {
-      kernel<<<c, kTB, 0, s>>>(src, R, h->work.get(), f);
-    });
-  };
-  switch (N / 16) {
-    case 2: go(panelize<2>); break;
-    case 4: go(panelize<4>); break;
-    case 8: go(panelize<8>); break;
-    case 16: go(panelize<16>); break;
-    default: return fail(BSP_ERR_UNSUPPORTED, "block SpMM needs N in {32, 64, 128, 256}");
-  }
+
+// B (K x N) in pair-panel layout in h->work, for the forward.
+static bsp_status fwd_panel(bsp_bspmm_s* h, const uint16_t* B, int64_t N, hipStream_t s) {
+  BSP_TRY(ensure(h->work, h->work_cap, h->bcols * 16 * N, s));
+  if (h->bcols == 0) return BSP_OK;
+#define PANEL_CASE(NT)                                                                  \
+  case NT:                                                                              \
+    in_slices(h->bcols, kTB, [&](int64_t f, int64_t c) {                                \
+      panelize<NT><<<c, kTB, 0, s>>>(B, h->K, h->work.get(), f);                        \
+    });                                                                                 \
+    break;
+  BSPMM_NT_SWITCH(N, PANEL_CASE)
+#undef PANEL_CASE
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+// The backward's panels in one launch: B's row panel in h->work2 (when dvals
+// is wanted) and dC's pair panel in h->work (when dB is).
+static bsp_status bwd_panel(bsp_bspmm_s* h, const uint16_t* B, const uint16_t* dC, int64_t N,
+                            bool want_dvals, bool want_dB, hipStream_t s) {
+  const int64_t bslabs = want_dvals ? h->bcols : 0, cslabs = want_dB ? h->brows : 0;
+  if (want_dvals) BSP_TRY(ensure(h->work2, h->work2_cap, h->bcols * 16 * N, s));
+  if (want_dB) BSP_TRY(ensure(h->work, h->work_cap, h->brows * 16 * N, s));
+  if (bslabs + cslabs == 0) return BSP_OK;
+#define BWD_PANEL_CASE(NT)                                                              \
+  case NT:                                                                              \
+    in_slices(bslabs + cslabs, kTB, [&](int64_t f, int64_t c) {                         \
+      bwd_panels<NT><<<c, kTB, 0, s>>>(B, h->K, bslabs, h->work2.get(), dC, h->M,       \
+                                       h->work.get(), f);                               \
+    });                                                                                 \
+    break;
+  BSPMM_NT_SWITCH(N, BWD_PANEL_CASE)
+#undef BWD_PANEL_CASE
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }
@@ -469,8 +542,8 @@ bsp_status bsp_bspmm_fwd(bsp_bspmm h, const void* B, int64_t N, float* Cout, bsp
   BSP_REQUIRE(h && B && Cout, "null argument");
   hipStream_t st = (hipStream_t)s;
   BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256, "block SpMM needs N in {32, 64, 128, 256}");
-  // B in panel layout: each block column's 16 x N slab in MFMA operand order
-  BSP_TRY(paneled(h, static_cast<const uint16_t*>(B), h->K, h->bcols, N, st));
+  // B in pair-panel layout: each block column's 16 x N slab in MFMA operand order
+  BSP_TRY(fwd_panel(h, static_cast<const uint16_t*>(B), N, st));
   if (h->ht == BSP_BF16) return launch_rows<true, false>(h, N, h->work.get(), Cout, h->M, st);
   return launch_rows<false, false>(h, N, h->work.get(), Cout, h->M, st);
 }
@@ -480,16 +553,18 @@ bsp_status bsp_bspmm_bwd(bsp_bspmm h, const void* B, const void* dC, int64_t N, 
   BSP_REQUIRE(h && B && dC, "null argument");
   hipStream_t st = (hipStream_t)s;
   const bool bf = h->ht == BSP_BF16;
+  BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256, "block SpMM needs N in {32, 64, 128, 256}");
+  // B's rows in row-panel layout (the A operand of each block's dvals
+  // product) and dC in pair-panel layout (the dB product's B operand)
+  BSP_TRY(bwd_panel(h, static_cast<const uint16_t*>(B), static_cast<const uint16_t*>(dC), N,
+                    dvals != nullptr, dB != nullptr, st));
   if (dvals) {
     if (bf)
-      BSP_TRY(launch_dvals<true>(h, N, (const uint16_t*)dC, (const uint16_t*)B, dvals, st));
+      BSP_TRY(launch_dvals<true>(h, N, (const uint16_t*)dC, h->work2.get(), dvals, st));
     else
-      BSP_TRY(launch_dvals<false>(h, N, (const uint16_t*)dC, (const uint16_t*)B, dvals, st));
+      BSP_TRY(launch_dvals<false>(h, N, (const uint16_t*)dC, h->work2.get(), dvals, st));
   }
   if (dB) {
-    BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256,
-                "block SpMM needs N in {32, 64, 128, 256}");
-    BSP_TRY(paneled(h, static_cast<const uint16_t*>(dC), h->M, h->brows, N, st));
     if (bf)
       BSP_TRY((launch_rows<true, true>(h, N, h->work.get(), dB, h->K, st)));
     else

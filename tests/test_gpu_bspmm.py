@@ -83,12 +83,19 @@ def _quantized_dense(A, g, o, half):
 
 @pytest.mark.parametrize("half", ["float16", "bfloat16"])
 @pytest.mark.parametrize("N", [32, 64, 128, 256])
-@pytest.mark.parametrize("shape", ["blocks", "ragged"])
+@pytest.mark.parametrize("shape", ["blocks", "ragged", "wide"])
 def test_fwd_bwd_small(oracle, bsp_host, half, N, shape):
     import torch
     rng = np.random.default_rng(N)
     if shape == "blocks":
         A = bsp_host.blocksparse(24, 20, 16, 90, seed=1)
+    elif shape == "wide":
+        # two full block rows of 300 blocks (more than 64 per wave, so block
+        # lists are read in several 64-block batches), block scales spread
+        # over nine powers of two
+        A = bsp_host.blocksparse(2, 300, 16, 600, seed=3)
+        rows = np.repeat(np.arange(A.nrows), np.diff(A.rowptr))
+        A.val *= np.exp2((A.col // 16 + 5 * (rows // 16)) % 9 - 4.0)
     else:  # sizes not multiples of 16, scattered entries
         A = bsp_host.powerlaw(100, 8.0, 1.0, 40, seed=2, ncols=90)
     bsr, S = _build(A, half)
