@@ -625,7 +625,9 @@ def gather_probe():
     with the column-index stream, then with the value stream too, beside the
     SpMV launch itself (kernel + partial sums), and the same number of
     random gathers from the same 2 MB slices with no stream at all
-    (bsp_gather_probe: the L2's gather rate, the ceiling of this layout).
+    (bsp_gather_probe: the L2's gather rate, the ceiling of this layout), and
+    whole-line 1 KB wave reads of as many 64-byte sectors from the same
+    slices (bsp_line_probe: the L2-to-CU rate for full lines).
     Two rotated copies of the matrix, as in the headline leg; median of 20
     event-timed launches each. Prints one JSON line (microseconds)."""
     import ctypes as C
@@ -656,7 +658,20 @@ def gather_probe():
         lib().bsp_gather_probe(ptr(xd), A.ncols, 4, A.nnz, ptr(gsink), gsink.numel(), None,
                                stream_handle())
 
-    runs = {"gathers_only": ceiling,
+    # whole 1 KB lines from the same slices, as many sectors as the gathers
+    # move (one 64-byte sector per 8-byte gather): the L2-to-CU line rate
+    line_bytes = A.nnz * 64
+    lp = C.c_int64()
+    lib().bsp_line_probe(None, A.ncols, 4, line_bytes, None, 0, C.byref(lp), None)
+    lsink = torch.empty(lp.value, dtype=torch.float64, device="cuda")
+    line_bytes = lp.value * 256 * 32 * 16  # whole workgroups
+
+    def lines(i):
+        lib().bsp_line_probe(ptr(xd), A.ncols, 4, line_bytes, ptr(lsink), lsink.numel(), None,
+                             stream_handle())
+
+    runs = {"line_reads": lines,
+            "gathers_only": ceiling,
             "gathers_col_stream": lambda i: probe(i, 0),
             "gathers_col_val_stream": lambda i: probe(i, 1),
             "spmv": lambda i: ops[i % 2](xd, y)}
@@ -675,7 +690,12 @@ def gather_probe():
             ts.append(e0.elapsed_time(e1) * 1e3)
         us = float(np.median(ts))
         out[name + "_us"] = round(us, 2)
-        out[name + "_Greq_per_s"] = round(A.nnz / (us * 1e-6) / 1e9, 1)
+        if name == "line_reads":
+            out["line_read_bytes"] = line_bytes
+            out["line_read_TBs"] = round(line_bytes / (us * 1e-6) / 1e12, 2)
+            out["line_read_Gsectors_per_s"] = round(line_bytes / 64 / (us * 1e-6) / 1e9, 1)
+        else:
+            out[name + "_Greq_per_s"] = round(A.nnz / (us * 1e-6) / 1e9, 1)
     print(json.dumps(out), flush=True)
 
 

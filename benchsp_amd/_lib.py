@@ -195,6 +195,7 @@ _SIGS: dict[str, tuple] = {
     "bsp_csr_check": (None, [C.POINTER(CsrDesc), vp, C.POINTER(i32)]),
     "bsp_csr_plan_probe": (None, [vp, C.c_int, vp, vp, i64, i64p, vp]),
     "bsp_gather_probe": (None, [vp, i64, C.c_int32, i64, vp, i64, i64p, vp]),
+    "bsp_line_probe": (None, [vp, i64, C.c_int32, i64, vp, i64, i64p, vp]),
     "bsp_csr_spmv": (None, [vp, vp, vp, vp]),
     "bsp_csr_plan_info": (None, [vp, i64p, i64p]),
     "bsp_plan_destroy": (None, [vp]),

@@ -167,6 +167,12 @@ bsp_status bsp_gather_probe(const double* x, int64_t n, int32_t slices, int64_t 
   return gather_probe(x, n, slices, gathers, sink, sink_len, grid, (hipStream_t)s);
 }
 
+bsp_status bsp_line_probe(const double* x, int64_t n, int32_t slices, int64_t nbytes,
+                          double* sink, int64_t sink_len, int64_t* grid, bsp_stream s) {
+  BSP_REQUIRE(x || !sink, "null argument");
+  return line_probe(x, n, slices, nbytes, sink, sink_len, grid, (hipStream_t)s);
+}
+
 bsp_status bsp_csr_check(const bsp_csr* A, bsp_stream s, int32_t* sorted) {
   BSP_TRY(check_csr_pattern(A));
   if (sorted) *sorted = 1;

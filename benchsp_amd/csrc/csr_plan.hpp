@@ -54,4 +54,6 @@ bsp_status csr_plan_probe(const CsrPlan* p, int with_values, const double* x, do
                           int64_t sink_len, int64_t* grid, hipStream_t s);
 bsp_status gather_probe(const double* x, int64_t n, int nslices, int64_t gathers, double* sink,
                         int64_t sink_len, int64_t* grid, hipStream_t s);
+bsp_status line_probe(const double* x, int64_t n, int nslices, int64_t nbytes, double* sink,
+                      int64_t sink_len, int64_t* grid, hipStream_t s);
 }  // namespace bsp

@@ -178,6 +178,35 @@ __global__ __launch_bounds__(kWG) void probe_random_gather(const double* __restr
   if (threadIdx.x == 0) sink[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Whole-line reads from the same slices: each wave reads kProbePer 1 KB
+// runs (16 bytes per lane, 64 consecutive lanes) at xorshift-chosen
+// 1 KB-aligned positions of its slice. Against probe_random_gather, which
+// moves one 64-byte sector per 8-byte gather, this is the rate at which
+// the L2 feeds the CUs full lines.
+__global__ __launch_bounds__(kWG) void probe_line_read(const double* __restrict__ x, int nslices,
+                                                       int slice_log2,
+                                                       double* __restrict__ sink) {
+  __shared__ double red[kWG / kWave];
+  const int s = (blockIdx.x % 8) % nslices;
+  const double* xs = x + ((int64_t)s << slice_log2);
+  const uint32_t mask = (1u << (slice_log2 - 7)) - 1;  // 1 KB runs of 128 doubles
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t h = (uint32_t)(blockIdx.x * (kWG / kWave) + threadIdx.x / kWave) * 2654435761u + 12345u;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < kProbePer; ++k) {
+    h ^= h << 13;
+    h ^= h >> 17;
+    h ^= h << 5;
+    const dbl2v v = *reinterpret_cast<const dbl2v*>(xs + ((int64_t)(h & mask) << 7) + 2 * lane);
+    acc += v.x + v.y;
+  }
+  acc = group_sum<kWave>(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sink[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void sum_partials(const double* __restrict__ part, int nseg, int64_t n,
                              double* __restrict__ y) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -780,6 +809,26 @@ bsp_status gather_probe(const double* x, int64_t n, int nslices, int64_t gathers
   if (!sink || grid == 0) return BSP_OK;  // grid query
   BSP_REQUIRE(sink_len >= grid, "sink must hold one double per workgroup");
   probe_random_gather<<<grid, kWG, 0, s>>>(x, nslices, lg, sink);
+  BSP_HIP(hipGetLastError());
+  return BSP_OK;
+}
+
+bsp_status line_probe(const double* x, int64_t n, int nslices, int64_t nbytes, double* sink,
+                      int64_t sink_len, int64_t* grid_out, hipStream_t s) {
+  BSP_REQUIRE(nslices == 1 || nslices == 2 || nslices == 4 || nslices == 8,
+              "slices must be 1, 2, 4 or 8");
+  BSP_REQUIRE(n > 0 && n % nslices == 0 && nbytes >= 0, "bad probe sizes");
+  const int64_t len = n / nslices;
+  int lg = 0;
+  while ((int64_t(1) << lg) < len) ++lg;
+  BSP_REQUIRE((int64_t(1) << lg) == len && lg >= 7 && lg <= 30,
+              "slice length must be a power of two of at least 128 doubles");
+  const int64_t grid = cdiv(cdiv(nbytes, int64_t(16) * kProbePer), kWG);
+  BSP_REQUIRE(grid * kWG < kDispatchLimit, "too many bytes for one launch");
+  if (grid_out) *grid_out = grid;
+  if (!sink || grid == 0) return BSP_OK;  // grid query
+  BSP_REQUIRE(sink_len >= grid, "sink must hold one double per workgroup");
+  probe_line_read<<<grid, kWG, 0, s>>>(x, nslices, lg, sink);
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

@@ -146,6 +146,15 @@ bsp_status bsp_csr_plan_probe(bsp_spmv_plan p, int with_values, const double* x,
  * workgroup goes to sink; sink == NULL returns the workgroup count only. */
 bsp_status bsp_gather_probe(const double* x, int64_t n, int32_t slices, int64_t gathers,
                             double* sink, int64_t sink_len, int64_t* grid, bsp_stream s);
+/* Measurement aid, no counterpart in the reference: about `nbytes` of
+ * whole-line reads from the same slices as bsp_gather_probe (slice length a
+ * power of two of at least 128 doubles): each wave reads 1 KB runs (16 bytes
+ * per lane) at random 1 KB-aligned positions of its slice, 32 per lane: the
+ * rate at which the L2 feeds the CUs full lines, against the 64-byte sector
+ * each 8-byte gather of bsp_gather_probe moves. One sum per workgroup goes
+ * to sink; sink == NULL returns the workgroup count only. */
+bsp_status bsp_line_probe(const double* x, int64_t n, int32_t slices, int64_t nbytes,
+                          double* sink, int64_t sink_len, int64_t* grid, bsp_stream s);
 /* y = A*x. x has ncols, y has nrows doubles (device). Stream-ordered. */
 bsp_status bsp_csr_spmv(bsp_spmv_plan p, const double* x, double* y, bsp_stream s);
 /* Number of workgroups one bsp_csr_spmv launch uses (its main kernel). */

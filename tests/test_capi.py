@@ -335,6 +335,19 @@ def test_gather_probe_grid_query_and_argument_checks():
         assert e.value.status == 1
 
 
+def test_line_probe_grid_query_and_argument_checks():
+    # 16 bytes per lane, 32 runs per lane, 256 lanes per workgroup; slices
+    # shorter than one 1 KB run are argument errors
+    from benchsp_amd._lib import BspError, lib
+    g = C.c_int64()
+    lib().bsp_line_probe(None, 1 << 20, 4, 256 << 20, None, 0, C.byref(g), None)
+    assert g.value == (256 << 20) // (16 * 32 * 256)
+    for n, slices in ((1 << 20, 3), (3 << 18, 4), (0, 1), (256, 4)):
+        with pytest.raises(BspError) as e:
+            lib().bsp_line_probe(None, n, slices, 1000, None, 0, C.byref(g), None)
+        assert e.value.status == 1
+
+
 def test_rocsparse_comparator_rejects_non_csr_algorithms():
     # rocsparse_spmv_alg codes other than the CSR ones (1 COO, 4 ELL, ...)
     # are argument errors, checked before any device work

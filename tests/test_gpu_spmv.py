@@ -299,6 +299,23 @@ def test_device_csr_check(rp, col, ncols, ok, sorted_):
         assert e.value.status == 1
 
 
+def test_line_probe_sums():
+    # x of ones: each workgroup's stored sum is its 256 threads x 32 runs x
+    # two doubles per 16-byte load
+    import ctypes as C
+    import torch
+    from benchsp_amd._lib import lib
+    from benchsp_amd.device import ptr, stream_handle
+    x = torch.ones(1 << 20, dtype=torch.float64, device="cuda")
+    g = C.c_int64()
+    lib().bsp_line_probe(None, x.numel(), 4, 16 << 20, None, 0, C.byref(g), None)
+    sink = torch.zeros(g.value, dtype=torch.float64, device="cuda")
+    lib().bsp_line_probe(ptr(x), x.numel(), 4, 16 << 20, ptr(sink), sink.numel(), None,
+                         stream_handle())
+    torch.cuda.synchronize()
+    assert torch.all(sink == 256 * 32 * 2)
+
+
 def test_gather_probe_sums():
     # x of ones: each workgroup's stored sum is its 256 threads x 32 gathers
     import ctypes as C
