@@ -170,7 +170,17 @@ __global__ __launch_bounds__(kTB) void bsr_valu_f64(
 // lanes 16r + 4a + t. Lane 16r + 4a (t = 0) contributes acc[a]; the four
 // lanes 16r + {0, 4, 8, 12} add up to y row r of the block row.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2v __attribute__((ext_vector_type(2)));
 
+//
+// Two products share one 16-byte operand load per lane: K index 2j + h
+// (j = kk >> 1, h = kk & 1) takes block a + 4j of a group of eight and, in
+// product m = 0, 1, its column 2h + m, so a lane's two A values (columns 2h,
+// 2h + 1 of row r) and its two x values are adjacent, and a wave's A load is
+// the group's eight blocks, 1 KB contiguous. The diagonal tile a then holds
+// blocks a and a + 4 summed. X16: x is 16-byte aligned (else its pairs are
+// two 8-byte loads).
+template <bool X16>
 __global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
@@ -178,29 +188,40 @@ __global__ __launch_bounds__(kTB) void bsr4_mfma_f64(
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t br = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (br >= brows) return;  // whole wave exits together
-  const int ai = lane & 15;       // A row / B column
-  const int kk = lane >> 4;       // k index (column within a block)
-  const int a = ai >> 2, r = ai & 3;
+  const int a = (lane & 15) >> 2, r = lane & 3;  // A row 4a + r / B column slot a
+  const int h = (lane >> 4) & 1, j = lane >> 5;  // k index 2j + h
   const int s0 = rowptr[br], e0 = rowptr[br + 1];
   dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-  // block column indices 64 at a time (one coalesced load), then four MFMA
-  // groups (16 blocks) per step with all their operand loads in flight
+  // block column indices 64 at a time (one coalesced load), then four
+  // groups of eight blocks per step with all their operand loads in flight
   for (int base = s0; base < e0; base += kWave) {
     const int nb = min(kWave, e0 - base);
     const int cid = lane < nb ? colidx[base + lane] : 0;
-    for (int k0 = 0; k0 < nb; k0 += 16) {
-      double av[4], bv[4];
+    for (int k0 = 0; k0 < nb; k0 += 32) {
+      dbl2v av[4], bv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int kb = k0 + 4 * u + a;
-        const int c4 = __shfl(cid, min(kb, kWave - 1), 64);
-        const int64_t c = (int64_t)c4 * 4 + kk;
+        const int kb = k0 + 8 * u + a + 4 * j;
+        const int64_t c0 = (int64_t)__shfl(cid, min(kb, kWave - 1), 64) * 4 + 2 * h;
         const bool ok = kb < nb;
-        av[u] = ok ? ld_stream(&bval[(int64_t)(base + kb) * 16 + r * 4 + kk]) : 0.0;
-        bv[u] = ok && c < ncols ? x[c] : 0.0;
+        av[u] = ok ? ld_stream(reinterpret_cast<const dbl2v*>(
+                         bval + (int64_t)(base + kb) * 16 + r * 4 + 2 * h))
+                   : dbl2v{0.0, 0.0};
+        if (ok && c0 + 1 < ncols) {
+          if (X16)
+            bv[u] = *reinterpret_cast<const dbl2v*>(x + c0);
+          else
+            bv[u] = dbl2v{x[c0], x[c0 + 1]};
+        } else {
+          bv[u] = dbl2v{ok && c0 < ncols ? x[c0] : 0.0, 0.0};
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+      for (int u = 0; u < 4; ++u)
+        if (k0 + 8 * u < nb) {
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].y, acc, 0, 0, 0);
+        }
     }
   }
   // lane 16r + 4a + t keeps acc[a]; sum over a (lane bits 2..3)
@@ -244,9 +265,17 @@ __device__ __forceinline__ void bal_store(const int4& d, int64_t w, int r, doubl
   }
 }
 
-// MFMA: the chunk's blocks four at a time as the 16x4 A operand of
-// v_mfma_f64_16x16x4f64 (layouts as in bsr4_mfma_f64), 32 blocks' operand
-// loads in flight per step.
+// MFMA: the chunk's blocks eight at a time, two v_mfma_f64_16x16x4f64 per
+// eight blocks sharing one 16-byte operand load per lane. A row 4a + r (lane
+// bits 0..3) is row r of block slot a; K index 2j + h (lane bits 4..5)
+// takes block a + 4j of the step and, in product m = 0, 1, its column
+// 2h + m. So a lane's two A values A_b[r][2h], A_b[r][2h+1] are adjacent (a
+// wave's A load is the step's eight blocks, 1 KB contiguous) and so are its
+// two x values x[4 col_b + 2h], x[4 col_b + 2h + 1]; the diagonal 4x4 tiles
+// of the accumulated result hold the rows of blocks a and a + 4 summed
+// (layouts otherwise as in bsr4_mfma_f64). 32 blocks' loads in flight per
+// step. X16: x is 16-byte aligned (else its pairs are two 8-byte loads).
+template <bool X16>
 __global__ __launch_bounds__(kTB) void bsr4_bal_mfma_f64(
     const int4* __restrict__ ctab, int64_t nwarps, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
@@ -255,27 +284,38 @@ __global__ __launch_bounds__(kTB) void bsr4_bal_mfma_f64(
   const int64_t w = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (w >= nwarps) return;  // whole wave exits together
   const int4 d = ctab[w];
-  const int ai = lane & 15, kk = lane >> 4, a = ai >> 2, r = ai & 3;
+  const int a = (lane & 15) >> 2, r = lane & 3, h = (lane >> 4) & 1, j = lane >> 5;
   const int cid = lane < d.z ? colidx[d.y + lane] : 0;
   dbl4 acc = {0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < d.z; k0 += 32) {
-    double av[8], bv[8];
+    dbl2v av[4], bv[4];
     // A operands first: they do not wait for the column indices
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int kb = k0 + 4 * u + a;
-      av[u] = kb < d.z ? ld_stream(&bval[(int64_t)(d.y + kb) * 16 + r * 4 + kk]) : 0.0;
+    for (int u = 0; u < 4; ++u) {
+      const int kb = k0 + 8 * u + a + 4 * j;
+      av[u] = kb < d.z ? ld_stream(reinterpret_cast<const dbl2v*>(
+                             bval + (int64_t)(d.y + kb) * 16 + r * 4 + 2 * h))
+                       : dbl2v{0.0, 0.0};
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int kb = k0 + 4 * u + a;
-      const int c4 = __shfl(cid, min(kb, kWave - 1), 64);
-      const int64_t col = (int64_t)c4 * 4 + kk;
-      bv[u] = kb < d.z && col < ncols ? x[col] : 0.0;
+    for (int u = 0; u < 4; ++u) {
+      const int kb = k0 + 8 * u + a + 4 * j;
+      const int64_t c0 = (int64_t)__shfl(cid, min(kb, kWave - 1), 64) * 4 + 2 * h;
+      if (kb < d.z && c0 + 1 < ncols) {
+        if (X16)
+          bv[u] = *reinterpret_cast<const dbl2v*>(x + c0);
+        else
+          bv[u] = dbl2v{x[c0], x[c0 + 1]};
+      } else {
+        bv[u] = dbl2v{kb < d.z && c0 < ncols ? x[c0] : 0.0, 0.0};
+      }
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (k0 + 4 * u < d.z) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+    for (int u = 0; u < 4; ++u)
+      if (k0 + 8 * u < d.z) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].y, acc, 0, 0, 0);
+      }
   }
   double t = a == 0 ? acc[0] : a == 1 ? acc[1] : a == 2 ? acc[2] : acc[3];
   t += __shfl_xor(t, 4, 64);
@@ -1164,9 +1204,12 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
     const bool b4 = v.tm == 4 && v.tk == 4, b8 = v.tm == 8 && v.tk == 8;
     if (!b4 && !b8) return fail(BSP_ERR_UNSUPPORTED, "MFMA BSR kernel needs 4x4 or 8x8 blocks");
     in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
-      if (b4)
-        bsr4_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
-                                                            v.brows, v.nrows, v.ncols, f);
+      if (b4 && ((uintptr_t)x & 15) == 0)
+        bsr4_mfma_f64<true><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
+                                                                  v.brows, v.nrows, v.ncols, f);
+      else if (b4)
+        bsr4_mfma_f64<false><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x,
+                                                                   y, v.brows, v.nrows, v.ncols, f);
       else
         bsr8_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
                                                             v.brows, v.nrows, v.ncols, f);
@@ -1176,8 +1219,12 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
       return fail(BSP_ERR_UNSUPPORTED, "balanced BSR kernels need 4x4 blocks");
     if (kernel == BSP_BSR_MFMA_BALANCED)  // a wave per chunk
       in_slices(v.warpnum, kWave, [&](int64_t f, int64_t c) {
-        bsr4_bal_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
-            h->ctab.get(), v.warpnum, v.colidx, v.val, x, y, h->part.get(), v.nrows, v.ncols, f);
+        if (((uintptr_t)x & 15) == 0)
+          bsr4_bal_mfma_f64<true><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+              h->ctab.get(), v.warpnum, v.colidx, v.val, x, y, h->part.get(), v.nrows, v.ncols, f);
+        else
+          bsr4_bal_mfma_f64<false><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+              h->ctab.get(), v.warpnum, v.colidx, v.val, x, y, h->part.get(), v.nrows, v.ncols, f);
       });
     else  // a wave per four chunks
       in_slices(cdiv(v.warpnum, 4), kWave, [&](int64_t f, int64_t c) {

@@ -101,6 +101,29 @@ def test_bsr_spmv(oracle, bsp_host, kernel, tile):
             assert torch.equal(y, y2), name
 
 
+@pytest.mark.parametrize("kernel", ["mfma", "mfma_bal"])
+def test_bsr4_mfma_x_not_16_byte_aligned(oracle, bsp_host, kernel):
+    # the 4x4 MFMA kernels load x in 16-byte pairs when x is 16-byte aligned
+    # and as two 8-byte loads otherwise: x as a view one double into a buffer
+    # (ragged last block column: 263 columns), both paths against the oracle
+    import torch
+    from benchsp_amd.device import Bsr, DeviceCsr
+    rng = np.random.default_rng(5)
+    A = _rand_csr(rng, rng.integers(0, 40, 301), 263)
+    x = bsp_host.uniform(A.ncols, 22)
+    B = Bsr(DeviceCsr.from_host(A), 4, 4)
+    buf = torch.zeros(A.ncols + 2, dtype=torch.float64, device="cuda")
+    ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
+    for off in (0, 1):
+        xd = buf[off:off + A.ncols]
+        xd.copy_(torch.from_numpy(x))
+        assert (xd.data_ptr() % 16 == 0) == (off == 0)
+        y = B(xd, kernel=_KERNELS[kernel])
+        torch.cuda.synchronize()
+        assert_spmv_close(y.cpu().numpy(), ref, abs_spmv(oracle, A, x),
+                          what=f"bsr4 {kernel} x offset {off}")
+
+
 def test_bsr4_builder_bit_exact_powerlaw_1m(oracle, bsp_host):
     # the 4x4 BSR of the power-law 2^20 matrix (about one entry per block)
     from benchsp_amd.device import Bsr, DeviceCsr
