@@ -538,8 +538,11 @@ bsp_status bsp_bspmm_view_get(bsp_bspmm h, const float** scale, const uint16_t**
   return BSP_OK;
 }
 
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 bsp_status bsp_bspmm_fwd(bsp_bspmm h, const void* B, int64_t N, float* Cout, bsp_stream s) {
   BSP_REQUIRE(h && B && Cout, "null argument");
+  BSP_REQUIRE(aligned16(B), "B must be 16-byte aligned");
   hipStream_t st = (hipStream_t)s;
   BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256, "block SpMM needs N in {32, 64, 128, 256}");
   // B in pair-panel layout: each block column's 16 x N slab in MFMA operand order
@@ -551,6 +554,8 @@ bsp_status bsp_bspmm_fwd(bsp_bspmm h, const void* B, int64_t N, float* Cout, bsp
 bsp_status bsp_bspmm_bwd(bsp_bspmm h, const void* B, const void* dC, int64_t N, float* dB,
                          float* dvals, bsp_stream s) {
   BSP_REQUIRE(h && B && dC, "null argument");
+  BSP_REQUIRE(aligned16(B) && aligned16(dC) && (!dvals || aligned16(dvals)),
+              "B, dC and dvals must be 16-byte aligned");
   hipStream_t st = (hipStream_t)s;
   const bool bf = h->ht == BSP_BF16;
   BSP_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256, "block SpMM needs N in {32, 64, 128, 256}");

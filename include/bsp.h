@@ -437,7 +437,9 @@ bsp_status bsp_spmm_plan_destroy(bsp_spmm_plan p);
  * Dense operands are row-major FP16/BF16 (the handle's type): B is K x N,
  * dC is M x N; outputs are FP32 row-major (C: M x N, dB: K x N) and dvals
  * (nblocks x 16 x 16, the gradient with respect to the unscaled block
- * values, block order of the BSR). N is 32, 64, 128 or 256. */
+ * values, block order of the BSR). N is 32, 64, 128 or 256. B, dC and
+ * dvals are 16-byte aligned (16-byte loads and stores; BSP_ERR_INVALID
+ * otherwise). */
 typedef struct bsp_bspmm_s* bsp_bspmm;
 /* A: a 16x16 BSR (bsp_bsr_build with tm = tk = 16) that must outlive h. */
 bsp_status bsp_bspmm_build(bsp_bsr A, bsp_dtype half_type, bsp_stream s, bsp_bspmm* out);

@@ -129,6 +129,34 @@ def test_fwd_bwd_small(oracle, bsp_host, half, N, shape):
     assert rel_fro(C, Cref[:A.nrows]) <= STATED_TOL
 
 
+def test_operands_not_16_byte_aligned_are_refused(bsp_host):
+    # 16-byte operand loads (B, dC) and dvals stores: a view one element into
+    # a buffer is an argument error, before any launch
+    import ctypes as C
+    import torch
+    from benchsp_amd._lib import BspError, lib
+    from benchsp_amd.device import ptr, stream_handle
+    A = bsp_host.blocksparse(4, 4, 16, 6, seed=1)
+    bsr, S = _build(A, "float16")
+    N = 32
+    Bb = torch.zeros(A.ncols * N + 8, dtype=torch.float16, device="cuda")
+    dCb = torch.zeros(A.nrows * N + 8, dtype=torch.float16, device="cuda")
+    out = torch.zeros(max(A.nrows, A.ncols) * N, dtype=torch.float32, device="cuda")
+    dvb = torch.zeros(6 * 256 + 4, dtype=torch.float32, device="cuda")
+    ok_B, bad_B = ptr(Bb), ptr(Bb[1:])
+    with pytest.raises(BspError) as e:
+        lib().bsp_bspmm_fwd(S._h, bad_B, N, ptr(out), stream_handle())
+    assert e.value.status == 1
+    for B_, dC_, dv_ in ((bad_B, ptr(dCb), ptr(dvb)), (ok_B, ptr(dCb[1:]), ptr(dvb)),
+                         (ok_B, ptr(dCb), ptr(dvb[1:]))):
+        with pytest.raises(BspError) as e:
+            lib().bsp_bspmm_bwd(S._h, B_, dC_, N, ptr(out), dv_, stream_handle())
+        assert e.value.status == 1
+    lib().bsp_bspmm_fwd(S._h, ok_B, N, ptr(out), stream_handle())
+    lib().bsp_bspmm_bwd(S._h, ok_B, ptr(dCb), N, ptr(out), ptr(dvb), stream_handle())
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("half", ["float16", "bfloat16"])
 def test_blocksparse_16k_fwd_bwd(oracle, bsp_host, half):
     import torch
