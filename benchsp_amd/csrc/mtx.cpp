@@ -116,6 +116,9 @@ static bsp_status mtx_read(const char* path, bsp_host_csr* out) {
   const long long N = std::strtoll(end, &end, 10);
   const long long L = std::strtoll(end, &end, 10);
   if (M < 0 || N < 0 || L < 0 || end == p) return fail(BSP_ERR_INVALID, "bad size line");
+  // symmetric and skew-symmetric storage mirror (i, j) to (j, i): square only
+  if ((symm || skew) && M != N)
+    return fail(BSP_ERR_INVALID, "symmetric storage needs a square matrix");
   if (!sizes_fit(M, N, symm || skew ? 2 * L : L))
     return fail(BSP_ERR_TOO_LARGE, "MatrixMarket sizes must fit int32 indices (at most 2^31 - 2^20)");
   // every entry takes at least four bytes of text ("1 1\n"): a header that

@@ -90,6 +90,10 @@ def test_coo_to_csr_order_and_errors():
     # an entry count the file cannot hold is refused before any array is sized
     "%%MatrixMarket matrix coordinate real symmetric\n2000000000 2000000000 1000000000\n"
     "1 1 1.0\n",
+    # symmetric storage mirrors (i, j) to (j, i): a non-square header is
+    # refused (the mirror of (1, 3) would be row 3 of a 2-row matrix)
+    "%%MatrixMarket matrix coordinate real symmetric\n2 3 1\n1 3 1.0\n",
+    "%%MatrixMarket matrix coordinate pattern skew-symmetric\n3 2 1\n3 1\n",
     "not a matrix market file\n"])
 def test_read_rejects(tmp_path, text):
     from benchsp_amd._lib import BspError
