@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -136,6 +137,14 @@ HostCsr* gen_stencil3d(int64_t nx, int64_t ny, int64_t nz, int points, uint64_t 
   return h;
 }
 
+// x rounded and clamped to [lo, hi] in floating point before the integer
+// conversion (a NaN or infinite x gives lo or hi, never an undefined cast).
+static int64_t clamp_count(double x, int64_t lo, int64_t hi) {
+  if (!(x >= (double)lo)) return lo;  // also NaN
+  if (x >= (double)hi) return hi;
+  return (int64_t)x;
+}
+
 // Random matrix with lognormal row lengths:
 //   L = min(cap, max(1, round(exp(mu + sigma*z)))), mu = ln(mean) - sigma^2/2,
 // z the first normal of the row stream; L distinct columns uniform in
@@ -149,8 +158,7 @@ HostCsr* gen_powerlaw(int64_t n, int64_t ncols, double mean, double sigma, int64
   std::vector<int64_t> len(n);
   parallel_rows(n, [&](int64_t r) {
     Rng g(row_seed(seed, r));
-    double L = std::nearbyint(std::exp(mu + sigma * g.normal()));
-    len[r] = std::max<int64_t>(1, std::min<int64_t>(cap, (int64_t)L));
+    len[r] = clamp_count(std::nearbyint(std::exp(mu + sigma * g.normal())), 1, cap);
   });
   scan_lengths(h, len);
   parallel_rows(n, [&](int64_t r) {
@@ -236,8 +244,7 @@ HostCsr* gen_graph(int64_t n, double avg_deg, double gamma, int symmetric, uint6
   double tot = cdf[n];
   std::vector<int64_t> k(n), off(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) {
-    int64_t ki = (int64_t)std::nearbyint(w[i] * 0.5);
-    k[i] = std::max<int64_t>(1, std::min<int64_t>(n / 2, ki));
+    k[i] = clamp_count(std::nearbyint(w[i] * 0.5), 1, std::max<int64_t>(1, n / 2));
     off[i + 1] = off[i] + k[i];
   }
   int64_t m = off[n];
@@ -279,6 +286,26 @@ HostCsr* gen_graph(int64_t n, double avg_deg, double gamma, int symmetric, uint6
   return h;
 }
 
+// Generators allocate host arrays sized by their arguments: an allocation
+// that fails is a status, not an exception through the C interface.
+template <class F>
+static bsp_status gen_call(bsp_host_csr* out, F&& make) {
+  try {
+    HostCsr* h = make();
+    if (!h) return BSP_ERR_INVALID;
+    *out = (bsp_host_csr)h;
+    return BSP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(BSP_ERR_ALLOC, "host allocation failed");
+  }
+}
+
+// every dimension below 2^31 before any product of them is formed
+static bool dims_ok(int64_t a, int64_t b, int64_t c = 1) {
+  const int64_t lim = int64_t(1) << 31;
+  return a > 0 && b > 0 && c > 0 && a < lim && b < lim && c < lim;
+}
+
 }  // namespace bsp
 
 using namespace bsp;
@@ -302,42 +329,38 @@ bsp_status bsp_gen_normal(int64_t n, uint64_t seed, double* out) {
 }
 
 bsp_status bsp_gen_stencil5(int64_t nx, int64_t ny, bsp_host_csr* out) {
-  if (nx <= 0 || ny <= 0 || !out || nx * ny >= (1ll << 31)) return BSP_ERR_INVALID;
-  *out = (bsp_host_csr)gen_stencil5(nx, ny);
-  return BSP_OK;
+  if (!out || !dims_ok(nx, ny) || nx * ny >= (1ll << 31)) return BSP_ERR_INVALID;
+  return gen_call(out, [&] { return gen_stencil5(nx, ny); });
 }
 
 bsp_status bsp_gen_stencil3d(int64_t nx, int64_t ny, int64_t nz, int points, uint64_t seed,
                              bsp_host_csr* out) {
-  if (nx <= 0 || ny <= 0 || nz <= 0 || !out || nx * ny * nz >= (1ll << 31)) return BSP_ERR_INVALID;
-  HostCsr* h = gen_stencil3d(nx, ny, nz, points, seed);
-  if (!h) return BSP_ERR_INVALID;
-  *out = (bsp_host_csr)h;
-  return BSP_OK;
+  if (!out || !dims_ok(nx, ny, nz) || nx * ny >= (1ll << 31) || nx * ny * nz >= (1ll << 31))
+    return BSP_ERR_INVALID;
+  return gen_call(out, [&] { return gen_stencil3d(nx, ny, nz, points, seed); });
 }
 
 bsp_status bsp_gen_powerlaw(int64_t n, int64_t ncols, double mean, double sigma, int64_t cap,
                             uint64_t seed, bsp_host_csr* out) {
-  if (n <= 0 || ncols <= 0 || mean <= 0 || cap <= 0 || !out || ncols >= (1ll << 31))
+  if (!out || !dims_ok(n, ncols) || !(mean > 0) || cap <= 0 || !std::isfinite(sigma) ||
+      !std::isfinite(mean))
     return BSP_ERR_INVALID;
-  *out = (bsp_host_csr)gen_powerlaw(n, ncols, mean, sigma, cap, seed);
-  return BSP_OK;
+  return gen_call(out, [&] { return gen_powerlaw(n, ncols, mean, sigma, cap, seed); });
 }
 
 bsp_status bsp_gen_blocksparse(int64_t nbr, int64_t nbc, int bs, int64_t nblocks, uint64_t seed,
                                bsp_host_csr* out) {
-  if (nbr <= 0 || nbc <= 0 || bs <= 0 || !out || nbc * bs >= (1ll << 31)) return BSP_ERR_INVALID;
-  HostCsr* h = gen_blocksparse(nbr, nbc, bs, nblocks, seed, nullptr);
-  if (!h) return BSP_ERR_INVALID;
-  *out = (bsp_host_csr)h;
-  return BSP_OK;
+  if (!out || !dims_ok(nbr, nbc, bs) || nbc * bs >= (1ll << 31) || nbr * bs >= (1ll << 31))
+    return BSP_ERR_INVALID;
+  return gen_call(out, [&] { return gen_blocksparse(nbr, nbc, bs, nblocks, seed, nullptr); });
 }
 
 bsp_status bsp_gen_graph(int64_t n, double avg_deg, double gamma, int symmetric, uint64_t seed,
                          bsp_host_csr* out) {
-  if (n <= 1 || avg_deg <= 0 || gamma <= 1.0 || !out || n >= (1ll << 31)) return BSP_ERR_INVALID;
-  *out = (bsp_host_csr)gen_graph(n, avg_deg, gamma, symmetric, seed);
-  return BSP_OK;
+  if (!out || n <= 1 || n >= (1ll << 31) || !(avg_deg > 0) || !std::isfinite(avg_deg) ||
+      !(gamma > 1.0) || !std::isfinite(gamma))
+    return BSP_ERR_INVALID;
+  return gen_call(out, [&] { return gen_graph(n, avg_deg, gamma, symmetric, seed); });
 }
 
 bsp_status bsp_host_csr_info(bsp_host_csr h, int64_t* nrows, int64_t* ncols, int64_t* nnz) {

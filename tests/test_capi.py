@@ -200,6 +200,34 @@ def test_generator_sizes(bsp_host):
     assert np.array_equal(np.unique(S.val), [-1.0, 6.0])
 
 
+def test_generators_reject_bad_arguments_and_clamp_row_lengths():
+    # non-finite distribution parameters and dimensions whose products would
+    # overflow are argument errors; an extreme but finite sigma or mean gives
+    # row lengths clamped to [1, cap] (never an undefined float-to-int cast)
+    from benchsp_amd._lib import BspError, lib
+    h = C.c_void_p()
+    inf, nan = float("inf"), float("nan")
+    bad = [("bsp_gen_powerlaw", (100, 50, 20.0, inf, 40, 1)),
+           ("bsp_gen_powerlaw", (100, 50, 20.0, nan, 40, 1)),
+           ("bsp_gen_stencil5", (1 << 40, 1 << 40)),
+           ("bsp_gen_stencil3d", (1 << 30, 1 << 30, 1 << 30, 7, 1)),
+           ("bsp_gen_blocksparse", (1 << 30, 1 << 26, 16, 10, 1)),
+           ("bsp_gen_graph", (100, inf, 2.1, 1, 1))]
+    for name, args in bad:
+        with pytest.raises(BspError) as e:
+            getattr(lib(), name)(*args, C.byref(h))
+        assert e.value.status == 1, name
+    for sigma, mean in ((50.0, 20.0), (1.0, 1e300)):
+        lib().bsp_gen_powerlaw(100, 50, mean, sigma, 40, 1, C.byref(h))
+        nr, nc, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().bsp_host_csr_info(h, C.byref(nr), C.byref(nc), C.byref(nz))
+        rp = np.zeros(101, np.int64)
+        lib().bsp_host_csr_copy(h, rp.ctypes.data_as(C.c_void_p), None, None)
+        lib().bsp_host_csr_free(h)
+        lens = np.diff(rp)
+        assert lens.min() >= 1 and lens.max() <= 40 and nz.value == rp[-1]
+
+
 def test_generator_determinism_and_rows(bsp_host):
     P1 = bsp_host.powerlaw(5000, 20.0, 1.2, 300, seed=9)
     P2 = bsp_host.powerlaw(5000, 20.0, 1.2, 300, seed=9)
