@@ -327,7 +327,11 @@ __global__ __launch_bounds__(kTB) void bsr4_bal_mfma_f64(
 // in four groups of four, group g takes blocks g, g+4, g+8, ... of the
 // chunk, lane r of a group row r of its block; only entries whose map bit
 // r*4+c is set are multiplied (a non-finite x entry in a padded column does
-// not reach y). Row sums over the groups by lane shuffles.
+// not reach y). Row sums over the groups by lane shuffles. X16: x is 16-byte
+// aligned, and a block's four x values are two 16-byte loads whenever the
+// block column lies inside x (else, and for an unaligned x, four masked
+// 8-byte loads).
+template <bool X16>
 __global__ __launch_bounds__(kTB) void bsr4_bal_valu_f64(
     const int4* __restrict__ ctab, int64_t nwarps, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const uint16_t* __restrict__ map,
@@ -357,10 +361,18 @@ __global__ __launch_bounds__(kTB) void bsr4_bal_valu_f64(
       const int64_t b = (int64_t)(d.y + i0 + g + 4 * i) * 16 + r * 4;
       v[i] = i0 + g + 4 * i < d.z ? ld_stream(reinterpret_cast<const dbl4*>(bval + b))
                                   : dbl4{0.0, 0.0, 0.0, 0.0};
+      const int64_t c0 = (int64_t)cb[i] * 4;
+      if (X16 && bits[i] && c0 + 3 < ncols) {
+        const dbl2v lo = *reinterpret_cast<const dbl2v*>(x + c0);
+        const dbl2v hi = *reinterpret_cast<const dbl2v*>(x + c0 + 2);
+        xv[i][0] = lo.x;
+        xv[i][1] = lo.y;
+        xv[i][2] = hi.x;
+        xv[i][3] = hi.y;
+      } else {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int64_t col = (int64_t)cb[i] * 4 + c;
-        xv[i][c] = ((bits[i] >> c) & 1u) && col < ncols ? x[col] : 0.0;
+        for (int c = 0; c < 4; ++c)
+          xv[i][c] = ((bits[i] >> c) & 1u) && c0 + c < ncols ? x[c0 + c] : 0.0;
       }
     }
 #pragma unroll
@@ -1228,9 +1240,14 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
       });
     else  // a wave per four chunks
       in_slices(cdiv(v.warpnum, 4), kWave, [&](int64_t f, int64_t c) {
-        bsr4_bal_valu_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
-            h->ctab.get(), v.warpnum, v.colidx, v.val, v.map, x, y, h->part.get(), v.nrows,
-            v.ncols, f);
+        if (((uintptr_t)x & 15) == 0)
+          bsr4_bal_valu_f64<true><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+              h->ctab.get(), v.warpnum, v.colidx, v.val, v.map, x, y, h->part.get(), v.nrows,
+              v.ncols, f);
+        else
+          bsr4_bal_valu_f64<false><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(
+              h->ctab.get(), v.warpnum, v.colidx, v.val, v.map, x, y, h->part.get(), v.nrows,
+              v.ncols, f);
       });
     if (h->nmulti > 0)
       bsr4_bal_combine<<<cdiv(h->nmulti * 4, kTB), kTB, 0, st>>>(

@@ -101,11 +101,12 @@ def test_bsr_spmv(oracle, bsp_host, kernel, tile):
             assert torch.equal(y, y2), name
 
 
-@pytest.mark.parametrize("kernel", ["mfma", "mfma_bal"])
-def test_bsr4_mfma_x_not_16_byte_aligned(oracle, bsp_host, kernel):
-    # the 4x4 MFMA kernels load x in 16-byte pairs when x is 16-byte aligned
-    # and as two 8-byte loads otherwise: x as a view one double into a buffer
-    # (ragged last block column: 263 columns), both paths against the oracle
+@pytest.mark.parametrize("kernel", ["mfma", "mfma_bal", "valu_bal"])
+def test_bsr4_x_not_16_byte_aligned(oracle, bsp_host, kernel):
+    # the 4x4 MFMA and balanced bitmap kernels load x in 16-byte pieces when
+    # x is 16-byte aligned and as 8-byte loads otherwise: x as a view one
+    # double into a buffer (ragged last block column: 263 columns), both
+    # paths against the oracle
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
     rng = np.random.default_rng(5)
