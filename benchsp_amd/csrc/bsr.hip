@@ -129,7 +129,11 @@ __global__ void row_len_sq_dev(const int32_t* __restrict__ rowptr, int64_t brows
 // ---- SpMV kernels -----------------------------------------------------------
 
 // Wave per block row; lane = slot*tm + r (slots = 64/tm blocks per step).
-// Partials of equal r are added in slot order through LDS.
+// Partials of equal r are added in slot order through LDS. TK > 0: the
+// block width known at compile time, a block row's TK values and x entries
+// loaded together before the sum (same order of additions as TK = 0, which
+// takes any width).
+template <int TK>
 __global__ __launch_bounds__(kTB) void bsr_valu_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
@@ -145,7 +149,19 @@ __global__ __launch_bounds__(kTB) void bsr_valu_f64(
       const int64_t c0 = (int64_t)colidx[k] * tk;
       const double* p = bval + (int64_t)k * tm * tk + r * tk;
       const int cmax = (int)min<int64_t>(tk, ncols - c0);
-      for (int c = 0; c < cmax; ++c) acc += ld_stream(&p[c]) * x[c0 + c];
+      if constexpr (TK > 0) {
+        double pv[TK], xv[TK];
+#pragma unroll
+        for (int c = 0; c < TK; ++c) {
+          pv[c] = c < cmax ? ld_stream(&p[c]) : 0.0;
+          xv[c] = c < cmax ? x[c0 + c] : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < TK; ++c)
+          if (c < cmax) acc += pv[c] * xv[c];
+      } else {
+        for (int c = 0; c < cmax; ++c) acc += ld_stream(&p[c]) * x[c0 + c];
+      }
     }
   }
   part[threadIdx.x] = acc;
@@ -1208,10 +1224,19 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
   if (kernel == BSP_BSR_AUTO) kernel = BSP_BSR_VALU;
   hipStream_t st = (hipStream_t)s;
   if (kernel == BSP_BSR_VALU) {
-    in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
-      bsr_valu_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows,
-                                                         v.nrows, v.ncols, v.tm, v.tk, f);
-    });
+    auto go = [&](auto kernel) {
+      in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
+        kernel<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y, v.brows,
+                                                     v.nrows, v.ncols, v.tm, v.tk, f);
+      });
+    };
+    switch (v.tk) {  // the widths the benchmarks use, unrolled; any other at run time
+      case 3: go(bsr_valu_f64<3>); break;
+      case 4: go(bsr_valu_f64<4>); break;
+      case 8: go(bsr_valu_f64<8>); break;
+      case 16: go(bsr_valu_f64<16>); break;
+      default: go(bsr_valu_f64<0>); break;
+    }
   } else if (kernel == BSP_BSR_MFMA) {
     const bool b4 = v.tm == 4 && v.tk == 4, b8 = v.tm == 8 && v.tk == 8;
     if (!b4 && !b8) return fail(BSP_ERR_UNSUPPORTED, "MFMA BSR kernel needs 4x4 or 8x8 blocks");
