@@ -83,12 +83,14 @@ def _quantized_dense(A, g, o, half):
 
 @pytest.mark.parametrize("half", ["float16", "bfloat16"])
 @pytest.mark.parametrize("N", [32, 64, 128, 256])
-@pytest.mark.parametrize("shape", ["blocks", "ragged", "wide"])
+@pytest.mark.parametrize("shape", ["blocks", "ragged", "wide", "empty"])
 def test_fwd_bwd_small(oracle, bsp_host, half, N, shape):
     import torch
     rng = np.random.default_rng(N)
     if shape == "blocks":
         A = bsp_host.blocksparse(24, 20, 16, 90, seed=1)
+    elif shape == "empty":  # no blocks at all: C, dB zero, dvals empty
+        A = bsp_host.blocksparse(3, 2, 16, 0, seed=1)
     elif shape == "wide":
         # two full block rows of 300 blocks (more than 64 per wave, so block
         # lists are read in several 64-block batches), block scales spread
