@@ -791,49 +791,91 @@ __global__ __launch_bounds__(kTB) void dbsr_spmv_f64(
 }
 
 // DBSR on v_mfma_f64_16x16x4f64 (the reference's DBSR kernels use
-// mma.m8n8k4 with a diagonal pick, src/amgT.cu:324-387): a wave per 4-row
-// group, four of its blocks per product. Lane (i, t) = (l%16, l/16) loads
-// slot t of row i%4 of block i/4 of the step (value and its column's x), as
-// A[i][t] and B[t][i], so the result's diagonal C[i][i] is block i/4's row
-// i%4 over its four slots (C layout as in bsr4_mfma_f64); y row r of the
-// group is the sum of C[4a+r][4a+r] over a.
+// mma.m8n8k4 with a diagonal pick, src/amgT.cu:324-387). Lane (i, t) =
+// (l%16, l/16) owns row i%4 of one block of the step: one 32-byte load of
+// its four values and one 16-byte load of their columns, and in product u
+// it supplies slot u (value and its column's x) as A[i][t] and B[t][i], so
+// the result's diagonal C[i][i] sums the row over the four blocks that lanes
+// (i, 0..3) own (C layout as in bsr4_mfma_f64: C[R][R] sits in acc[R/4] of
+// lane 16(R%4) + R). A wide group takes a wave, sixteen blocks per step
+// (lane (i, t) on block 4t + i/4), and its row r is the sum of C[4a+r][4a+r]
+// over a; two narrow groups (at most 8 blocks wide, the prefix of the
+// length-sorted groups) share a wave, A rows 8h..8h+7 for group h and lane
+// (i, t) on block 2t + (i/4)%2 of its group, so each group's row r is
+// C[8h+r][8h+r] + C[8h+4+r][8h+4+r].
+__device__ __forceinline__ void dbsr_mfma_step(const double* __restrict__ bval,
+                                               const int32_t* __restrict__ colidx,
+                                               const double* __restrict__ x, int blk, bool ok,
+                                               int r, dbl4& acc) {
+  typedef int i4v __attribute__((ext_vector_type(4)));
+  dbl4 av = {0.0, 0.0, 0.0, 0.0};
+  double xv[4] = {0.0, 0.0, 0.0, 0.0};
+  if (ok) {
+    const int64_t o = (int64_t)blk * 16 + r * 4;
+    av = ld_stream(reinterpret_cast<const dbl4*>(bval + o));
+    const i4v c = ld_stream(reinterpret_cast<const i4v*>(colidx + o));
+    xv[0] = x[c.x];
+    xv[1] = x[c.y];
+    xv[2] = x[c.z];
+    xv[3] = x[c.w];
+  }
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, xv[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, xv[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.z, xv[2], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.w, xv[3], acc, 0, 0, 0);
+}
+
+__device__ __forceinline__ double dbsr_mfma_diag(const dbl4& acc, int lane) {
+  const int R = lane & 15;
+  if ((lane >> 4) != (R & 3)) return 0.0;
+  return (R >> 2) == 0 ? acc[0] : (R >> 2) == 1 ? acc[1] : (R >> 2) == 2 ? acc[2] : acc[3];
+}
+
 __global__ __launch_bounds__(kTB) void dbsr_spmv_f64_mfma(
     const int32_t* __restrict__ blcptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const int32_t* __restrict__ rec, const double* __restrict__ x,
-    double* __restrict__ y, int64_t brows, int64_t nrows, int sorted_order, int64_t first) {
+    double* __restrict__ y, int64_t brows, int64_t nrows, int64_t narrow, int sorted_order,
+    int64_t first) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t g = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (g >= brows) return;  // whole wave exits together
-  const int i = lane & 15, t = lane >> 4;
-  const int off = (i >> 2) * 16 + (i & 3) * 4 + t;  // block i/4 of the step, row i%4, slot t
-  const int s0 = blcptr[g], e0 = blcptr[g + 1];
-  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = s0; k0 < e0; k0 += 16) {  // 16 blocks' loads in flight per step
-    double av[4], bv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int blk = k0 + 4 * u + (i >> 2);
-      const int64_t o = (int64_t)(k0 + 4 * u) * 16 + off;
-      const bool ok = blk < e0;
-      av[u] = ok ? ld_stream(&bval[o]) : 0.0;
-      bv[u] = ok ? x[ld_stream(&colidx[o])] : 0.0;
+  const int64_t w = first + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t npair = (narrow + 1) / 2;
+  const int i = lane & 15, t = lane >> 4, r = lane & 3, R = lane & 15;
+  auto store = [&](int64_t g, int rr, double v) {
+    const int64_t row = 4 * g + rr;
+    if (row < nrows) y[sorted_order ? row : rec[row]] = v;
+  };
+  auto wide = [&](int64_t g) {  // 16 blocks per step
+    const int s0 = blcptr[g], e0 = blcptr[g + 1];
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = s0; k0 < e0; k0 += 16) {
+      const int blk = k0 + 4 * t + (i >> 2);
+      dbsr_mfma_step(bval, colidx, x, blk, blk < e0, r, acc);
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + 4 * u < e0) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+    double d = dbsr_mfma_diag(acc, lane);
+    d += __shfl_xor(d, 4, 64);
+    d += __shfl_xor(d, 8, 64);
+    if (R == t) store(g, t, d);
+  };
+  if (w >= npair) {  // a wide group, one wave
+    const int64_t g = narrow + (w - npair);
+    if (g < brows) wide(g);
+    return;
   }
-  // C[R][R] sits in acc[R/4] of lane 16(R%4) + R; row r: lanes 16r + 4a + r
-  const int R = lane & 15;
-  double d = 0.0;
-  if ((lane >> 4) == (R & 3))
-    d = (R >> 2) == 0 ? acc[0] : (R >> 2) == 1 ? acc[1] : (R >> 2) == 2 ? acc[2] : acc[3];
-  d += __shfl_xor(d, 4, 64);
-  d += __shfl_xor(d, 8, 64);
-  const int r = lane >> 4;
-  if ((lane & 15) == r) {
-    const int64_t row = 4 * g + r;
-    if (row < nrows) y[sorted_order ? row : rec[row]] = d;
+  const int64_t g0 = 2 * w;
+  const bool pair = g0 + 1 < narrow && blcptr[g0 + 1] - blcptr[g0] <= 8 &&
+                    blcptr[g0 + 2] - blcptr[g0 + 1] <= 8;
+  if (pair) {
+    const int h = i >> 3;
+    const int s0 = blcptr[g0 + h], e0 = blcptr[g0 + h + 1];
+    const int blk = s0 + 2 * t + ((i >> 2) & 1);
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    dbsr_mfma_step(bval, colidx, x, blk, blk < e0, r, acc);
+    double d = dbsr_mfma_diag(acc, lane);
+    d += __shfl_xor(d, 4, 64);
+    if (R == t || R == 8 + t) store(g0 + (R >> 3), t, d);
+    return;
   }
+  for (int64_t g = g0; g < g0 + 2 && g < narrow; ++g) wide(g);
 }
 
 // f[g] = 1 when 4-row group g is at most 8 blocks wide
@@ -1315,15 +1357,16 @@ bsp_status bsp_dbsr_spmv_ex(bsp_dbsr h, int kernel, const double* x, double* y, 
   const bsp_dbsr_view& v = h->v;
   BSP_REQUIRE((v.ncols == 0 || x) && (v.nrows == 0 || y), "null x/y");
   if (v.brows == 0) return BSP_OK;
+  const int64_t waves = (h->narrow + 1) / 2 + (v.brows - h->narrow);
   if (kernel == BSP_DBSR_MFMA) {
-    in_slices(v.brows, kWave, [&](int64_t f, int64_t c) {
+    in_slices(waves, kWave, [&](int64_t f, int64_t c) {
       dbsr_spmv_f64_mfma<<<cdiv(c * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
-          v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, sorted_order, f);
+          v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, h->narrow, sorted_order,
+          f);
     });
     BSP_HIP(hipGetLastError());
     return BSP_OK;
   }
-  const int64_t waves = (h->narrow + 1) / 2 + (v.brows - h->narrow);
   in_slices(waves, kWave, [&](int64_t f, int64_t c) {
     dbsr_spmv_f64<<<cdiv(c * kWave, kTB), kTB, 0, (hipStream_t)s>>>(
         v.blcptr, v.colidx, v.val, v.recorder, x, y, v.brows, v.nrows, h->narrow, sorted_order, f);
