@@ -60,36 +60,43 @@ __device__ __forceinline__ double mfma_diag(const dbl4m& acc, int lane) {
 // as they are and supplies its own vectors.
 //
 // MF: long rows and row blocks on FP64 MFMA, as the reference's fused
-// kernel does on mma.m8n8k4 (src/dasp.cu:405-812): lane l loads one entry
-// (value as A[l%16][l/16], its gathered x as B[l/16][l%16]), so
-// v_mfma_f64_16x16x4f64's diagonal C[g][g] is the sum over the four entries
-// of A row g. A row block's 64-entry step stacks its two 8x4 slabs as A
-// rows 0..7 and 8..15 (row g of the block is C[g][g] + C[g+8][g+8]); a long
-// row's 64-entry window is 16 pieces of four entries (the row sum is the
-// trace). Short rows stay on VALU in both.
+// kernel does on mma.m8n8k4 (src/dasp.cu:405-812). Lane l = 16t + i loads
+// four consecutive entries (one 32-byte value and one 16-byte column load)
+// and, in product u, supplies entry u as A[i][t] with its gathered x as
+// B[t][i], so v_mfma_f64_16x16x4f64's diagonal C[i][i] sums the entries
+// lanes (i, 0..3) hold. A row block's step covers eight 32-entry slabs
+// (8 rows x 4 columns each): lane (i, t) takes row i%8 of slab 2t + i/8, so
+// row g of the block is C[g][g] + C[g+8][g+8]; a long row's step covers four
+// 64-entry windows, lane (i, t) entries 4i..4i+3 of window t (the row sum is
+// the trace). Short rows stay on VALU in both.
 template <bool MF = false>
 __device__ __forceinline__ void dasp_block(const DaspArgs& a, const double* __restrict__ x,
                                            double* __restrict__ y, int64_t bid) {
   __shared__ double red[kTB / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   if (MF && bid < a.wg_long) {
+    typedef int i4v __attribute__((ext_vector_type(4)));
     const int i = (int)bid;
     const int64_t base = (int64_t)a.lrpt[i] * 64;
     const int64_t nwin = a.lrpt[i + 1] - a.lrpt[i];  // 64-entry windows, zero padded
-    const int off = 4 * (lane & 15) + (lane >> 4);  // entry of this lane in a window
+    const int t = lane >> 4, off = 4 * (lane & 15);
     dbl4m acc = {0.0, 0.0, 0.0, 0.0};
-    int64_t wdw = tid / kWave;
-    for (; wdw + 4 < nwin; wdw += 8) {  // two windows' loads in flight
-      const int64_t p0 = base + wdw * 64 + off, p1 = p0 + 4 * 64;
-      const double v0 = ld_stream(&a.lv[p0]), v1 = ld_stream(&a.lv[p1]);
-      const double x0 = x[ld_stream(&a.lc[p0])], x1 = x[ld_stream(&a.lc[p1])];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, x0, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, x1, acc, 0, 0, 0);
-    }
-    if (wdw < nwin) {
-      const int64_t p = base + wdw * 64 + off;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ld_stream(&a.lv[p]), x[ld_stream(&a.lc[p])],
-                                                 acc, 0, 0, 0);
+    for (int64_t w0 = 4 * (tid / kWave); w0 < nwin; w0 += 16) {  // four windows per step
+      dbl4m v = {0.0, 0.0, 0.0, 0.0};
+      double xv[4] = {0.0, 0.0, 0.0, 0.0};
+      if (w0 + t < nwin) {
+        const int64_t p = base + (w0 + t) * 64 + off;
+        v = ld_stream(reinterpret_cast<const dbl4m*>(a.lv + p));
+        const i4v c = ld_stream(reinterpret_cast<const i4v*>(a.lc + p));
+        xv[0] = x[c.x];
+        xv[1] = x[c.y];
+        xv[2] = x[c.z];
+        xv[3] = x[c.w];
+      }
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, xv[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, xv[1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.z, xv[2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.w, xv[3], acc, 0, 0, 0);
     }
     const double v = group_sum<kWave>(mfma_diag(acc, lane));
     if (lane == 0) red[tid / kWave] = v;
@@ -115,22 +122,29 @@ __device__ __forceinline__ void dasp_block(const DaspArgs& a, const double* __re
     return;
   }
   if (MF && bid < a.wg_long + a.wg_reg) {
+    typedef int i4v __attribute__((ext_vector_type(4)));
     const int b = (int)((bid - a.wg_long) * (kTB / kWave) + tid / kWave);
     if (b >= a.blocknum) return;
     const int p0 = a.bptr[b], p1 = a.bptr[b + 1];
-    const int g = lane & 15;
-    const int off = (g >> 3) * 32 + (g & 7) * 4 + (lane >> 4);  // slab, row, column
+    const int g = lane & 15, ts = lane >> 4;
+    const int off = (2 * ts + (g >> 3)) * 32 + (g & 7) * 4;  // slab, row
     dbl4m acc = {0.0, 0.0, 0.0, 0.0};
-    for (int base = p0; base < p1; base += 2 * kWave) {  // two steps' loads in flight
-      double v[2], xv[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int p = base + u * kWave + off;
-        v[u] = p < p1 ? ld_stream(&a.rv[p]) : 0.0;
-        xv[u] = p < p1 ? x[ld_stream(&a.rc[p])] : 0.0;
+    for (int base = p0; base < p1; base += 4 * kWave) {  // eight slabs per step
+      const int p = base + off;
+      dbl4m v = {0.0, 0.0, 0.0, 0.0};
+      double xv[4] = {0.0, 0.0, 0.0, 0.0};
+      if (p < p1) {
+        v = ld_stream(reinterpret_cast<const dbl4m*>(a.rv + p));
+        const i4v c = ld_stream(reinterpret_cast<const i4v*>(a.rc + p));
+        xv[0] = x[c.x];
+        xv[1] = x[c.y];
+        xv[2] = x[c.z];
+        xv[3] = x[c.w];
       }
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v[0], xv[0], acc, 0, 0, 0);
-      if (base + kWave < p1) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v[1], xv[1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, xv[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, xv[1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.z, xv[2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v.w, xv[3], acc, 0, 0, 0);
     }
     // row g (0..7): C[g][g] on lane 16(g%4) + g plus C[g+8][g+8] eight lanes up
     const double d = mfma_diag(acc, lane);
