@@ -434,11 +434,15 @@ __global__ void bal_multi_scatter(const int32_t* __restrict__ flag, const int32_
 
 // 8x8 blocks on the same v_mfma_f64_16x16x4f64, two blocks per product:
 // A rows 8a..8a+7 are rows of block a (a = 0, 1) at four of its columns
-// (4h..4h+3, h = 0, 1: two products per pair), B column j is the matching x
-// segment of block j/8. The two 8x8 diagonal tiles of the 16x16 result then
-// hold A_a*x_a in every column: result row R, column R sits in acc[R/4] of
-// lane 16(R%4) + R (C layout above), and y row r of the block row is the sum
-// of result rows r and r+8, found in lanes L and L+8, L = 16(r%4) + r.
+// (columns 2k + h at K index k in product h = 0, 1: two products per pair),
+// B column j is the matching x segment of block j/8. The two 8x8 diagonal
+// tiles of the 16x16 result then hold A_a*x_a in every column: result row R,
+// column R sits in acc[R/4] of lane 16(R%4) + R (C layout above), and y row
+// r of the block row is the sum of result rows r and r+8, found in lanes L
+// and L+8, L = 16(r%4) + r. A lane's two A values (columns 2k, 2k + 1) and
+// its two x values are adjacent: one 16-byte load each (X16: x 16-byte
+// aligned, else two 8-byte loads).
+template <bool X16>
 __global__ __launch_bounds__(kTB) void bsr8_mfma_f64(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const double* __restrict__ bval, const double* __restrict__ x, double* __restrict__ y,
@@ -457,22 +461,30 @@ __global__ __launch_bounds__(kTB) void bsr8_mfma_f64(
     const int nb = min(kWave, e0 - base);
     const int cid = lane < nb ? colidx[base + lane] : 0;
     for (int k0 = 0; k0 < nb; k0 += 8) {
-      double av[8], bv[8];
+      dbl2v av[4], bv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int kb = k0 + 2 * u + a;
         const int c8 = __shfl(cid, min(kb, kWave - 1), 64);
         const bool ok = kb < nb;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int cb = 4 * h + kk;  // column within the block
-          const int64_t c = (int64_t)c8 * 8 + cb;
-          av[2 * u + h] = ok ? ld_stream(&bval[(int64_t)(base + kb) * 64 + r * 8 + cb]) : 0.0;
-          bv[2 * u + h] = ok && c < ncols ? x[c] : 0.0;
+        const int64_t c = (int64_t)c8 * 8 + 2 * kk;  // columns 2kk, 2kk + 1 of the block
+        av[u] = ok ? ld_stream(reinterpret_cast<const dbl2v*>(bval + (int64_t)(base + kb) * 64 +
+                                                              r * 8 + 2 * kk))
+                   : dbl2v{0.0, 0.0};
+        if (ok && c + 1 < ncols) {
+          if (X16)
+            bv[u] = *reinterpret_cast<const dbl2v*>(x + c);
+          else
+            bv[u] = dbl2v{x[c], x[c + 1]};
+        } else {
+          bv[u] = dbl2v{ok && c < ncols ? x[c] : 0.0, 0.0};
         }
       }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+      for (int u = 0; u < 4; ++u) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].y, acc, 0, 0, 0);
+      }
     }
   }
   // lane (J = lane%16, g = lane/16) holds result row g + 4v, column J in
@@ -1289,9 +1301,12 @@ bsp_status bsp_bsr_spmv(bsp_bsr h, int kernel, const double* x, double* y, bsp_s
       else if (b4)
         bsr4_mfma_f64<false><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x,
                                                                    y, v.brows, v.nrows, v.ncols, f);
+      else if (((uintptr_t)x & 15) == 0)
+        bsr8_mfma_f64<true><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
+                                                                  v.brows, v.nrows, v.ncols, f);
       else
-        bsr8_mfma_f64<<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x, y,
-                                                            v.brows, v.nrows, v.ncols, f);
+        bsr8_mfma_f64<false><<<cdiv(c * kWave, kTB), kTB, 0, st>>>(v.rowptr, v.colidx, v.val, x,
+                                                                   y, v.brows, v.nrows, v.ncols, f);
     });
   } else if (kernel == BSP_BSR_MFMA_BALANCED || kernel == BSP_BSR_VALU_BALANCED) {
     if (v.tm != 4 || v.tk != 4)

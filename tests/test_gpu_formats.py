@@ -101,18 +101,19 @@ def test_bsr_spmv(oracle, bsp_host, kernel, tile):
             assert torch.equal(y, y2), name
 
 
-@pytest.mark.parametrize("kernel", ["mfma", "mfma_bal", "valu_bal"])
-def test_bsr4_x_not_16_byte_aligned(oracle, bsp_host, kernel):
-    # the 4x4 MFMA and balanced bitmap kernels load x in 16-byte pieces when
-    # x is 16-byte aligned and as 8-byte loads otherwise: x as a view one
-    # double into a buffer (ragged last block column: 263 columns), both
-    # paths against the oracle
+@pytest.mark.parametrize("kernel,tile", [("mfma", 4), ("mfma_bal", 4), ("valu_bal", 4),
+                                         ("mfma", 8)])
+def test_bsr_x_not_16_byte_aligned(oracle, bsp_host, kernel, tile):
+    # the MFMA and balanced bitmap kernels load x in 16-byte pieces when x is
+    # 16-byte aligned and as 8-byte loads otherwise: x as a view one double
+    # into a buffer (ragged last block column: 263 columns), both paths
+    # against the oracle
     import torch
     from benchsp_amd.device import Bsr, DeviceCsr
     rng = np.random.default_rng(5)
     A = _rand_csr(rng, rng.integers(0, 40, 301), 263)
     x = bsp_host.uniform(A.ncols, 22)
-    B = Bsr(DeviceCsr.from_host(A), 4, 4)
+    B = Bsr(DeviceCsr.from_host(A), tile, tile)
     buf = torch.zeros(A.ncols + 2, dtype=torch.float64, device="cuda")
     ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
     for off in (0, 1):
@@ -122,7 +123,7 @@ def test_bsr4_x_not_16_byte_aligned(oracle, bsp_host, kernel):
         y = B(xd, kernel=_KERNELS[kernel])
         torch.cuda.synchronize()
         assert_spmv_close(y.cpu().numpy(), ref, abs_spmv(oracle, A, x),
-                          what=f"bsr4 {kernel} x offset {off}")
+                          what=f"bsr{tile} {kernel} x offset {off}")
 
 
 def test_bsr4_builder_bit_exact_powerlaw_1m(oracle, bsp_host):
