@@ -800,7 +800,7 @@ def roofline(per: dict, method: str) -> dict:
         adaptive = method == "adaptive" or (fmt == "csr" and m["layout"].get("chunk") == 2048)
         gbs = m["bytes"] / (m["us"] * 1e-6) / 1e9
         if fmt == "dia":
-            kernel, keys = "dia_spmv_f64", ("dia_spmv_f64<",)
+            kernel, keys = "dia_spmv2_f64", ("dia_spmv2_f64<",)
         elif adaptive:
             kernel = ("csr_adaptive_xcd_f64<2048> + sum_partials" if segmented else
                       "csr_adaptive_f64<2048>")

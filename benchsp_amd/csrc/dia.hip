@@ -91,15 +91,62 @@ __global__ __launch_bounds__(kTB) void dia_spmv_f64(DiaOffsets off,
   y[r] = s;
 }
 
+// The same sums two rows per lane (rows 2t, 2t + 1): one 16-byte value load
+// per diagonal, so a wave's load is 1 KB of one diagonal, and one 8-byte mask
+// load. Needs an even row count (each diagonal's run then starts 16-byte
+// aligned) and a 16-byte aligned y. Each row's arithmetic is the one-row
+// kernel's, so y is the same bits. 128-lane workgroups: 27-point 64^3
+// measured 11.1 us against 12.8 us for a row per lane (a read of the same
+// bytes alone: 11.0 us).
+constexpr int kTB2 = 128;
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+
 template <int K>
-void launch_dia(const bsp_dia_s* h, const double* x, double* y, int64_t grid, hipStream_t s) {
+__global__ __launch_bounds__(kTB2) void dia_spmv2_f64(DiaOffsets off,
+                                                      const double* __restrict__ val,
+                                                      const uint32_t* __restrict__ mask,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y, int32_t n,
+                                                      int32_t ncols) {
+#pragma clang fp contract(off)
+  const int32_t r = 2 * (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= n) return;
+  dbl2v a[K > 0 ? K : 1];
+  double b0[K > 0 ? K : 1], b1[K > 0 ? K : 1];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    a[k] = ld_stream(reinterpret_cast<const dbl2v*>(&val[(int64_t)k * n + r]));
+    const int32_t c = r + off.d[k];
+    b0[k] = x[min(max(c, 0), ncols - 1)];
+    b1[k] = x[min(max(c + 1, 0), ncols - 1)];
+  }
+  const u2v m = *reinterpret_cast<const u2v*>(&mask[r]);
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double t0 = s0 + a[k].x * b0[k];
+    s0 = ((m.x >> k) & 1u) ? t0 : s0;
+    const double t1 = s1 + a[k].y * b1[k];
+    s1 = ((m.y >> k) & 1u) ? t1 : s1;
+  }
+  *reinterpret_cast<dbl2v*>(&y[r]) = dbl2v{s0, s1};
+}
+
+template <int K>
+void launch_dia(const bsp_dia_s* h, const double* x, double* y, bool pairs, hipStream_t s) {
   if constexpr (K <= kDiaMax) {
     if (h->ndiag == K) {
-      dia_spmv_f64<K><<<grid, kTB, 0, s>>>(h->off, h->val.get(), h->mask.get(), x, y,
-                                           (int32_t)h->nrows, (int32_t)h->ncols);
+      const int64_t n = h->nrows;
+      if (pairs)
+        dia_spmv2_f64<K><<<cdiv(n / 2, kTB2), kTB2, 0, s>>>(h->off, h->val.get(), h->mask.get(),
+                                                             x, y, (int32_t)n, (int32_t)h->ncols);
+      else
+        dia_spmv_f64<K><<<cdiv(n, kTB), kTB, 0, s>>>(h->off, h->val.get(), h->mask.get(), x, y,
+                                                     (int32_t)n, (int32_t)h->ncols);
       return;
     }
-    launch_dia<K + 1>(h, x, y, grid, s);
+    launch_dia<K + 1>(h, x, y, pairs, s);
   }
 }
 
@@ -177,9 +224,8 @@ bsp_status bsp_dia_spmv(bsp_dia h, const double* x, double* y, bsp_stream stream
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = h->nrows;
   if (n == 0) return BSP_OK;
-  const int64_t grid = cdiv(n, kTB);
-  BSP_REQUIRE(grid * kTB < kDispatchLimit, "too many rows for one launch");
-  launch_dia<0>(h, x, y, grid, s);
+  BSP_REQUIRE(cdiv(n, kTB) * kTB < kDispatchLimit, "too many rows for one launch");
+  launch_dia<0>(h, x, y, n % 2 == 0 && ((uintptr_t)y & 15) == 0, s);
   BSP_HIP(hipGetLastError());
   return BSP_OK;
 }

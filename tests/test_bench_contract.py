@@ -30,7 +30,7 @@ def test_roofline_reports_the_dominant_launch():
     per["a"]["layout"] = {"format": "dia", "kernel": 0, "chunk": 0, "seg_cols": 0}
     per["b"]["layout"] = {"format": "csr", "kernel": 0, "chunk": 2048, "seg_cols": 0}
     r = bench.roofline(per, "tuned")
-    assert r["kernel"] == "dia_spmv_f64" and abs(r["achieved"] - 2000.0) < 1e-6
+    assert r["kernel"] == "dia_spmv2_f64" and abs(r["achieved"] - 2000.0) < 1e-6
     assert r["other_launches"]["b"]["kernel"].startswith("csr_adaptive_xcd_f64")
 
 

@@ -86,6 +86,21 @@ def test_dia_non_finite_x_only_where_referenced(oracle, bsp_host):
     _check(oracle, A, x, "non-finite x")
 
 
+def test_dia_y_not_16_byte_aligned(oracle, bsp_host):
+    # an even row count takes two rows per lane (16-byte y stores) only when
+    # y is 16-byte aligned; an 8-byte aligned y takes the row-per-lane kernel
+    import torch
+    from benchsp_amd.device import DeviceCsr, Dia
+    A = bsp_host.poisson5_256()
+    x = bsp_host.uniform(A.ncols, 6)
+    D = Dia(DeviceCsr.from_host(A))
+    ref = oracle.spmv_f64(A.rowptr, A.col, A.val, x)
+    buf = torch.full((A.nrows + 1,), 7.0, dtype=torch.float64, device="cuda")
+    y = D(torch.from_numpy(x).cuda(), buf[1:])
+    assert np.array_equal(y.cpu().numpy(), ref)
+    assert buf[0].item() == 7.0
+
+
 def test_dia_empty_and_degenerate(oracle):
     rng = np.random.default_rng(2)
     for name, A in (("no entries", Csr(37, 29, np.zeros(38, np.int64), np.zeros(0, np.int32),
