@@ -170,6 +170,23 @@ def test_adaptive_forced_segments(oracle, bsp_host, seg_cols, seg_mode):
                       what="stencil forced segments")
 
 
+def test_segment_sums_y_not_16_byte_aligned(bsp_host):
+    # the partial sums of an even row count are added two rows per lane
+    # (16-byte stores) only into a 16-byte aligned y; an 8-byte aligned y
+    # takes the row-per-lane launch, with the same bits
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    A = bsp_host.powerlaw(300_000, 20.0, 1.2, 5000, seed=1)
+    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=75_000)
+    assert op.segments() == (75_000, 4)
+    x = torch.from_numpy(bsp_host.uniform(A.ncols, 4)).cuda()
+    buf = torch.full((A.nrows + 1,), 7.0, dtype=torch.float64, device="cuda")
+    y = op(x)
+    y1 = op(x, buf[1:])
+    torch.cuda.synchronize()
+    assert torch.equal(y, y1) and buf[0].item() == 7.0
+
+
 @pytest.mark.parametrize("chunk", [1024, 2048])
 def test_adaptive_variants(oracle, bsp_host, chunk):
     import torch
