@@ -801,6 +801,8 @@ def roofline(per: dict, method: str) -> dict:
         gbs = m["bytes"] / (m["us"] * 1e-6) / 1e9
         if fmt == "dia":
             kernel, keys = "dia_spmv2_f64", ("dia_spmv2_f64<",)
+        elif fmt == "csr" and m["layout"].get("kernel") == 1 and segmented:
+            kernel, keys = "csr_sorted_xcd_f64 + sum_partials", ("csr_sorted_xcd", "sum_partials")
         elif adaptive:
             kernel = ("csr_adaptive_xcd_f64<2048> + sum_partials" if segmented else
                       "csr_adaptive_f64<2048>")

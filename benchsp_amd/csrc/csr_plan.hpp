@@ -38,6 +38,8 @@ struct bsp_spmv_plan_s {
   int64_t xcd_grid = 0;
   bsp::DevBuf<double> part;      // per-segment partial y (xcd_mode)
   bsp::DevBuf<char> segtab;      // device table of segment descriptors (xcd_mode)
+  bool sorted = false;           // sorted-block segments (seg_mode 2)
+  bsp::DevBuf<uint16_t> slot;    // sorted blocks: each entry's row-order position in its block
   bsp::DevBuf<int16_t> col16;    // 16-bit column offsets from each block's first row
   bool narrow = false;           // unsegmented plan reading col16
 };

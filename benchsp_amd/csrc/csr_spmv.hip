@@ -13,17 +13,20 @@ namespace bsp {
 // ---------------------------------------------------------------------------
 // Adaptive kernel (block routine in csr_adaptive.hpp).
 
+// (rshift > 0: a row weighs 2^-rshift units instead of one, so a window holds
+// more rows. Rows longer than `longrow` take a block of their own, so a
+// block has fewer than chunk + longrow entries.)
 __global__ void csr_block_flags(const int32_t* __restrict__ rowptr, int64_t nrows, int chunk,
-                                int32_t* __restrict__ flags) {
+                                int rshift, int longrow, int32_t* __restrict__ flags) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
-  const int longrow = chunk / 2;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int f = 1;
   if (r > 0) {
     int32_t sp = rowptr[r - 1];
     bool lng = (e - s) > longrow, lngp = (s - sp) > longrow;
-    int64_t key = ((int64_t)s + r) / chunk, keyp = ((int64_t)sp + r - 1) / chunk;
+    int64_t key = ((int64_t)s + (r >> rshift)) / chunk,
+            keyp = ((int64_t)sp + ((r - 1) >> rshift)) / chunk;
     f = (key != keyp) || lng || lngp;
   }
   flags[r] = f;
@@ -99,6 +102,7 @@ struct SegDesc {
   const int2* blk;
   int64_t nblk;
   double* out;
+  const uint16_t* slot;  // sorted-block plans: each entry's row-order position in its block
 };
 
 template <int CHUNK>
@@ -112,6 +116,158 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_xcd_f64(const SegDesc* __res
   if (b >= d.nblk) return;
   // partial sums stay cached (sum_partials reads them back at once)
   adaptive_block<CHUNK, false, false>(d.rowptr, d.col, d.val, x, d.out, d.blk, b);
+}
+
+// Sorted-block segmented plans (seg_mode 2). A segment's blocks are cut on
+// rowptr[r] + r/4 with windows of kSortChunk entries; a row longer than
+// kSortLong has a block of its own, so a block holds fewer than
+// kSortChunk + kSortLong = kSortCap entries. Within a block the entries are
+// stored in column order, each with a 2-byte slot: its position in row
+// order. A wave's gathers then fall on nearby x entries of the XCD's slice
+// and share cache lines, while the products go back to row order through
+// LDS (prod[slot]) for the row sums. Every row is summed in row order from
+// LDS, by one lane (up to kSortShort entries: the CPU loop's order) or by a
+// wave (strided lanes, then a fixed butterfly). 512-lane workgroups, two per
+// CU: the launch asks for kSortPad bytes of LDS it does not use, which
+// keeps a third from being resident (power-law 2^20, four 2 MB segments:
+// 140.9 us with two per CU, 159.4 us with three, 197 us with one; blocks of
+// 4,096 / 5,120 / 8,192 entries: 162 / 157 / 152-155 us; the row-order
+// plan 155 us).
+constexpr int kSortCap = 6144;
+constexpr int kSortWG = 512;
+constexpr int kSortLong = 1024;
+constexpr int kSortChunk = kSortCap - kSortLong;
+constexpr int kSortRowShift = 2;
+constexpr int kSortShort = 32;
+constexpr int kSortRows = 2;  // row bounds a lane loads with the entries
+constexpr int kSortPad = 24 * 1024;
+
+__global__ __launch_bounds__(kSortWG) void csr_sorted_xcd_f64(const SegDesc* __restrict__ segs,
+                                                              int nseg,
+                                                              const double* __restrict__ x) {
+#pragma clang fp contract(off)
+  __shared__ double prod[kSortCap];
+  __shared__ int longrow[kSortCap / (kSortShort + 1) + 1];
+  __shared__ int nlong;
+  __shared__ double red[kSortWG / kWave];
+  const int xcd = blockIdx.x % 8;
+  const int s = xcd % nseg, q = xcd / nseg, reps = 8 / nseg;
+  const SegDesc d = segs[s];
+  const int64_t b = (int64_t)(blockIdx.x / 8) * reps + q;
+  if (b >= d.nblk) return;
+  const int2 s0 = d.blk[b], s1 = d.blk[b + 1];
+  const int r0 = s0.x, p0 = s0.y, r1 = s1.x, p1 = s1.y;
+  const int tid = threadIdx.x, nnz = p1 - p0;
+  if (r1 - r0 == 1 && nnz > kSortLong) {
+    // one long row (its entries are in column order, which is row order):
+    // the workgroup strides over it
+    double a0 = 0.0, a1 = 0.0;
+    int i = p0 + tid;
+    for (; i + kSortWG < p1; i += 2 * kSortWG) {
+      a0 += ld_stream(&d.val[i]) * x[ld_stream(&d.col[i])];
+      a1 += ld_stream(&d.val[i + kSortWG]) * x[ld_stream(&d.col[i + kSortWG])];
+    }
+    if (i < p1) a0 += ld_stream(&d.val[i]) * x[ld_stream(&d.col[i])];
+    const double v = group_sum<kWave>(a0 + a1);
+    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kSortWG / kWave; ++w) t += red[w];
+      d.out[r0] = t;
+    }
+    return;
+  }
+  constexpr int IT = kSortCap / kSortWG;
+  int c[IT], sl[IT];
+  double v[IT], xv[IT];
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int i = tid + k * kSortWG;
+    if (i < nnz) {
+      c[k] = ld_stream(&d.col[p0 + i]);
+      v[k] = ld_stream(&d.val[p0 + i]);
+      sl[k] = ld_stream(&d.slot[p0 + i]);
+    }
+  }
+  // the first rows' bounds load with the entries, not after the barrier
+  int rb[kSortRows], re_[kSortRows];
+#pragma unroll
+  for (int k = 0; k < kSortRows; ++k) {
+    const int r = r0 + tid + k * kSortWG;
+    if (r < r1) {
+      rb[k] = d.rowptr[r];
+      re_[k] = d.rowptr[r + 1];
+    }
+  }
+  if (tid == 0) nlong = 0;
+#pragma unroll
+  for (int k = 0; k < IT; ++k)
+    if (tid + k * kSortWG < nnz) xv[k] = x[c[k]];
+#pragma unroll
+  for (int k = 0; k < IT; ++k)
+    if (tid + k * kSortWG < nnz) prod[sl[k]] = v[k] * xv[k];
+  __syncthreads();
+  auto row = [&](int r, int rs, int re) {
+    if (re - rs > kSortShort) {
+      longrow[atomicAdd(&nlong, 1)] = r;  // list order varies; each row's sum does not
+      return;
+    }
+    double a = 0.0;
+    for (int j = rs; j < re; ++j) a += prod[j];
+    d.out[r] = a;
+  };
+#pragma unroll
+  for (int k = 0; k < kSortRows; ++k) {
+    const int r = r0 + tid + k * kSortWG;
+    if (r < r1) row(r, rb[k] - p0, re_[k] - p0);
+  }
+  for (int r = r0 + tid + kSortRows * kSortWG; r < r1; r += kSortWG)
+    row(r, d.rowptr[r] - p0, d.rowptr[r + 1] - p0);
+  __syncthreads();
+  const int w = tid / kWave, lane = tid & (kWave - 1);
+  for (int t = w; t < nlong; t += kSortWG / kWave) {
+    const int r = longrow[t];
+    const int rs = d.rowptr[r] - p0, re = d.rowptr[r + 1] - p0;
+    double a = 0.0;
+    for (int j = rs + lane; j < re; j += kWave) a += prod[j];
+    a = group_sum<kWave>(a);
+    if (lane == 0) d.out[r] = a;
+  }
+}
+
+// Sort keys of a segment's entries, a workgroup per block: (global block id,
+// column) and the entry's global index; gstart[block] = its first entry.
+__global__ __launch_bounds__(kWG) void sorted_keys(const int2* __restrict__ blk,
+                                                   const int32_t* __restrict__ col,
+                                                   int64_t block0, int64_t entry0,
+                                                   int64_t* __restrict__ keys,
+                                                   int32_t* __restrict__ idx,
+                                                   int64_t* __restrict__ gstart, int64_t first) {
+  const int64_t b = first + blockIdx.x;
+  const int p0 = blk[b].y, p1 = blk[b + 1].y;
+  if (threadIdx.x == 0) gstart[block0 + b] = entry0 + p0;
+  for (int i = p0 + (int)threadIdx.x; i < p1; i += kWG) {
+    keys[entry0 + i] = ((block0 + b) << 32) | (uint32_t)col[i];
+    idx[entry0 + i] = (int32_t)(entry0 + i);
+  }
+}
+
+// Entries in their sorted order, with their slots.
+__global__ __launch_bounds__(kWG) void sorted_apply(const int64_t* __restrict__ keys,
+                                                    const int32_t* __restrict__ idx,
+                                                    const int64_t* __restrict__ gstart,
+                                                    const int32_t* __restrict__ col,
+                                                    const double* __restrict__ val, int64_t n,
+                                                    int32_t* __restrict__ ocol,
+                                                    double* __restrict__ oval,
+                                                    uint16_t* __restrict__ slot) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t j = idx[i];
+  ocol[i] = col[j];
+  oval[i] = val[j];
+  slot[i] = (uint16_t)(j - gstart[keys[i] >> 32]);
 }
 
 // Data movement of the column-segmented launch without its arithmetic: on
@@ -550,10 +706,12 @@ __global__ __launch_bounds__(kWG) void seg_scatter(const int32_t* __restrict__ r
   }
 }
 
-static bsp_status adaptive_partition(CsrSegment& g, int64_t n, int chunk, hipStream_t s) {
+static bsp_status adaptive_partition(CsrSegment& g, int64_t n, int chunk, hipStream_t s,
+                                     int rshift = 0, int longrow = 0) {
   DevBuf<int32_t> flags;
   BSP_TRY(flags.alloc(n));
-  csr_block_flags<<<cdiv(n, kWG), kWG, 0, s>>>(g.rowptr, n, chunk, flags.get());
+  csr_block_flags<<<cdiv(n, kWG), kWG, 0, s>>>(g.rowptr, n, chunk, rshift,
+                                               longrow > 0 ? longrow : chunk / 2, flags.get());
   BSP_HIP(hipGetLastError());
   DevBuf<int32_t> starts;
   BSP_TRY(stable_partition(flags.get(), n, starts, &g.nblk, s));
@@ -562,6 +720,56 @@ static bsp_status adaptive_partition(CsrSegment& g, int64_t n, int chunk, hipStr
                                                   reinterpret_cast<int2*>(g.blk.get()));
   BSP_HIP(hipGetLastError());
   BSP_HIP(hipStreamSynchronize(s));
+  return BSP_OK;
+}
+
+// Sorted blocks: every block's entries reordered by column (stable), with
+// each entry's row-order position in its block kept as a 2-byte slot. One
+// radix sort over all segments on (global block, column) keys.
+static bsp_status sort_blocks(CsrPlan* p, hipStream_t s) {
+  const int64_t total = p->A.nnz;
+  if (total == 0) return BSP_OK;
+  int64_t nblk = 0;
+  for (const auto& g : p->seg) nblk += g.nblk;
+  DevBuf<int64_t> keys, keys2, gstart;
+  DevBuf<int32_t> idx, idx2;
+  BSP_TRY(keys.alloc(total));
+  BSP_TRY(keys2.alloc(total));
+  BSP_TRY(idx.alloc(total));
+  BSP_TRY(idx2.alloc(total));
+  BSP_TRY(gstart.alloc(nblk));
+  int64_t block0 = 0;
+  for (const auto& g : p->seg) {
+    const int64_t entry0 = g.col - p->seg_col.get();
+    if (g.nblk > 0)
+      in_slices(g.nblk, kWG, [&](int64_t f, int64_t c) {
+        sorted_keys<<<c, kWG, 0, s>>>(reinterpret_cast<const int2*>(g.blk.get()), g.col, block0,
+                                      entry0, keys.get(), idx.get(), gstart.get(), f);
+      });
+    block0 += g.nblk;
+  }
+  BSP_HIP(hipGetLastError());
+  int bits = 1;
+  while ((int64_t(1) << bits) < nblk) ++bits;
+  BSP_TRY(sort_pairs_i64key(keys.get(), keys2.get(), idx.get(), idx2.get(), total, 32 + bits, s));
+  DevBuf<int32_t> col;
+  DevBuf<double> val;
+  BSP_TRY(col.alloc(total));
+  BSP_TRY(val.alloc(total));
+  BSP_TRY(p->slot.alloc(total));
+  sorted_apply<<<cdiv(total, kWG), kWG, 0, s>>>(keys2.get(), idx2.get(), gstart.get(),
+                                                p->seg_col.get(), p->seg_val.get(), total,
+                                                col.get(), val.get(), p->slot.get());
+  BSP_HIP(hipGetLastError());
+  BSP_HIP(hipStreamSynchronize(s));
+  // the blocks keep their entry ranges: segment pointers move with the arrays
+  for (auto& g : p->seg) {
+    const int64_t entry0 = g.col - p->seg_col.get();
+    g.col = col.get() + entry0;
+    g.val = val.get() + entry0;
+  }
+  p->seg_col = std::move(col);
+  p->seg_val = std::move(val);
   return BSP_OK;
 }
 
@@ -585,7 +793,7 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
     BSP_TRY(fetch(far.get(), &nfar, s));
     BSP_TRY(fetch(unsorted.get(), &uns, s));
     W = (!uns && (int64_t)nfar * 4 > A.nnz) ? kDefaultSegCols : 0;
-    if (W > 0 && p->opt.seg_mode == 0) {
+    if (W > 0 && (p->opt.seg_mode == 0 || p->opt.seg_mode == 2)) {
       // one launch, one segment per XCD group: the smallest power-of-two
       // segment count (<= 8) whose x slices are at most 2 MB
       int64_t S = 1;
@@ -679,11 +887,17 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
         break;
       }
   }
+  // sorted blocks (seg_mode 2): only for the single XCD-pinned launch
+  p->sorted = p->opt.seg_mode == 2 && (nseg == 2 || nseg == 4 || nseg == 8);
   p->nblk = 0;
   for (auto& g : p->seg) {
-    BSP_TRY(adaptive_partition(g, n, p->chunk, s));
+    if (p->sorted)
+      BSP_TRY(adaptive_partition(g, n, kSortChunk, s, kSortRowShift, kSortLong));
+    else
+      BSP_TRY(adaptive_partition(g, n, p->chunk, s));
     p->nblk += g.nblk;
   }
+  if (p->sorted) BSP_TRY(sort_blocks(p, s));
   // 16-bit column offsets, when asked for and every block allows them (its
   // kernel loads values two at a time: 16-byte aligned values needed). Not
   // the default: on the 27-point 64^3 stencil the plan reads 16% fewer bytes
@@ -719,7 +933,7 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
   // were asked for)
   p->xcd_mode = false;
   if (nseg == 2 || nseg == 4 || nseg == 8) {
-    if (p->opt.seg_mode == 0) {
+    if (p->opt.seg_mode == 0 || p->opt.seg_mode == 2) {
       p->xcd_mode = true;
       BSP_TRY(p->part.alloc(nseg * n));
       std::vector<SegDesc> tab(nseg);
@@ -727,7 +941,8 @@ static bsp_status adaptive_build(CsrPlan* p, hipStream_t s) {
       for (int64_t k = 0; k < nseg; ++k) {
         const CsrSegment& g = p->seg[k];
         tab[k] = SegDesc{g.rowptr, g.col, g.val, reinterpret_cast<const int2*>(g.blk.get()),
-                         g.nblk, p->part.get() + k * n};
+                         g.nblk, p->part.get() + k * n,
+                         p->sorted ? p->slot.get() + (g.col - p->seg_col.get()) : nullptr};
         most = std::max(most, cdiv(g.nblk, 8 / nseg));
       }
       p->xcd_grid = 8 * most;
@@ -870,7 +1085,9 @@ bsp_status csr_plan_run(const CsrPlan* p, const double* x, double* y, hipStream_
       if (p->xcd_mode) {
         const int nseg = (int)p->seg.size();
         const auto* tab = reinterpret_cast<const SegDesc*>(p->segtab.get());
-        if (p->chunk > 1024)
+        if (p->sorted)
+          csr_sorted_xcd_f64<<<p->xcd_grid, kSortWG, kSortPad, s>>>(tab, nseg, x);
+        else if (p->chunk > 1024)
           csr_adaptive_xcd_f64<2048><<<p->xcd_grid, kWG, 0, s>>>(tab, nseg, x);
         else
           csr_adaptive_xcd_f64<1024><<<p->xcd_grid, kWG, 0, s>>>(tab, nseg, x);

@@ -37,6 +37,8 @@ bsp_status check_record(const bsp_tune_record& r) {
   switch (r.format) {
     case BSP_TUNE_CSR:
       BSP_REQUIRE(r.chunk >= 128 && r.chunk <= 2048, "CSR record: chunk must be in 128..2048");
+      BSP_REQUIRE(r.kernel == 0 || r.kernel == 1,
+                  "CSR record: kernel must be 0 (row order) or 1 (sorted blocks)");
       return BSP_OK;
     case BSP_TUNE_BSR4:
       BSP_REQUIRE(r.kernel >= BSP_BSR_VALU && r.kernel <= BSP_BSR_VALU_BALANCED,
@@ -71,6 +73,7 @@ bsp_status build(const bsp_csr& A, const bsp_tune_record& r, hipStream_t s, bsp_
       bsp_csr_options o{};
       o.seg_cols = r.seg_cols;
       o.chunk = r.chunk;
+      o.seg_mode = r.kernel == 1 ? 2 : 0;
       return bsp_csr_plan_create_ex(&A, BSP_CSR_ADAPTIVE, &o, s, &h->csr);
     }
     case BSP_TUNE_BSR4:
@@ -146,8 +149,15 @@ bsp_status bsp_autotune(const bsp_csr* A, const bsp_tune_options* opt, bsp_strea
     const bsp_status st = bsp_csr_plan_segments(p, &w, &k);
     bsp_plan_destroy(p);
     BSP_TRY(st);
-    if (k > 1)
+    if (k > 1) {
       for (int c : {1024, 2048}) cands.push_back(csr_rec(c, -1));
+      // and the sorted-block segments, where they apply (2, 4 or 8 segments)
+      if (k == 2 || k == 4 || k == 8) {
+        bsp_tune_record r = csr_rec(2048, 0);
+        r.kernel = 1;
+        cands.push_back(r);
+      }
+    }
   }
   // block sizes: 4x4 and 8x8 BSR, each only where its blocks are at least
   // half full on average (sparser blocks move several times the CSR bytes)

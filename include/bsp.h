@@ -96,7 +96,11 @@ typedef struct bsp_csr_options {
   /* segmented plans: 0 = automatic (2, 4 or 8 segments run in one launch,
    * each pinned to a group of XCDs, partial results added by a second
    * kernel; automatic width then targets 2 MB of x per segment),
-   * 1 = one launch per segment, later ones accumulating into y */
+   * 1 = one launch per segment, later ones accumulating into y,
+   * 2 = as 0, with each segment cut into blocks of fewer than 6,144 entries
+   *   stored in column order (plus a 2-byte row-order slot per entry), so a
+   *   wave's x gathers share cache lines; chunk is not used. Plans that do
+   *   not segment into 2, 4 or 8 pieces run as with 0 / 1. */
   int32_t seg_mode;
   /* unsegmented plans: > 0 = use 16-bit column offsets where possible,
    * 0 (default) or < 0 = int32 indices. When every block's column indices
@@ -356,7 +360,8 @@ bsp_status bsp_dia_destroy(bsp_dia h);
  * kernels, not a cold HBM read. The choice is a plain record: store it and
  * rebuild the same layout later with bsp_tuned_from_record, no timing. */
 typedef enum bsp_tune_format {
-  BSP_TUNE_CSR = 0,  /* adaptive plan: chunk, seg_cols as in bsp_csr_options */
+  BSP_TUNE_CSR = 0,  /* adaptive plan: chunk, seg_cols as in bsp_csr_options;
+                        kernel 1 = sorted-block segments (seg_mode 2) */
   BSP_TUNE_BSR4 = 1, /* 4x4 BSR: kernel = any bsp_bsr_kernel but AUTO */
   BSP_TUNE_DBSR = 2, /* kernel = BSP_DBSR_VALU or BSP_DBSR_MFMA */
   BSP_TUNE_DASP = 3, /* threshold, block_longest as in bsp_dasp_build; kernel = bsp_dasp_kernel */

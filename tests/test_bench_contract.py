@@ -32,6 +32,10 @@ def test_roofline_reports_the_dominant_launch():
     r = bench.roofline(per, "tuned")
     assert r["kernel"] == "dia_spmv2_f64" and abs(r["achieved"] - 2000.0) < 1e-6
     assert r["other_launches"]["b"]["kernel"].startswith("csr_adaptive_xcd_f64")
+    # the sorted-block segments (CSR kernel 1)
+    per["b"]["layout"] = dict(per["b"]["layout"], kernel=1)
+    r = bench.roofline(per, "tuned")
+    assert r["other_launches"]["b"]["kernel"].startswith("csr_sorted_xcd_f64")
 
 
 def _csv(path, rows):

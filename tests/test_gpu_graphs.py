@@ -40,10 +40,11 @@ def _ops(bsp_host):
         yield f"{name}/noalloc", A, dA, (lambda d: (lambda x, y: _noalloc(d, x, y)))(dA)
         if name == "27pt-64":
             yield f"{name}/dia", A, dA, Dia(dA)
-    # forced column segments, both modes (XCD-pinned one launch; one launch per segment)
+    # forced column segments, every mode (XCD-pinned one launch; one launch per
+    # segment; XCD-pinned sorted blocks)
     A = mats["powerlaw-300k"]
     dA = DeviceCsr.from_host(A)
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         yield f"powerlaw-300k/segments-mode{mode}", A, dA, CsrSpmv(dA, "adaptive", seg_cols=40_000,
                                                                   seg_mode=mode)
 

@@ -170,6 +170,47 @@ def test_adaptive_forced_segments(oracle, bsp_host, seg_cols, seg_mode):
                       what="stencil forced segments")
 
 
+@pytest.mark.parametrize("seg_cols", [20_000, 10_000, 5_000])
+def test_sorted_blocks(oracle, seg_cols):
+    # seg_mode 2: 2, 4 or 8 XCD-pinned segments whose blocks store their
+    # entries in column order; rows of every kind: empty, short, exactly 32
+    # and 33 entries (the one-lane and the wave sums), a row long enough to
+    # take a block of its own in every segment, thousands of short rows
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    from benchsp_amd.host import Csr
+    rng = np.random.default_rng(23)
+    lens = np.array([0, 3, 0, 15_000, 1, 129, 2047, 33, 32, 31, 17] + [0] * 40 + [1] * 3000 +
+                    [40] * 500 + [5] * 20_000 + [32, 33, 64, 65])
+    n, ncols = lens.size, 40_000
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(ncols, L, replace=False)) for L in lens]).astype(np.int32)
+    A = Csr(n, ncols, rp, col, rng.uniform(-1, 1, rp[-1]))
+    x = rng.uniform(-1, 1, ncols)
+    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=seg_cols, seg_mode=2)
+    assert op.segments() == (seg_cols, ncols // seg_cols)
+    xt = torch.from_numpy(x).cuda()
+    y, y2 = op(xt), op(xt)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y2)  # deterministic
+    assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                      abs_spmv(oracle, A, x), what=f"sorted blocks {seg_cols}")
+
+
+def test_sorted_blocks_powerlaw_auto(oracle, bsp_host):
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    A = bsp_host.powerlaw_1m()
+    x = bsp_host.uniform(A.ncols, seed=13)
+    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_mode=2)
+    assert op.segments() == (1 << 18, 4)  # 2 MB slices of x, two XCDs each
+    y = op(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                      abs_spmv(oracle, A, x), what="powerlaw sorted blocks")
+
+
 def test_segment_sums_y_not_16_byte_aligned(bsp_host):
     # the partial sums of an even row count are added two rows per lane
     # (16-byte stores) only into a 16-byte aligned y; an 8-byte aligned y

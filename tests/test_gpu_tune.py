@@ -75,6 +75,26 @@ def test_autotune_format_mask(oracle, bsp_host, fmt):
                       what=f"mask {fmt}")
 
 
+def test_autotune_sorted_block_candidate(oracle, bsp_host):
+    # a matrix that segments into 4 XCD-pinned pieces also gets the sorted
+    # blocks (CSR kernel 1) as a candidate; its record rebuilds the plan
+    import torch
+    from benchsp_amd.device import DeviceCsr, Tuned
+    A = bsp_host.powerlaw_1m()
+    dA = DeviceCsr.from_host(A)
+    T = Tuned(dA, formats=["csr"], reps=2)
+    sorted_recs = [t for t in T.trials() if t["kernel"] == 1]
+    assert {t["kernel"] for t in T.trials()} == {0, 1} and len(sorted_recs) == 1
+    x = bsp_host.uniform(A.ncols, 5)
+    xd = torch.from_numpy(x).cuda()
+    R = Tuned.from_record(dA, sorted_recs[0])
+    y, y2 = R(xd), R(xd)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y2)
+    assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                      abs_spmv(oracle, A, x), what="tuned sorted blocks")
+
+
 def test_autotune_sparse_matrix_skips_bsr4(bsp_host):
     # about one entry per 4x4 block: BSR4 is screened out before any build,
     # so a BSR4-only mask leaves no candidate
