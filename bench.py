@@ -728,7 +728,7 @@ def measured_peaks() -> dict:
             "read_GBs": round(nbytes / read_s / 1e9, 1)}
 
 
-SPMV_KERNELS = ("csr_adaptive", "sum_partials", "dia_spmv")
+SPMV_KERNELS = ("csr_adaptive", "csr_sorted", "sum_partials", "dia_spmv")
 
 
 def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=None):

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_xcd_f64(const SegDesc* __res
 }
 
 // Sorted-block segmented plans (seg_mode 2). A segment's blocks are cut on
-// rowptr[r] + r/4 with windows of kSortChunk entries; a row longer than
+// rowptr[r] + r/2 with windows of kSortChunk units; a row longer than
 // kSortLong has a block of its own, so a block holds fewer than
 // kSortChunk + kSortLong = kSortCap entries. Within a block the entries are
 // stored in column order, each with a 2-byte slot: its position in row
@@ -132,12 +132,14 @@ __global__ __launch_bounds__(kWG) void csr_adaptive_xcd_f64(const SegDesc* __res
 // keeps a third from being resident (power-law 2^20, four 2 MB segments:
 // 140.9 us with two per CU, 159.4 us with three, 197 us with one; blocks of
 // 4,096 / 5,120 / 8,192 entries: 162 / 157 / 152-155 us; the row-order
-// plan 155 us).
+// plan 155 us. Rows weighing 1/2 unit in the cut: 138.9 us, against 140.6
+// with 1/4 and 138.7 with 1; rows of more than 256 / 512 / 1,536 entries
+// alone: 155.9 / 145.8 / 139.5 us).
 constexpr int kSortCap = 6144;
 constexpr int kSortWG = 512;
 constexpr int kSortLong = 1024;
 constexpr int kSortChunk = kSortCap - kSortLong;
-constexpr int kSortRowShift = 2;
+constexpr int kSortRowShift = 1;
 constexpr int kSortShort = 32;
 constexpr int kSortRows = 2;  // row bounds a lane loads with the entries
 constexpr int kSortPad = 24 * 1024;
@@ -158,9 +160,10 @@ __global__ __launch_bounds__(kSortWG) void csr_sorted_xcd_f64(const SegDesc* __r
   const int2 s0 = d.blk[b], s1 = d.blk[b + 1];
   const int r0 = s0.x, p0 = s0.y, r1 = s1.x, p1 = s1.y;
   const int tid = threadIdx.x, nnz = p1 - p0;
-  if (r1 - r0 == 1 && nnz > kSortLong) {
-    // one long row (its entries are in column order, which is row order):
-    // the workgroup strides over it
+  if (nnz >= kSortCap) {
+    // one long row (only a row longer than kSortLong has a block this
+    // large; its entries are in column order, which is row order): the
+    // workgroup strides over it
     double a0 = 0.0, a1 = 0.0;
     int i = p0 + tid;
     for (; i + kSortWG < p1; i += 2 * kSortWG) {
