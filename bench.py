@@ -452,6 +452,7 @@ def formats_leg(args, dev, mats, xs):
         cands = {
             "csr_adaptive": lambda d: CsrSpmv(d, "adaptive"),
             "csr_adaptive_narrow": lambda d: CsrSpmv(d, "adaptive", narrow_cols=1),
+            "csr_adaptive_sorted": lambda d: CsrSpmv(d, "adaptive", seg_mode=2),
             "csr_scalar": lambda d: CsrSpmv(d, "scalar"),
             "csr_balanced": lambda d: CsrSpmv(d, "balanced"),
             "csr_split": lambda d: CsrSpmv(d, "split"),
