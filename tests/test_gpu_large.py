@@ -52,6 +52,17 @@ def test_adaptive_segmented(big):
     P.close()
 
 
+def test_sorted_segments(big):
+    # eight segments stored as column-sorted blocks: over 30,000 blocks,
+    # a radix sort of every entry, the row-order slots
+    from benchsp_amd.device import CsrSpmv
+    ncols = big[1].ncols
+    P = CsrSpmv(big[1], "adaptive", seg_cols=-(-ncols // 8), seg_mode=2)
+    assert P.segments()[1] == 8
+    _check(P(big[2]), big, "adaptive, sorted-block segments")
+    P.close()
+
+
 def test_dbsr(big):
     from benchsp_amd.device import Dbsr
     D = Dbsr(big[1])

@@ -61,6 +61,7 @@ def test_random_matrix_every_method(oracle, bsp_host, seed):
     ops = {m: CsrSpmv(dA, m) for m in ("adaptive", "scalar", "balanced", "split")}
     if A.ncols > 8:
         ops["adaptive-segments"] = CsrSpmv(dA, "adaptive", seg_cols=max(1, A.ncols // 5))
+        ops["adaptive-sorted"] = CsrSpmv(dA, "adaptive", seg_cols=-(-A.ncols // 2), seg_mode=2)
     ops["dasp"] = Dasp(dA)
     ops["dbsr"] = Dbsr(dA)
     B4 = Bsr(dA, 4, 4)
