@@ -624,7 +624,8 @@ def gather_probe():
     """The power-law 2^20 SpMV's data movement without its arithmetic
     (bsp_csr_plan_probe over the same XCD-pinned column segments): x gathers
     with the column-index stream, then with the value stream too, beside the
-    SpMV launch itself (kernel + partial sums), and the same number of
+    SpMV launch itself (kernel + partial sums), each for the row-order plan
+    and for the sorted-block plan (seg_mode 2), and the same number of
     random gathers from the same 2 MB slices with no stream at all
     (bsp_gather_probe: the L2's gather rate, the ceiling of this layout), and
     whole-line 1 KB wave reads of as many 64-byte sectors from the same
@@ -641,12 +642,15 @@ def gather_probe():
     xd = torch.from_numpy(host.uniform(A.ncols, 1)).cuda()
     y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
     ops = [CsrSpmv(DeviceCsr.from_host(A), "adaptive") for _ in range(2)]
+    # the sorted-block plan: the same segments, each block's entries in
+    # column order
+    sops = [CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_mode=2) for _ in range(2)]
     g = C.c_int64()
     lib().bsp_csr_plan_probe(ops[0]._h, 0, 0, None, 0, C.byref(g), None)
     sink = torch.empty(g.value, dtype=torch.float64, device="cuda")
 
-    def probe(i, vals):
-        lib().bsp_csr_plan_probe(ops[i % 2]._h, vals, ptr(xd), ptr(sink), sink.numel(), None,
+    def probe(i, vals, plans=ops):
+        lib().bsp_csr_plan_probe(plans[i % 2]._h, vals, ptr(xd), ptr(sink), sink.numel(), None,
                                  stream_handle())
 
     # the same number of random gathers from the same four 2 MB slices of x,
@@ -675,7 +679,10 @@ def gather_probe():
             "gathers_only": ceiling,
             "gathers_col_stream": lambda i: probe(i, 0),
             "gathers_col_val_stream": lambda i: probe(i, 1),
-            "spmv": lambda i: ops[i % 2](xd, y)}
+            "spmv": lambda i: ops[i % 2](xd, y),
+            "sorted_gathers_col_stream": lambda i: probe(i, 0, sops),
+            "sorted_gathers_col_val_stream": lambda i: probe(i, 1, sops),
+            "sorted_spmv": lambda i: sops[i % 2](xd, y)}
     out = {"matrix": f"power-law 2^20, nnz={A.nnz}, {ops[0].segments()[1]} XCD-pinned segments",
            "gathers": A.nnz, "workgroups": g.value}
     for name, fn in runs.items():
