@@ -198,6 +198,30 @@ def test_sorted_blocks(oracle, seg_cols):
                       abs_spmv(oracle, A, x), what=f"sorted blocks {seg_cols}")
 
 
+def test_sorted_blocks_degenerate(oracle):
+    # no entries at all; entries in one column segment only (the other
+    # segments empty); a single row
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    from benchsp_amd.host import Csr
+    rng = np.random.default_rng(31)
+    cases = [("no entries", Csr(50, 4000, np.zeros(51, np.int64), np.zeros(0, np.int32),
+                                np.zeros(0))),
+             ("one segment", Csr(300, 4000, np.arange(0, 901, 3, dtype=np.int64),
+                                 np.tile(np.array([5, 17, 900], np.int32), 300),
+                                 rng.uniform(-1, 1, 900))),
+             ("one row", Csr(1, 4000, np.array([0, 4000], np.int64),
+                             np.arange(4000, dtype=np.int32), rng.uniform(-1, 1, 4000)))]
+    for name, A in cases:
+        x = rng.uniform(-1, 1, A.ncols)
+        op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=1000, seg_mode=2)
+        assert op.segments() == (1000, 4), name
+        y = op(torch.from_numpy(x).cuda())
+        torch.cuda.synchronize()
+        assert_spmv_close(y.cpu().numpy(), oracle.spmv_f64(A.rowptr, A.col, A.val, x),
+                          abs_spmv(oracle, A, x), what=f"sorted blocks, {name}")
+
+
 def test_sorted_blocks_powerlaw_auto(oracle, bsp_host):
     import torch
     from benchsp_amd.device import CsrSpmv, DeviceCsr
