@@ -198,6 +198,37 @@ def test_sorted_blocks(oracle, seg_cols):
                       abs_spmv(oracle, A, x), what=f"sorted blocks {seg_cols}")
 
 
+@pytest.mark.parametrize("nseg", [2, 4])
+def test_sorted_blocks_bits(oracle, nseg):
+    # rows of at most 32 entries per segment are summed left to right by one
+    # lane, and the segment sums added in segment order from 0.0: y equals,
+    # bit for bit, the oracle's row loop over each segment's columns alone,
+    # then those partial results added in order
+    import torch
+    from benchsp_amd.device import CsrSpmv, DeviceCsr
+    from benchsp_amd.host import Csr
+    rng = np.random.default_rng(41 + nseg)
+    n, ncols = 20_000, 64_000
+    lens = np.minimum(rng.poisson(12, n), 32)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(ncols, L, replace=False)) for L in lens]).astype(np.int32)
+    A = Csr(n, ncols, rp, col, rng.uniform(-1, 1, rp[-1]))
+    x = rng.uniform(-1, 1, ncols)
+    w = ncols // nseg
+    op = CsrSpmv(DeviceCsr.from_host(A), "adaptive", seg_cols=w, seg_mode=2)
+    assert op.segments() == (w, nseg)
+    y = op(torch.from_numpy(x).cuda()).cpu().numpy()
+    rows = np.repeat(np.arange(n), lens)
+    want = np.zeros(n)
+    for k in range(nseg):
+        keep = (A.col >= k * w) & (A.col < (k + 1) * w)
+        rpk = np.zeros(n + 1, np.int64)
+        rpk[1:] = np.cumsum(np.bincount(rows[keep], minlength=n))
+        want = want + oracle.spmv_f64(rpk, A.col[keep], A.val[keep], x)
+    assert np.array_equal(y, want)
+
+
 def test_sorted_blocks_degenerate(oracle):
     # no entries at all; entries in one column segment only (the other
     # segments empty); a single row
