@@ -4,8 +4,10 @@ value) and FP16 block-SpMM forward+backward GFLOP/s against a plain FP32 CSR
 SpMM (secondary leg).
 
 SpMV step: y = A*x, each matrix in the layout the library's autotuner picks
-for it (bsp_autotune, timed once before the run; `--method adaptive` keeps
-the adaptive CSR plan on both), on
+for it (bsp_autotune, timed once before the run: the diagonal layout for
+the stencil, the column-segmented CSR plan with column-sorted blocks for
+the power-law matrix; `--method adaptive` keeps the row-order adaptive CSR
+plan on both), on
   (a) the 27-point stencil on a 64^3 grid (n=262,144, nnz=6,859,000) and
   (b) the power-law matrix (n=2^20, lognormal row lengths mean 20, sigma 1.2,
       cap 5000, ~21M nnz),
@@ -28,7 +30,7 @@ forward's error against a dense FP64 product is reported.
 
 Formats leg: the same two matrices through every SpMV format and method of
 the library (scalar / balanced / split / adaptive CSR, adaptive CSR with
-16-bit column offsets, 4x4 BSR on MFMA and VALU per block row and balanced
+16-bit column offsets or column-sorted segment blocks, 4x4 BSR on MFMA and VALU per block row and balanced
 over 64-block chunks, DBSR and DASP on VALU and on FP64 MFMA, the diagonal
 layout where the matrix has at most 32 diagonals), the layout the
 per-matrix autotuner picks, and rocSPARSE with each of its CSR algorithms
